@@ -1,0 +1,166 @@
+#!/usr/bin/env python3
+"""Benchmark of the PHOENIX RK4 time step on one MI355X.
+
+Workload (synthetic data, shapes of the 1024^2 workload): scalar GP + reservoir,
+1024 x 1024 cells, dx = 0.5 um (L = 511.5 um), periodic boundaries, dt = 0.04 ps,
+ring pump P = 1.5 P_th exp(-(r-40)^2 / (2*5^2)), psi0 = 0.01 complex Gaussian
+noise (seed 4242), n0 = 0.  One "step" = one RK4 step over the whole grid.
+The k-space filter of that workload (every 50 steps) is timed separately when
+available and is not part of the per-step figure.
+
+Prints ONE JSON line (see the contract in the repository documentation).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "µs/RK4 step at 1024² fp32/fp64 + % HBM roofline; batched & adjoint vs plain"
+PTH = 0.15 * 0.225 / 0.01
+HBM_PEAK_GBS = 8000.0
+
+
+def workload_inputs(N: int, L: float, seed: int = 4242):
+    dx = L / (N - 1)
+    x = -L / 2 + dx * np.arange(N)
+    X, Y = np.meshgrid(x, x)
+    r = np.sqrt(X ** 2 + Y ** 2)
+    P = 1.5 * PTH * np.exp(-((r - 40.0) ** 2) / (2 * 5.0 ** 2))
+    rng = np.random.default_rng(seed)
+    psi = 0.01 * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))
+    n = np.zeros((N, N))
+    return psi, n, P
+
+
+def solver_for(prec: str, N: int, L: float):
+    from phoenix_amd._lib import Solver
+    args = ["--N", N, N, "--L", L, L, "--boundary", "periodic", "periodic", "--tstep", 0.04, "--tmax", 1e9,
+            "--fftEvery", 1e12, "--outEvery", 1e12,
+            "--pump", 1.0, "add", 10, 10, 0, 0, "both", 1, "none", "gauss+noDivide"]
+    s = Solver([str(a) for a in args], prec)
+    psi, n, P = workload_inputs(N, L)
+    s.set("psi_plus", psi)
+    s.set("n_plus", n)
+    s.set("pump_plus", P)
+    return s
+
+
+def time_rk4(prec: str, N: int, L: float, steps: int, warmup: int, kernel_samples: int = 200):
+    import torch
+    s = solver_for(prec, N, L)
+    stream = torch.cuda.Stream()
+    s.set_stream(stream.cuda_stream)
+    s.step(warmup)
+    s.sync()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    done = s.step(steps)
+    e1.record(stream)
+    s.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    total_ms = e0.elapsed_time(e1)
+    assert done == steps
+    # per-launch duration of the fused kernel: one event pair around each single-kernel step
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(kernel_samples)]
+    for a, b in evs:
+        a.record(stream)
+        s.step(1)
+        b.record(stream)
+    s.sync()
+    torch.cuda.synchronize()
+    k_ms = [a.elapsed_time(b) for a, b in evs]
+    psi = s.get("psi_plus")
+    finite = bool(np.isfinite(psi).all())
+    s.close()
+    return dict(total_ms=total_ms, wall_s=wall, us_per_step=1e3 * total_ms / steps,
+                kernel_us_mean=1e3 * float(np.mean(k_ms)), kernel_us_median=1e3 * float(np.median(k_ms)),
+                finite=finite)
+
+
+def cpu_baseline(N: int, L: float, steps: int):
+    """The FP64 oracle (single-threaded C restatement of the reference CPU path) on a bounded sample."""
+    from oracle import oracle as O
+    p = O.params(N, N, L, L, periodic=(True, True), pump=True)
+    psi, n, P = workload_inputs(N, L)
+    O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=1)  # warm-up (page-in)
+    t0 = time.perf_counter()
+    O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=steps)
+    dt = time.perf_counter() - t0
+    return dict(value=1e6 * dt / steps, unit="µs/step", cores=1, kind="port",
+                sample=f"FP64 oracle (C, 1 thread, reference subgrid+halo structure), {N}x{N}, {steps} RK4 steps, "
+                       f"{dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--size", type=int, default=1024)
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64"], default=None, help="time one precision only (profiling)")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); the path shards by halo exchange, "
+              "which is designed but not run here.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    assert torch.cuda.is_available(), "no HIP device visible"
+    N = a.size
+    L = 0.5 * (N - 1)
+    cells = N * N
+    res = {}
+    for prec in (["fp32", "fp64"] if a.only is None else [a.only]):
+        res[prec] = time_rk4(prec, N, L, a.steps, a.warmup)
+    main_prec = "fp32" if "fp32" in res else "fp64"
+    r = res[main_prec]
+    bytes_per_cell = {"fp32": 28, "fp64": 56}
+    kernel_s = r["kernel_us_mean"] * 1e-6
+    achieved = bytes_per_cell[main_prec] * cells / kernel_s / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(r["us_per_step"], 4),
+        "unit": "µs/step",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": r["us_per_step"] / 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": main_prec,
+        "data": "synthetic",
+        "config": {"workload": f"scalar GP+reservoir RK4, {N}x{N}, dx=0.5um, periodic, ring pump 1.5 P_th, dt=0.04ps",
+                   "cells": cells, "parallelism": "single GPU"},
+        "gcell_steps_per_s": round(cells / (r["us_per_step"] * 1e-6) / 1e9, 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "rk4_scalar_fused", "kernel_us": round(r["kernel_us_mean"], 3),
+                     "bytes_per_cell": bytes_per_cell[main_prec]},
+        "precisions": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                       for k, v in res.items()},
+    }
+    for k, v in res.items():
+        out["precisions"][k]["roofline_frac_kernel"] = round(
+            bytes_per_cell[k] * cells / (v["kernel_us_mean"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+        out["precisions"][k]["roofline_frac_step"] = round(
+            bytes_per_cell[k] * cells / (v["us_per_step"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(N, L, a.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,163 @@
+/*
+ * phx.h -- C ABI of the MI355X-native PHOENIX time-stepping library (libphx.so).
+ *
+ * The reference exposes this path through three surfaces, none of them a C ABI:
+ * the PHOENIX_{cpu,gpu}_{fp32,fp64} executables and their flags
+ * (source/system/system_initialization.cpp:13-261, source/system/config_readin.cpp:7-42),
+ * the internal C++ Solver interface (include/solver/solver.hpp:141-160 with
+ * SolverFactory::create, include/solver/solver_factory.hpp:53-84), and pyphoenix,
+ * which spawns the executable (pyphoenix/pyphoenix.py:162-215).  Each entry point
+ * below names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Status: 0 = PHX_OK, negative = error; phx_last_error() (thread local) holds
+ *     the message.  The library never calls exit(); the CLI maps errors to exit(1)
+ *     (system_validate_input.cpp:53-56).
+ *   - The solver owns its device memory; callers own host buffers.
+ *   - A handle is not thread safe.  Each handle owns one HIP stream (or uses the
+ *     one given by phx_set_stream).
+ *   - Host field layout: full grid, row-major N_r x N_c, index row*N_c + col.
+ *     Complex fields (psi, pulse) are interleaved {re, im}; real fields (n, P, V,
+ *     FFT mask) are plain.  Element type is float for PHX_FP32, double for PHX_FP64.
+ */
+#ifndef PHX_H
+#define PHX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct phx_solver phx_solver;
+
+enum phx_precision { PHX_FP32 = 0, PHX_FP64 = 1 };
+
+enum phx_status {
+    PHX_OK = 0,
+    PHX_ERR_INVALID = -1,      /* invalid input (system_validate_input.cpp:19-56)      */
+    PHX_ERR_HIP = -2,          /* HIP runtime failure                                   */
+    PHX_ERR_UNSUPPORTED = -3,  /* flag / iterator outside this library's path           */
+    PHX_ERR_IO = -4,           /* file could not be read                                */
+    PHX_ERR_STEP = -5,         /* adaptive step rejected more than the allowed count    */
+    PHX_ERR_STATE = -6         /* call out of order (e.g. step before initialize)       */
+};
+
+enum phx_field {
+    PHX_PSI_PLUS = 0,
+    PHX_PSI_MINUS = 1,
+    PHX_N_PLUS = 2,
+    PHX_N_MINUS = 3,
+    PHX_PUMP_PLUS = 4,
+    PHX_PUMP_MINUS = 5,
+    PHX_POTENTIAL_PLUS = 6,
+    PHX_POTENTIAL_MINUS = 7,
+    PHX_FFT_MASK_PLUS = 8,
+    PHX_FFT_MASK_MINUS = 9
+};
+
+/* Summary of a configured solver (read-only view of SystemParameters,
+   include/system/system_parameters.hpp:25-64). */
+typedef struct {
+    int32_t N_c, N_r;
+    int32_t precision;          /* phx_precision */
+    int32_t tetm;               /* -tetm */
+    int32_t use_reservoir, use_pump, use_potential, use_stochastic, use_fft_mask;
+    int32_t adaptive;           /* -adaptive with an adaptive iterator */
+    int32_t periodic_x, periodic_y;
+    int32_t halo;               /* k_max of the iterator: 4 (RK4), 7 (DP45) */
+    char iterator[16];          /* "RK4" or "DP45" */
+    double dx, dy, dt, t, t_max, fft_every, tolerance, dt_min, dt_max;
+    int64_t iteration;          /* accepted steps so far */
+    int64_t rejected;           /* rejected DP45 attempts so far */
+} phx_info;
+
+/* Parse PHOENIX flags (argv[0] is the program name and is skipped), including
+   --config files, derive constants and validate.  No device work.
+   Replaces: SystemParameters(argc, argv) (system.cpp:162-184) +
+   readConfigFromFile (config_readin.cpp:7-42) + SolverFactory::create
+   (solver_factory.hpp:53-60; unknown --iterator -> PHX_ERR_UNSUPPORTED). */
+int phx_create(phx_solver** out, int precision, int argc, const char* const* argv);
+
+/* Allocate device fields, evaluate envelopes (pump, potential, initial state,
+   initial reservoir, FFT mask) and upload them.
+   Replaces: Solver::initialize / initializeMatricesFromSystem
+   (solver_initialization.cu:13-174). */
+int phx_initialize(phx_solver* s);
+
+/* Run up to n_steps iterate() calls; stops early when t >= t_max.
+   *n_done receives the number of accepted steps taken.
+   Replaces: Solver::iterate (solver.cu:22-70) in the main loop (main.cu:80-117). */
+int phx_step(phx_solver* s, int64_t n_steps, int64_t* n_done);
+
+/* Iterate until t >= t_max.  Replaces the main loop of main.cu:80-117. */
+int phx_run(phx_solver* s, int64_t* n_done);
+
+/* Apply the k-space filter now (forward FFT, mask/N, inverse FFT on psi).
+   Replaces: Solver::applyFFTFilter(true) (solver_fft.cu:67-100). */
+int phx_apply_fft_filter(phx_solver* s);
+
+/* Copy a field to / from host memory (full grid, layout above).  bytes must equal
+   N_c*N_r*elements*sizeof(real).  Replaces the host mirror of CUDAMatrix
+   (cuda_matrix.cuh:281-306,375-434) used by the reference's output path. */
+int phx_get_field(phx_solver* s, int field, void* host, size_t bytes);
+int phx_set_field(phx_solver* s, int field, const void* host, size_t bytes);
+
+/* Device pointer of a field (current buffer; valid until the next step), for
+   zero-copy interop (torch tensors over the same HIP device).  No reference
+   counterpart: the reference keeps device pointers internal (getDevicePtr,
+   cuda_matrix.cuh:445-453). */
+int phx_device_ptr(phx_solver* s, int field, void** dptr);
+
+/* Time bookkeeping: t and dt of SystemParameters::p (system_parameters.hpp:33). */
+int phx_get_time(phx_solver* s, double* t, double* dt);
+int phx_set_time(phx_solver* s, double t, double dt);
+
+int phx_get_info(phx_solver* s, phx_info* info);
+
+/* Use an external HIP stream (hipStream_t passed as void*) for all work of this
+   handle; NULL restores the handle's own stream. */
+int phx_set_stream(phx_solver* s, void* stream);
+
+/* Wait for all device work of the handle. */
+int phx_sync(phx_solver* s);
+
+void phx_destroy(phx_solver* s);
+
+/* Thread-local message of the last failing call ("" if none). */
+const char* phx_last_error(void);
+
+/* Library version string. */
+const char* phx_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Device-pointer entry points (no handle).  Fields are device pointers in the
+ * layout above; the caller owns them.  Used by the batched-ensemble and adjoint
+ * forms and by the benchmark.  `stream` is a hipStream_t (NULL = default).
+ * ------------------------------------------------------------------------- */
+
+/* Physical constants for the device entry points, as derived by
+   SystemParameters::calculateAuto (system.cpp:116-160). */
+typedef struct {
+    double dx, dy, dV;
+    double m_eff_scaled, one_over_h_bar_s;
+    double gamma_c, gamma_r, g_c, g_r, R, g_pm, delta_LT;
+} phx_physics;
+
+/* Fill phx_physics from the reference defaults (system.cpp:18-72) for a grid
+   of N_c x N_r cells over L_x x L_y micrometres; returns the magic dt. */
+double phx_physics_defaults(phx_physics* ph, int N_c, int N_r, double L_x, double L_y);
+
+/* `n_steps` fixed RK4 steps of the scalar GP + reservoir model on one grid.
+   psi_a/n_a hold the state and receive the result; psi_b/n_b are scratch of
+   the same size (double buffering).  pump / pot may be NULL.  Replaces one
+   RungeKutta4::step per step (runge_kutta.cu:74-94). */
+int phx_rk4_scalar_device(int precision, const phx_physics* ph, int N_c, int N_r, int periodic_x, int periodic_y,
+                          void* psi_a, void* n_a, void* psi_b, void* n_b, const void* pump, const void* pot,
+                          double dt, int64_t n_steps, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHX_H */

@@ -1,0 +1,226 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/libphx_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the parity checker.  The product package
+(phoenix_amd) never imports this module.
+
+Arrays use the product layout: full grid, row-major (N_r, N_c); psi as complex128,
+n / P / V as float64.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libphx_oracle.so")
+
+
+class OrcParams(C.Structure):
+    _fields_ = [
+        ("N_c", C.c_int), ("N_r", C.c_int),
+        ("subgrids_columns", C.c_int), ("subgrids_rows", C.c_int),
+        ("periodic_x", C.c_int), ("periodic_y", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dV", C.c_double),
+        ("one_over_dx2", C.c_double), ("one_over_dy2", C.c_double), ("m2_over_dx2_p_dy2", C.c_double),
+        ("m_eff_scaled", C.c_double), ("one_over_h_bar_s", C.c_double), ("h_bar_s", C.c_double), ("m_eff", C.c_double),
+        ("gamma_c", C.c_double), ("gamma_r", C.c_double), ("g_c", C.c_double), ("g_r", C.c_double),
+        ("R", C.c_double), ("g_pm", C.c_double), ("delta_LT", C.c_double),
+        ("tetm", C.c_int), ("use_reservoir", C.c_int), ("use_pump", C.c_int), ("use_potential", C.c_int),
+        ("use_stochastic", C.c_int),
+        ("stochastic_amplitude", C.c_double),
+        ("pump_amp", C.c_double), ("pump_amp_next", C.c_double), ("pot_amp", C.c_double), ("pot_amp_next", C.c_double),
+        ("noise_seed", C.c_uint64),
+    ]
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (gcc); returns the library path."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        P = C.POINTER(OrcParams)
+        L.orc_defaults.argtypes = [P]
+        L.orc_derive.argtypes = [P] + [C.c_double] * 7
+        L.orc_derive.restype = C.c_double
+        L.orc_rk4_run.argtypes = [P, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_int, C.c_int64]
+        L.orc_dp45_attempt.argtypes = [P, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int64, dp, dp]
+        L.orc_adapt_dt.argtypes = [C.c_double] * 6 + [dp, dp]
+        L.orc_dp45_run.argtypes = [P, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double,
+                                   C.c_double, C.c_int, C.POINTER(C.c_int64)]
+        L.orc_dp45_run.restype = C.c_int64
+        L.orc_fft_filter.argtypes = [dp, dp, C.c_int, C.c_int]
+        L.orc_fft2d.argtypes = [dp, C.c_int, C.c_int, C.c_int]
+        L.orc_envelope_add.argtypes = [dp, C.c_int, C.c_int] + [C.c_double] * 10 + [C.c_int] + [C.c_double] * 3 + [C.c_int]
+        L.orc_noise_normal2.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, dp, dp]
+        L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.orc_gaussian_field.argtypes = [dp, C.c_int, C.c_double, C.c_uint64]
+        _lib = L
+    return _lib
+
+
+# reference defaults: system.cpp:20-23 (SI constants)
+M_E = 9.10938356e-31
+H_BAR = 1.0545718e-34
+E_E = 1.60217662e-19
+
+
+def params(N_c, N_r, L_x, L_y, *, periodic=(True, True), subgrids=(1, 1), tetm=False, reservoir=True,
+           pump=False, potential=False, stochastic=0.0, noise_seed=0, **phys) -> OrcParams:
+    p = OrcParams()
+    lib().orc_defaults(C.byref(p))
+    p.N_c, p.N_r = N_c, N_r
+    p.subgrids_columns, p.subgrids_rows = subgrids
+    p.periodic_x, p.periodic_y = int(periodic[0]), int(periodic[1])
+    for k, v in phys.items():
+        setattr(p, k, v)
+    h_bar_s = phys.get("h_bar_s", -1.0)
+    m_eff = phys.get("m_eff", -1.0)
+    lib().orc_derive(C.byref(p), L_x, L_y, H_BAR, E_E, M_E, h_bar_s, m_eff)
+    p.tetm = int(tetm)
+    p.use_reservoir = int(reservoir)
+    p.use_pump = int(pump)
+    p.use_potential = int(potential)
+    p.use_stochastic = int(stochastic > 0)
+    p.stochastic_amplitude = stochastic
+    p.noise_seed = noise_seed
+    return p
+
+
+def _dp(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _c128(a):
+    return np.ascontiguousarray(a, dtype=np.complex128).copy()
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64).copy()
+
+
+def rk4_run(p: OrcParams, psi, n=None, pump=None, pot=None, dt=0.04, steps=1, psi_minus=None, n_minus=None,
+            pump_minus=None, pot_minus=None, step0=0):
+    psi = _c128(psi)
+    pm = _c128(psi_minus) if psi_minus is not None else None
+    n = _f64(n) if n is not None else np.zeros(psi.shape)
+    nm = _f64(n_minus) if n_minus is not None else (np.zeros(psi.shape) if psi_minus is not None else None)
+    pump = _f64(pump)
+    pot = _f64(pot)
+    rc = lib().orc_rk4_run(C.byref(p), _dp(psi.view(np.float64)), _dp(pm.view(np.float64)) if pm is not None else None,
+                           _dp(n), _dp(nm), _dp(pump), _dp(_f64(pump_minus)), _dp(pot), _dp(_f64(pot_minus)),
+                           dt, steps, step0)
+    if rc:
+        raise RuntimeError(f"orc_rk4_run failed: {rc}")
+    if psi_minus is not None:
+        return psi, n, pm, nm
+    return psi, n
+
+
+def dp45_attempt(p, psi, n, pump=None, pot=None, dt=0.04, noise_dt=None, step=0, psi_minus=None, n_minus=None,
+                 pump_minus=None, pot_minus=None):
+    psi = _c128(psi)
+    n = _f64(n)
+    pm = _c128(psi_minus) if psi_minus is not None else None
+    nm = _f64(n_minus)
+    es, nr = C.c_double(), C.c_double()
+    rc = lib().orc_dp45_attempt(C.byref(p), _dp(psi.view(np.float64)), _dp(pm.view(np.float64)) if pm is not None else None,
+                                _dp(n), _dp(nm), _dp(_f64(pump)), _dp(_f64(pump_minus)), _dp(_f64(pot)),
+                                _dp(_f64(pot_minus)), dt, dt if noise_dt is None else noise_dt, step,
+                                C.byref(es), C.byref(nr))
+    if rc:
+        raise RuntimeError("orc_dp45_attempt failed")
+    out = (psi, n, es.value, nr.value)
+    if psi_minus is not None:
+        out = out + (pm, nm)
+    return out
+
+
+def adapt_dt(err_sum, norm, tol, dt, dt_min, dt_max):
+    nd, fe = C.c_double(), C.c_double()
+    acc = lib().orc_adapt_dt(err_sum, norm, tol, dt, dt_min, dt_max, C.byref(nd), C.byref(fe))
+    return bool(acc), nd.value, fe.value
+
+
+def dp45_run(p, psi, n, pump=None, pot=None, t=0.0, dt=0.04, t_max=1.0, tol=1e-6, dt_min=1e-4, dt_max=0.1,
+             max_rejections=1000, psi_minus=None, n_minus=None, pump_minus=None, pot_minus=None):
+    psi = _c128(psi)
+    n = _f64(n)
+    pm = _c128(psi_minus) if psi_minus is not None else None
+    nm = _f64(n_minus)
+    tt, dd = C.c_double(t), C.c_double(dt)
+    nrej = C.c_int64()
+    acc = lib().orc_dp45_run(C.byref(p), _dp(psi.view(np.float64)), _dp(pm.view(np.float64)) if pm is not None else None,
+                             _dp(n), _dp(nm), _dp(_f64(pump)), _dp(_f64(pump_minus)), _dp(_f64(pot)),
+                             _dp(_f64(pot_minus)), C.byref(tt), C.byref(dd), t_max, tol, dt_min, dt_max,
+                             max_rejections, C.byref(nrej))
+    if acc < 0:
+        raise RuntimeError(f"orc_dp45_run failed: {acc}")
+    res = dict(psi=psi, n=n, t=tt.value, dt=dd.value, accepted=int(acc), rejected=int(nrej.value))
+    if pm is not None:
+        res.update(psi_minus=pm, n_minus=nm)
+    return res
+
+
+def fft_filter(psi, mask):
+    psi = _c128(psi)
+    mask = _f64(mask)
+    Nr, Nc = psi.shape
+    if lib().orc_fft_filter(_dp(psi.view(np.float64)), _dp(mask), Nc, Nr):
+        raise RuntimeError("orc_fft_filter failed")
+    return psi
+
+
+def fft2d(psi, sign=-1):
+    psi = _c128(psi)
+    Nr, Nc = psi.shape
+    lib().orc_fft2d(_dp(psi.view(np.float64)), Nc, Nr, sign)
+    return psi
+
+
+ENV_GAUSS, ENV_OUTER, ENV_RING, ENV_NODIVIDE, ENV_LOCAL = 1, 2, 4, 8, 16
+BEH_ADD, BEH_MULTIPLY, BEH_REPLACE, BEH_ADAPTIVE, BEH_COMPLEX = 1, 2, 4, 8, 16
+
+
+def envelope(N_c, N_r, L_x, L_y, dx, dy, amp, width_x, width_y, pos_x, pos_y, exponent, type_flags,
+             charge=0.0, k0=(0.0, 0.0), behaviour=1, out=None):
+    out = np.zeros((N_r, N_c), np.complex128) if out is None else _c128(out)
+    lib().orc_envelope_add(_dp(out.view(np.float64)), N_c, N_r, L_x, L_y, dx, dy, amp, width_x, width_y, pos_x,
+                           pos_y, exponent, type_flags, charge, k0[0], k0[1], behaviour)
+    return out
+
+
+def philox(ctr, key):
+    c = (C.c_uint32 * 4)(*ctr)
+    k = (C.c_uint32 * 2)(*key)
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return list(o)
+
+
+def noise_normal2(seed, step, cell, component=0):
+    re, im = C.c_double(), C.c_double()
+    lib().orc_noise_normal2(seed, step, cell, component, C.byref(re), C.byref(im))
+    return complex(re.value, im.value)
+
+
+def gaussian_field(N_c, N_r, amplitude, seed):
+    out = np.zeros((N_r, N_c), np.complex128)
+    lib().orc_gaussian_field(_dp(out.view(np.float64)), N_c * N_r, amplitude, seed)
+    return out

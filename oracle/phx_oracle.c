@@ -1,0 +1,754 @@
+/*
+ * phx_oracle.c -- FP64 CPU restatement of PHOENIX's RK time step.
+ *
+ * TEST INFRASTRUCTURE ONLY (parity checker).  Never linked into or called by
+ * the product library.  Build: oracle/Makefile (gcc -std=c11 -O2
+ * -ffp-contract=off, as the reference CPU build rounds without contraction).
+ *
+ * Every function cites the reference unit it restates.  The structure is the
+ * reference CPU path: padded subgrids with halo = k_max, one halo sync per
+ * step, stage s on extent (halo - s), full-region final sum.
+ */
+#include "phx_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <float.h>
+
+/* ------------------------------------------------------------------------ */
+/* parameters                                                               */
+/* ------------------------------------------------------------------------ */
+
+/* SystemParameters::SystemParameters() defaults (system.cpp:18-72). */
+void orc_defaults(orc_params* p) {
+    memset(p, 0, sizeof(*p));
+    p->gamma_c = 0.15;
+    p->gamma_r = 1.5 * 0.15;
+    p->g_c = 3.e-6;
+    p->g_r = 2. * 3.e-6;
+    p->R = 0.01;
+    p->g_pm = -3.e-6 / 5;
+    p->delta_LT = 0.025e-3;
+    p->N_c = p->N_r = 400;
+    p->subgrids_columns = p->subgrids_rows = 1;
+    p->pump_amp = p->pump_amp_next = 1.0;
+    p->pot_amp = p->pot_amp_next = 1.0;
+}
+
+/* SystemParameters::calculateAuto (system.cpp:116-160). */
+double orc_derive(orc_params* p, double L_x, double L_y, double h_bar, double e_e, double m_e, double h_bar_s, double m_eff) {
+    if (h_bar_s < 0) h_bar_s = h_bar / e_e * 1e12;
+    if (m_eff < 0) m_eff = m_e / e_e * 1e8;
+    const double dt_scaling_factor = m_e / e_e * 1e8 / m_eff;
+    p->dx = L_x / (p->N_c - 1);
+    p->dy = L_y / (p->N_r - 1);
+    p->dV = p->dx * p->dy;
+    p->one_over_dx2 = 1.0 / (p->dx * p->dx);
+    p->one_over_dy2 = 1.0 / (p->dy * p->dy);
+    p->m2_over_dx2_p_dy2 = -2.0 * (p->one_over_dx2 + p->one_over_dy2);
+    p->h_bar_s = h_bar_s;
+    p->m_eff = m_eff;
+    p->m_eff_scaled = -0.5 * h_bar_s * h_bar_s / m_eff;
+    p->one_over_h_bar_s = 1.0 / h_bar_s;
+    return 0.5 * p->dx * p->dy / dt_scaling_factor;
+}
+
+/* ------------------------------------------------------------------------ */
+/* noise: Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) + Box-Muller    */
+/* ------------------------------------------------------------------------ */
+void orc_philox4x32_10(uint32_t ctr[4], const uint32_t key_in[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key_in[0], k1 = key_in[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static void box_muller(uint32_t a, uint32_t b, double* z0, double* z1) {
+    const double u1 = ((double)a + 0.5) * (1.0 / 4294967296.0);
+    const double u2 = ((double)b + 0.5) * (1.0 / 4294967296.0);
+    const double r = sqrt(-2.0 * log(u1));
+    const double ang = 6.283185307179586476925286766559 * u2;
+    *z0 = r * cos(ang);
+    *z1 = r * sin(ang);
+}
+
+/* counter = {cell, step lo, step hi, component}, key = seed (64 bit). */
+void orc_noise_normal2(uint64_t seed, uint64_t step, uint32_t cell, int component, double* re, double* im) {
+    uint32_t ctr[4] = {cell, (uint32_t)step, (uint32_t)(step >> 32), (uint32_t)component};
+    uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+    uint32_t out[4];
+    orc_philox4x32_10(ctr, key, out);
+    box_muller(out[0], out[1], re, im);
+}
+
+void orc_gaussian_field(double* psi, int N, double amplitude, uint64_t seed) {
+    for (int i = 0; i < N; i++) {
+        double re, im;
+        orc_noise_normal2(seed, 0xFFFFFFFFull, (uint32_t)i, 0x7fffffff, &re, &im);
+        psi[2 * i] = amplitude * re;
+        psi[2 * i + 1] = amplitude * im;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* subgridded workspace (cuda_matrix.cuh:193-254, matrix_container.cuh)     */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    const orc_params* p;
+    int h, sgc, sgr, ro, N2h, S, kmax, ncomp;
+    double *psi[2], *bpsi[2], *kpsi[2];     /* complex: 2 doubles per cell */
+    double *n[2], *bn[2], *kn[2];           /* real */
+    double *pump[2], *pot[2];               /* real */
+    double *xi;                             /* complex noise per padded cell, per subgrid */
+    double dt;
+} ws_t;
+
+static void* xcalloc(size_t n, size_t s) { return calloc(n ? n : 1, s); }
+
+static int ws_init(ws_t* w, const orc_params* p, int kmax) {
+    memset(w, 0, sizeof(*w));
+    w->p = p;
+    w->h = kmax;
+    w->kmax = kmax;
+    if (p->N_c % p->subgrids_columns || p->N_r % p->subgrids_rows) return -1;
+    w->sgc = p->N_c / p->subgrids_columns;
+    w->sgr = p->N_r / p->subgrids_rows;
+    /* the halo map copies h rows/columns of the neighbour's interior: needs subgrid edge >= halo */
+    if (w->sgc < kmax || w->sgr < kmax) return -1;
+    w->ro = w->sgc + 2 * w->h;
+    w->N2h = w->ro * (w->sgr + 2 * w->h);
+    w->S = p->subgrids_columns * p->subgrids_rows;
+    w->ncomp = p->tetm ? 2 : 1;
+    size_t cells = (size_t)w->S * w->N2h;
+    for (int c = 0; c < w->ncomp; c++) {
+        w->psi[c] = xcalloc(2 * cells, sizeof(double));
+        w->bpsi[c] = xcalloc(2 * cells, sizeof(double));
+        w->kpsi[c] = xcalloc(2 * cells * kmax, sizeof(double));
+        w->n[c] = xcalloc(cells, sizeof(double));
+        w->bn[c] = xcalloc(cells, sizeof(double));
+        w->kn[c] = xcalloc(cells * kmax, sizeof(double));
+        w->pump[c] = xcalloc(cells, sizeof(double));
+        w->pot[c] = xcalloc(cells, sizeof(double));
+    }
+    w->xi = xcalloc(2 * cells, sizeof(double));
+    return 0;
+}
+
+static void ws_free(ws_t* w) {
+    for (int c = 0; c < w->ncomp; c++) {
+        free(w->psi[c]); free(w->bpsi[c]); free(w->kpsi[c]);
+        free(w->n[c]); free(w->bn[c]); free(w->kn[c]);
+        free(w->pump[c]); free(w->pot[c]);
+    }
+    free(w->xi);
+}
+
+/* full grid -> padded subgrid interiors (kernel_halo.cuh:7-24) */
+static void to_sub(const ws_t* w, double* sub, const double* full, int elem) {
+    const orc_params* p = w->p;
+    for (int r = 0; r < p->N_r; r++)
+        for (int c = 0; c < p->N_c; c++) {
+            int g = (r / w->sgr) * p->subgrids_columns + c / w->sgc;
+            size_t li = (size_t)g * w->N2h + (size_t)(w->h + r % w->sgr) * w->ro + (w->h + c % w->sgc);
+            for (int e = 0; e < elem; e++) sub[li * elem + e] = full[((size_t)r * p->N_c + c) * elem + e];
+        }
+}
+
+/* padded subgrid interiors -> full grid (kernel_halo.cuh:26-43) */
+static void to_full(const ws_t* w, const double* sub, double* full, int elem) {
+    const orc_params* p = w->p;
+    for (int r = 0; r < p->N_r; r++)
+        for (int c = 0; c < p->N_c; c++) {
+            int g = (r / w->sgr) * p->subgrids_columns + c / w->sgc;
+            size_t li = (size_t)g * w->N2h + (size_t)(w->h + r % w->sgr) * w->ro + (w->h + c % w->sgc);
+            for (int e = 0; e < elem; e++) full[((size_t)r * p->N_c + c) * elem + e] = sub[li * elem + e];
+        }
+}
+
+/* Global cell behind a padded position; returns -1 outside a zero-BC domain. */
+static long global_cell(const ws_t* w, int g, int tr, int tc) {
+    const orc_params* p = w->p;
+    int R = g / p->subgrids_columns, C = g % p->subgrids_columns;
+    long row = (long)R * w->sgr + tr - w->h;
+    long col = (long)C * w->sgc + tc - w->h;
+    if (row < 0 || row >= p->N_r) {
+        if (!p->periodic_y) return -1;
+        row = (row % p->N_r + p->N_r) % p->N_r;
+    }
+    if (col < 0 || col >= p->N_c) {
+        if (!p->periodic_x) return -1;
+        col = (col % p->N_c + p->N_c) % p->N_c;
+    }
+    return row * p->N_c + col;
+}
+
+/* Halo synchronisation (kernel_halo.cuh:50-76 with the map of
+   solver_initialization.cu:181-220): every halo cell copies the interior cell
+   of the neighbouring subgrid in direction (dr,dc); wraps when periodic,
+   otherwise writes zero.  Neighbour indices wrap modulo the subgrid count. */
+static void sync_halos(const ws_t* w, double* f, int elem) {
+    const orc_params* p = w->p;
+    const int h = w->h, sgc = w->sgc, sgr = w->sgr;
+    for (int g = 0; g < w->S; g++) {
+        int R = g / p->subgrids_columns, C = g % p->subgrids_columns;
+        for (int tr = 0; tr < sgr + 2 * h; tr++)
+            for (int tc = 0; tc < sgc + 2 * h; tc++) {
+                int dr = tr < h ? -1 : (tr >= h + sgr ? 1 : 0);
+                int dc = tc < h ? -1 : (tc >= h + sgc ? 1 : 0);
+                if (!dr && !dc) continue;
+                size_t to = (size_t)g * w->N2h + (size_t)tr * w->ro + tc;
+                int Rn = R + dr, Cn = C + dc;
+                if ((!p->periodic_x && (Cn < 0 || Cn >= p->subgrids_columns)) || (!p->periodic_y && (Rn < 0 || Rn >= p->subgrids_rows))) {
+                    for (int e = 0; e < elem; e++) f[to * elem + e] = 0.0;
+                    continue;
+                }
+                Rn = (Rn + p->subgrids_rows) % p->subgrids_rows;
+                Cn = (Cn + p->subgrids_columns) % p->subgrids_columns;
+                int gn = Rn * p->subgrids_columns + Cn;
+                size_t from = (size_t)gn * w->N2h + (size_t)(tr - dr * sgr) * w->ro + (tc - dc * sgc);
+                for (int e = 0; e < elem; e++) f[to * elem + e] = f[from * elem + e];
+            }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* RHS: Kernel::Compute::gp_scalar (kernel_gp_compute.cuh:18-236)           */
+/* ------------------------------------------------------------------------ */
+/* -i*z (typedef.cuh:164-166) */
+#define MI_RE(re, im) (im)
+#define MI_IM(re, im) (-(re))
+
+static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
+    const orc_params* p = w->p;
+    const int ch = w->h - index;
+    const int ro = w->ro;
+    const double ohb = p->one_over_h_bar_s;
+    const double pamp = p->pump_amp * (1.0 - wt) + wt * p->pump_amp_next;
+    const double vamp = p->pot_amp * (1.0 - wt) + wt * p->pot_amp_next;
+    const int stoch = p->use_stochastic && p->stochastic_amplitude > 0.0;
+    for (int g = 0; g < w->S; g++) {
+        size_t base = (size_t)g * w->N2h;
+        size_t kbase = ((size_t)(index - 1) * w->S + g) * w->N2h;
+        for (int r = w->h - ch; r < w->h + w->sgr + ch; r++)
+            for (int c = w->h - ch; c < w->h + w->sgc + ch; c++) {
+                size_t i = base + (size_t)r * ro + c;
+                size_t ki = kbase + (size_t)r * ro + c;
+                if (!p->tetm) {
+                    const double* in = from_buffer ? w->bpsi[0] : w->psi[0];
+                    const double* inn = from_buffer ? w->bn[0] : w->n[0];
+                    const double pr = in[2 * i], pi = in[2 * i + 1];
+                    const double mr = MI_RE(pr, pi), mi = MI_IM(pr, pi);
+                    const double nrm = pr * pr + pi * pi;
+                    /* m_eff_scaled * (C*psi + (N+S)/dy^2 + (E+W)/dx^2) */
+                    double hr = p->m_eff_scaled * (p->m2_over_dx2_p_dy2 * pr + (in[2 * (i + ro)] + in[2 * (i - ro)]) * p->one_over_dy2 + (in[2 * (i + 1)] + in[2 * (i - 1)]) * p->one_over_dx2);
+                    double hi = p->m_eff_scaled * (p->m2_over_dx2_p_dy2 * pi + (in[2 * (i + ro) + 1] + in[2 * (i - ro) + 1]) * p->one_over_dy2 + (in[2 * (i + 1) + 1] + in[2 * (i - 1) + 1]) * p->one_over_dx2);
+                    double wr = MI_RE(hr, hi) * ohb, wi = MI_IM(hr, hi) * ohb;
+                    wr += ohb * p->g_c * nrm * mr;
+                    wi += ohb * p->g_c * nrm * mi;
+                    wr -= 0.5 * p->gamma_c * pr;
+                    wi -= 0.5 * p->gamma_c * pi;
+                    if (p->use_reservoir) {
+                        const double nv = inn[i];
+                        wr += ohb * p->g_r * nv * mr;
+                        wi += ohb * p->g_r * nv * mi;
+                        wr += 0.5 * p->R * nv * pr;
+                        wi += 0.5 * p->R * nv * pi;
+                        double kn = -(p->gamma_r + p->R * nrm) * nv;
+                        if (p->use_pump) kn += w->pump[0][i] * pamp;
+                        if (stoch) kn += p->R * nv / p->dV;
+                        w->kn[0][ki] = kn;
+                    }
+                    if (p->use_potential) {
+                        const double v = w->pot[0][i] * vamp;
+                        wr += ohb * v * mr;
+                        wi += ohb * v * mi;
+                    }
+                    if (stoch) {
+                        wr -= ohb * p->g_c * mr / p->dV;
+                        wi -= ohb * p->g_c * mi / p->dV;
+                    }
+                    w->kpsi[0][2 * ki] = wr;
+                    w->kpsi[0][2 * ki + 1] = wi;
+                } else {
+                    /* TE-TM branch (kernel_gp_compute.cuh:112-235) */
+                    const double* P = from_buffer ? w->bpsi[0] : w->psi[0];
+                    const double* M = from_buffer ? w->bpsi[1] : w->psi[1];
+                    const double* NP = from_buffer ? w->bn[0] : w->n[0];
+                    const double* NM = from_buffer ? w->bn[1] : w->n[1];
+                    const double idx2 = p->one_over_dx2, idy2 = p->one_over_dy2;
+                    double horp_r = (P[2 * (i + 1)] + P[2 * (i - 1)]) * idx2, horp_i = (P[2 * (i + 1) + 1] + P[2 * (i - 1) + 1]) * idx2;
+                    double verp_r = (P[2 * (i + ro)] + P[2 * (i - ro)]) * idy2, verp_i = (P[2 * (i + ro) + 1] + P[2 * (i - ro) + 1]) * idy2;
+                    double horm_r = (M[2 * (i + 1)] + M[2 * (i - 1)]) * idx2, horm_i = (M[2 * (i + 1) + 1] + M[2 * (i - 1) + 1]) * idx2;
+                    double verm_r = (M[2 * (i + ro)] + M[2 * (i - ro)]) * idy2, verm_i = (M[2 * (i + ro) + 1] + M[2 * (i - ro) + 1]) * idy2;
+                    /* diagonal combination d = f[i+ro+1] + f[i-ro-1] - f[i+ro-1] - f[i-ro+1] */
+                    double dm_r = M[2 * (i + ro + 1)] + M[2 * (i - ro - 1)] - M[2 * (i + ro - 1)] - M[2 * (i - ro + 1)];
+                    double dm_i = M[2 * (i + ro + 1) + 1] + M[2 * (i - ro - 1) + 1] - M[2 * (i + ro - 1) + 1] - M[2 * (i - ro + 1) + 1];
+                    double dp_r = P[2 * (i + ro + 1)] + P[2 * (i - ro - 1)] - P[2 * (i + ro - 1)] - P[2 * (i - ro + 1)];
+                    double dp_i = P[2 * (i + ro + 1) + 1] + P[2 * (i - ro - 1) + 1] - P[2 * (i + ro - 1) + 1] - P[2 * (i - ro + 1) + 1];
+                    const double hdd = 0.5 / p->dx / p->dy; /* (i/2)/dx/dy */
+                    /* (i*hdd) * d = (-hdd*d_i, hdd*d_r) */
+                    double crp_r = horm_r - verm_r + (-hdd * dm_i), crp_i = horm_i - verm_i + hdd * dm_r;
+                    double crm_r = horp_r - verp_r - (-hdd * dp_i), crm_i = horp_i - verp_i - hdd * dp_r;
+                    const double ppr = P[2 * i], ppi = P[2 * i + 1], pmr = M[2 * i], pmi = M[2 * i + 1];
+                    double regp_r = p->m2_over_dx2_p_dy2 * ppr + horp_r + verp_r, regp_i = p->m2_over_dx2_p_dy2 * ppi + horp_i + verp_i;
+                    double regm_r = p->m2_over_dx2_p_dy2 * pmr + horm_r + verm_r, regm_i = p->m2_over_dx2_p_dy2 * pmi + horm_i + verm_i;
+                    const double np_ = NP[i], nm_ = NM[i];
+                    const double nrmp = ppr * ppr + ppi * ppi, nrmm = pmr * pmr + pmi * pmi;
+                    for (int comp = 0; comp < 2; comp++) {
+                        const double sr = comp ? pmr : ppr, si = comp ? pmi : ppi;
+                        const double mr = MI_RE(sr, si), mi = MI_IM(sr, si);
+                        const double rr = comp ? regm_r : regp_r, ri = comp ? regm_i : regp_i;
+                        const double cr = comp ? crm_r : crp_r, ci = comp ? crm_i : crp_i;
+                        const double nv = comp ? nm_ : np_;
+                        const double nself = comp ? nrmm : nrmp, nother = comp ? nrmp : nrmm;
+                        double res_r = ohb * p->m_eff_scaled * MI_RE(rr, ri);
+                        double res_i = ohb * p->m_eff_scaled * MI_IM(rr, ri);
+                        if (p->use_potential) {
+                            const double v = w->pot[comp][i] * vamp;
+                            res_r += ohb * v * mr;
+                            res_i += ohb * v * mi;
+                        }
+                        res_r += ohb * p->g_c * nself * mr;  res_i += ohb * p->g_c * nself * mi;
+                        res_r += ohb * p->g_r * nv * mr;     res_i += ohb * p->g_r * nv * mi;
+                        res_r += 0.5 * p->R * nv * sr;       res_i += 0.5 * p->R * nv * si;
+                        res_r -= 0.5 * p->gamma_c * sr;      res_i -= 0.5 * p->gamma_c * si;
+                        res_r += ohb * p->g_pm * nother * mr; res_i += ohb * p->g_pm * nother * mi;
+                        res_r += ohb * p->delta_LT * MI_RE(cr, ci);
+                        res_i += ohb * p->delta_LT * MI_IM(cr, ci);
+                        if (stoch) {
+                            res_r -= ohb * p->g_c * mr / p->dV;
+                            res_i -= ohb * p->g_c * mi / p->dV;
+                        }
+                        w->kpsi[comp][2 * ki] = res_r;
+                        w->kpsi[comp][2 * ki + 1] = res_i;
+                        double kn = -(p->gamma_r + p->R * nself) * nv;
+                        if (p->use_pump) kn += w->pump[comp][i] * pamp;
+                        if (stoch) kn += p->R * nv / p->dV;
+                        w->kn[comp][ki] = kn;
+                    }
+                }
+            }
+    }
+}
+
+/* sum_{j} weight_j k_slot_j with zero weights skipped (kernel_summation.cuh:7-29),
+   left-to-right accumulation. */
+static inline void ksum(const ws_t* w, const double* kf, int elem, size_t off, int nw, const int* slot, const double* wts, double* out) {
+    out[0] = 0.0; if (elem == 2) out[1] = 0.0;
+    int first = 1;
+    for (int j = 0; j < nw; j++) {
+        if (wts[j] == 0.0) continue;
+        size_t o = ((size_t)slot[j] * w->S * w->N2h + off) * elem;
+        if (first) {
+            out[0] = wts[j] * kf[o];
+            if (elem == 2) out[1] = wts[j] * kf[o + 1];
+            first = 0;
+        } else {
+            out[0] += wts[j] * kf[o];
+            if (elem == 2) out[1] += wts[j] * kf[o + 1];
+        }
+    }
+}
+
+/* Stochastic increment (kernel_summation.cuh:209-225): w_sum * xi * sqrt((R n + gamma_c)/(4 dV)),
+   n = step-start reservoir+ (used for both components). */
+static void add_dw(const ws_t* w, size_t i, double wsum, double* out_r, double* out_i) {
+    const orc_params* p = w->p;
+    if (!(p->use_stochastic && p->stochastic_amplitude > 0.0)) return;
+    double s = p->use_reservoir ? sqrt((p->R * w->n[0][i] + p->gamma_c) / (4.0 * p->dV)) : sqrt(p->gamma_c / (4.0 * p->dV));
+    *out_r += wsum * w->xi[2 * i] * s;
+    *out_i += wsum * w->xi[2 * i + 1] * s;
+}
+
+/* INTERMEDIATE_SUM_K(index, weights) (cuda_macro.cuh:153-182): buffer = X + dt*sum
+   on extent halo-index; single weight -> slot index-1, else slots 0..n-1. */
+static void sum_to_buffer(ws_t* w, int index, int nw, const double* wts) {
+    const int ch = w->h - index;
+    int slot[8];
+    for (int j = 0; j < nw; j++) slot[j] = (nw == 1) ? index - 1 : j;
+    double wsum = 0.0;
+    for (int j = 0; j < nw; j++) wsum += wts[j];
+    const double dt = w->dt;
+    for (int g = 0; g < w->S; g++)
+        for (int r = w->h - ch; r < w->h + w->sgr + ch; r++)
+            for (int c = w->h - ch; c < w->h + w->sgc + ch; c++) {
+                size_t i = (size_t)g * w->N2h + (size_t)r * w->ro + c;
+                for (int comp = 0; comp < w->ncomp; comp++) {
+                    double s[2];
+                    ksum(w, w->kpsi[comp], 2, i, nw, slot, wts, s);
+                    double orr = w->psi[comp][2 * i] + dt * s[0];
+                    double oi = w->psi[comp][2 * i + 1] + dt * s[1];
+                    add_dw(w, i, wsum, &orr, &oi);
+                    w->bpsi[comp][2 * i] = orr;
+                    w->bpsi[comp][2 * i + 1] = oi;
+                    if (w->p->use_reservoir) {
+                        ksum(w, w->kn[comp], 1, i, nw, slot, wts, s);
+                        w->bn[comp][i] = w->n[comp][i] + dt * s[0];
+                    }
+                }
+            }
+}
+
+/* FINAL_SUM_K (cuda_macro.cuh:200-229): X += dt*sum b_j k_j on the whole padded region. */
+static void final_sum(ws_t* w, int nw, const double* wts) {
+    const int ch = w->h;
+    int slot[8];
+    for (int j = 0; j < nw; j++) slot[j] = (nw == 1) ? nw - 1 : j;
+    double wsum = 0.0;
+    for (int j = 0; j < nw; j++) wsum += wts[j];
+    const double dt = w->dt;
+    for (int g = 0; g < w->S; g++)
+        for (int r = w->h - ch; r < w->h + w->sgr + ch; r++)
+            for (int c = w->h - ch; c < w->h + w->sgc + ch; c++) {
+                size_t i = (size_t)g * w->N2h + (size_t)r * w->ro + c;
+                /* the noise amplitude reads the step-start reservoir: evaluate before updating n */
+                double dwr = 0.0, dwi = 0.0;
+                add_dw(w, i, wsum, &dwr, &dwi);
+                for (int comp = 0; comp < w->ncomp; comp++) {
+                    double s[2];
+                    ksum(w, w->kpsi[comp], 2, i, nw, slot, wts, s);
+                    w->psi[comp][2 * i] += dt * s[0];
+                    w->psi[comp][2 * i + 1] += dt * s[1];
+                    w->psi[comp][2 * i] += dwr;
+                    w->psi[comp][2 * i + 1] += dwi;
+                }
+                if (w->p->use_reservoir)
+                    for (int comp = 0; comp < w->ncomp; comp++) {
+                        double s[2];
+                        ksum(w, w->kn[comp], 1, i, nw, slot, wts, s);
+                        w->n[comp][i] += dt * s[0];
+                    }
+            }
+}
+
+static void sync_state(ws_t* w) {
+    for (int c = 0; c < w->ncomp; c++) {
+        sync_halos(w, w->psi[c], 2);
+        if (w->p->use_reservoir) sync_halos(w, w->n[c], 1);
+    }
+}
+
+/* Own noise stream for one step: xi = amp*sqrt(dt)*N2(seed, step, global cell), 0 outside a zero-BC domain. */
+static void gen_noise(ws_t* w, int64_t step, double dt) {
+    const orc_params* p = w->p;
+    if (!(p->use_stochastic && p->stochastic_amplitude > 0.0)) return;
+    const double a = p->stochastic_amplitude * sqrt(dt);
+    for (int g = 0; g < w->S; g++)
+        for (int tr = 0; tr < w->sgr + 2 * w->h; tr++)
+            for (int tc = 0; tc < w->sgc + 2 * w->h; tc++) {
+                size_t i = (size_t)g * w->N2h + (size_t)tr * w->ro + tc;
+                long gc = global_cell(w, g, tr, tc);
+                double re = 0.0, im = 0.0;
+                if (gc >= 0) orc_noise_normal2(p->noise_seed, (uint64_t)step, (uint32_t)gc, 0, &re, &im);
+                w->xi[2 * i] = a * re;
+                w->xi[2 * i + 1] = a * im;
+            }
+}
+
+static void load_state(ws_t* w, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                       const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus) {
+    const orc_params* p = w->p;
+    to_sub(w, w->psi[0], psi_plus, 2);
+    if (p->use_reservoir) to_sub(w, w->n[0], n_plus, 1);
+    if (p->use_pump) { to_sub(w, w->pump[0], pump_plus, 1); sync_halos(w, w->pump[0], 1); }
+    if (p->use_potential) { to_sub(w, w->pot[0], pot_plus, 1); sync_halos(w, w->pot[0], 1); }
+    if (p->tetm) {
+        to_sub(w, w->psi[1], psi_minus, 2);
+        if (p->use_reservoir) to_sub(w, w->n[1], n_minus, 1);
+        if (p->use_pump) { to_sub(w, w->pump[1], pump_minus, 1); sync_halos(w, w->pump[1], 1); }
+        if (p->use_potential) { to_sub(w, w->pot[1], pot_minus, 1); sync_halos(w, w->pot[1], 1); }
+    }
+}
+
+static void store_state(ws_t* w, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus, int from_buffer) {
+    const orc_params* p = w->p;
+    to_full(w, from_buffer ? w->bpsi[0] : w->psi[0], psi_plus, 2);
+    if (p->use_reservoir) to_full(w, from_buffer ? w->bn[0] : w->n[0], n_plus, 1);
+    if (p->tetm) {
+        to_full(w, from_buffer ? w->bpsi[1] : w->psi[1], psi_minus, 2);
+        if (p->use_reservoir) to_full(w, from_buffer ? w->bn[1] : w->n[1], n_minus, 1);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* RK4 (runge_kutta.cu:74-94)                                               */
+/* ------------------------------------------------------------------------ */
+static void rk4_step(ws_t* w) {
+    static const double half = 0.5, one = 1.0;
+    static const double b[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    sync_state(w);
+    calc_k(w, 1, 0.0, 0);
+    sum_to_buffer(w, 1, 1, &half);
+    calc_k(w, 2, 0.5, 1);
+    sum_to_buffer(w, 2, 1, &half);
+    calc_k(w, 3, 0.5, 1);
+    sum_to_buffer(w, 3, 1, &one);
+    calc_k(w, 4, 1.0, 1);
+    final_sum(w, 4, b);
+}
+
+int orc_rk4_run(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                double dt, int steps, int64_t step0) {
+    ws_t w;
+    if (ws_init(&w, p, 4)) return -1;
+    w.dt = dt;
+    load_state(&w, psi_plus, psi_minus, n_plus, n_minus, pump_plus, pump_minus, pot_plus, pot_minus);
+    for (int s = 0; s < steps; s++) {
+        gen_noise(&w, step0 + s, dt);
+        rk4_step(&w);
+    }
+    store_state(&w, psi_plus, psi_minus, n_plus, n_minus, 0);
+    ws_free(&w);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dormand-Prince 5(4) (dormand_prince.cu:37-95)                            */
+/* ------------------------------------------------------------------------ */
+static const double DP_A2[1] = {1.0 / 5.0};
+static const double DP_A3[2] = {3.0 / 40.0, 9.0 / 40.0};
+static const double DP_A4[3] = {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0};
+static const double DP_A5[4] = {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0};
+static const double DP_A6[5] = {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0};
+static const double DP_B[6] = {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0};
+static const double DP_E[7] = {35.0 / 384.0 - 5179.0 / 57600.0, 0.0, 500.0 / 1113.0 - 7571.0 / 16695.0, 125.0 / 192.0 - 393.0 / 640.0,
+                               -2187.0 / 6784.0 + 92097.0 / 339200.0, 11.0 / 84.0 - 187.0 / 2100.0, -1.0 / 40.0};
+
+static void dp45_stages(ws_t* w) {
+    sync_state(w);
+    calc_k(w, 1, 0.0, 0);
+    sum_to_buffer(w, 1, 1, DP_A2);
+    calc_k(w, 2, 1.0 / 5.0, 1);
+    sum_to_buffer(w, 2, 2, DP_A3);
+    calc_k(w, 3, 3.0 / 10.0, 1);
+    sum_to_buffer(w, 3, 3, DP_A4);
+    calc_k(w, 4, 4.0 / 5.0, 1);
+    sum_to_buffer(w, 4, 4, DP_A5);
+    calc_k(w, 5, 8.0 / 9.0, 1);
+    sum_to_buffer(w, 5, 5, DP_A6);
+    calc_k(w, 6, 1.0, 1);
+    sum_to_buffer(w, 6, 6, DP_B);
+    calc_k(w, 7, 1.0, 1);
+}
+
+int orc_dp45_attempt(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                     const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                     double dt, double noise_dt, int64_t step_index, double* err_sum, double* norm) {
+    ws_t w;
+    if (ws_init(&w, p, 7)) return -1;
+    w.dt = dt;
+    load_state(&w, psi_plus, psi_minus, n_plus, n_minus, pump_plus, pump_minus, pot_plus, pot_minus);
+    gen_noise(&w, step_index, noise_dt);
+    dp45_stages(&w);
+    /* ERROR_K (runge_sum_to_error, kernel_summation.cuh:270-289) on interior cells;
+       norm = sum |buffer psi+|^2 on interior cells. */
+    double es = 0.0, ns = 0.0;
+    int slot[7] = {0, 1, 2, 3, 4, 5, 6};
+    for (int g = 0; g < w.S; g++)
+        for (int r = w.h; r < w.h + w.sgr; r++)
+            for (int c = w.h; c < w.h + w.sgc; c++) {
+                size_t i = (size_t)g * w.N2h + (size_t)r * w.ro + c;
+                double s[2];
+                ksum(&w, w.kpsi[0], 2, i, 7, slot, DP_E, s);
+                double er = dt * s[0], ei = dt * s[1];
+                double e = er * er + ei * ei;
+                es += p->tetm ? 2.0 * e : e;
+                ns += w.bpsi[0][2 * i] * w.bpsi[0][2 * i] + w.bpsi[0][2 * i + 1] * w.bpsi[0][2 * i + 1];
+            }
+    *err_sum = es;
+    *norm = ns;
+    store_state(&w, psi_plus, psi_minus, n_plus, n_minus, 1);
+    ws_free(&w);
+    return 0;
+}
+
+/* Solver::adaptTimeStep (solver.cu:72-105), power 1/4. */
+int orc_adapt_dt(double err_sum, double norm, double tolerance, double dt, double dt_min, double dt_max, double* new_dt, double* final_error) {
+    double fe = fabs(err_sum / norm);
+    double dh_arg = tolerance / 2.0 / (fe > DBL_MIN ? fe : DBL_MIN);
+    double dh = pow(dh_arg, 0.25);
+    if (isnan(dh)) { dh = 0.9; fe = DBL_MAX; }
+    if (isnan(fe)) { dh = 0.9; fe = DBL_MAX; }
+    double nd = dt * dh < dt_max ? dt * dh : dt_max;
+    if (nd < dt_min) nd = dt_min;
+    *new_dt = nd;
+    *final_error = fe;
+    return fe < tolerance;
+}
+
+int64_t orc_dp45_run(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                     const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                     double* t, double* dt, double t_max, double tolerance, double dt_min, double dt_max,
+                     int max_rejections, int64_t* n_rejected) {
+    const size_t N = (size_t)p->N_c * p->N_r;
+    const int nc = p->tetm ? 2 : 1;
+    double *ps[2] = {psi_plus, psi_minus}, *ns[2] = {n_plus, n_minus};
+    double *tp[2] = {0, 0}, *tn[2] = {0, 0};
+    for (int c = 0; c < nc; c++) {
+        tp[c] = malloc(2 * N * sizeof(double));
+        tn[c] = malloc(N * sizeof(double));
+    }
+    int64_t accepted = 0, rejected = 0;
+    int64_t step = 0;
+    while (*t < t_max) {
+        const double noise_dt = *dt; /* noise drawn once per iterate() with sqrt(dt) (solver.cu:36-45) */
+        *t = *t + *dt; /* solver.cu:50, before the attempts */
+        int rej = 0;
+        for (;;) {
+            for (int c = 0; c < nc; c++) {
+                memcpy(tp[c], ps[c], 2 * N * sizeof(double));
+                if (p->use_reservoir) memcpy(tn[c], ns[c], N * sizeof(double));
+            }
+            double es, nm, nd, fe;
+            if (orc_dp45_attempt(p, tp[0], tp[1], tn[0], tn[1], pump_plus, pump_minus, pot_plus, pot_minus, *dt, noise_dt, step, &es, &nm)) return -1;
+            int acc = orc_adapt_dt(es, nm, tolerance, *dt, dt_min, dt_max, &nd, &fe);
+            *dt = nd;
+            if (acc) break;
+            rejected++;
+            if (++rej > max_rejections) {
+                *n_rejected = rejected;
+                for (int c = 0; c < nc; c++) { free(tp[c]); free(tn[c]); }
+                return -2;
+            }
+        }
+        for (int c = 0; c < nc; c++) {
+            memcpy(ps[c], tp[c], 2 * N * sizeof(double));
+            if (p->use_reservoir) memcpy(ns[c], tn[c], N * sizeof(double));
+        }
+        accepted++;
+        step++;
+    }
+    for (int c = 0; c < nc; c++) { free(tp[c]); free(tn[c]); }
+    *n_rejected = rejected;
+    return accepted;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FFT filter (solver_fft.cu:67-119, kernel_fft.cuh:15-39)                  */
+/* ------------------------------------------------------------------------ */
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* 1D unnormalised DFT of length n with stride; radix-2 iterative for powers
+   of two, direct summation otherwise. */
+static void dft1d(double* x, int n, size_t stride, int sign, double* tmp) {
+    for (int i = 0; i < n; i++) { tmp[2 * i] = x[2 * i * stride]; tmp[2 * i + 1] = x[2 * i * stride + 1]; }
+    if (is_pow2(n)) {
+        for (int i = 1, j = 0; i < n; i++) {
+            int bit = n >> 1;
+            for (; j & bit; bit >>= 1) j ^= bit;
+            j ^= bit;
+            if (i < j) {
+                double a = tmp[2 * i], b = tmp[2 * i + 1];
+                tmp[2 * i] = tmp[2 * j]; tmp[2 * i + 1] = tmp[2 * j + 1];
+                tmp[2 * j] = a; tmp[2 * j + 1] = b;
+            }
+        }
+        for (int len = 2; len <= n; len <<= 1) {
+            for (int i = 0; i < n; i += len)
+                for (int k = 0; k < len / 2; k++) {
+                    double ang = sign * 2.0 * M_PI * k / len;
+                    double wr = cos(ang), wi = sin(ang);
+                    double* u = tmp + 2 * (i + k);
+                    double* v = tmp + 2 * (i + k + len / 2);
+                    double vr = v[0] * wr - v[1] * wi, vi = v[0] * wi + v[1] * wr;
+                    v[0] = u[0] - vr; v[1] = u[1] - vi;
+                    u[0] += vr; u[1] += vi;
+                }
+        }
+        for (int i = 0; i < n; i++) { x[2 * i * stride] = tmp[2 * i]; x[2 * i * stride + 1] = tmp[2 * i + 1]; }
+    } else {
+        double* o = tmp + 2 * n;
+        for (int k = 0; k < n; k++) {
+            double sr = 0, si = 0;
+            for (int j = 0; j < n; j++) {
+                double ang = sign * 2.0 * M_PI * (double)((long)j * k % n) / n;
+                sr += tmp[2 * j] * cos(ang) - tmp[2 * j + 1] * sin(ang);
+                si += tmp[2 * j] * sin(ang) + tmp[2 * j + 1] * cos(ang);
+            }
+            o[2 * k] = sr; o[2 * k + 1] = si;
+        }
+        for (int i = 0; i < n; i++) { x[2 * i * stride] = o[2 * i]; x[2 * i * stride + 1] = o[2 * i + 1]; }
+    }
+}
+
+int orc_fft2d(double* data, int N_c, int N_r, int sign) {
+    int m = N_c > N_r ? N_c : N_r;
+    double* tmp = malloc(4 * (size_t)m * sizeof(double));
+    if (!tmp) return -1;
+    for (int r = 0; r < N_r; r++) dft1d(data + 2 * (size_t)r * N_c, N_c, 1, sign, tmp);
+    for (int c = 0; c < N_c; c++) dft1d(data + 2 * (size_t)c, N_r, (size_t)N_c, sign, tmp);
+    free(tmp);
+    return 0;
+}
+
+int orc_fft_filter(double* psi, const double* mask, int N_c, int N_r) {
+    const size_t N = (size_t)N_c * N_r;
+    if (orc_fft2d(psi, N_c, N_r, -1)) return -1;
+    /* fftshift of the mask by quadrant swap (fft_shift_2D), then F <- F / N * mask */
+    for (int r = 0; r < N_r; r++)
+        for (int c = 0; c < N_c; c++) {
+            int rs = (r + N_r / 2) % N_r, cs = (c + N_c / 2) % N_c;
+            double m = mask[(size_t)rs * N_c + cs];
+            double* f = psi + 2 * ((size_t)r * N_c + c);
+            f[0] = f[0] / (double)N * m;
+            f[1] = f[1] / (double)N * m;
+        }
+    return orc_fft2d(psi, N_c, N_r, +1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Envelope (envelope.cpp:332-412)                                          */
+/* ------------------------------------------------------------------------ */
+void orc_envelope_add(double* out, int N_c, int N_r, double L_x, double L_y, double dx, double dy,
+                      double amp, double width_x, double width_y, double pos_x, double pos_y,
+                      double exponent, int type_flags, double charge_m, double k0_x, double k0_y, int behaviour) {
+    for (int row = 0; row < N_r; row++)
+        for (int col = 0; col < N_c; col++) {
+            size_t i = (size_t)row * N_c + col;
+            double cx = -L_x / 2.0 + dx * col;
+            double cy = -L_y / 2.0 + dy * row;
+            if (type_flags & 16) { /* Local */
+                cx = -1.0 + 2.0 * col / (N_c - 1);
+                cy = -1.0 + 2.0 * row / (N_r - 1);
+            }
+            double ar = amp, ai = 0.0;
+            double ex = 0.5 * (((cx - pos_x) / width_x) * ((cx - pos_x) / width_x) + ((cy - pos_y) / width_y) * ((cy - pos_y) / width_y));
+            double ef = exp(-pow(ex, exponent));
+            if (type_flags & 2) ef = pow(exp(-ex), exponent); /* OuterExponent */
+            double pre = (type_flags & 4) ? ex : 1.0;         /* Ring */
+            double phi = charge_m * atan2(cx - pos_x, cy - pos_y);
+            double chr = cos(phi), chi = sin(phi);
+            if (!(type_flags & 8)) { /* not NoDivide: A / sqrt(2*3.1415*wx*wy) */
+                double d = sqrt(2 * 3.1415 * width_x * width_y);
+                ar /= d; ai /= d;
+            }
+            double mp = k0_x * (cx - pos_x) + k0_y * (cy - pos_y);
+            double mr = cos(mp), mi = sin(mp);
+            if (behaviour & 8) { /* Adaptive: amplitude = amp * buffer */
+                ar = amp * out[2 * i]; ai = amp * out[2 * i + 1];
+            }
+            if (behaviour & 16) { /* Complex: amplitude = i * real(amplitude) */
+                ai = ar; ar = 0.0;
+            }
+            /* contribution = amplitude * pre * ef * charge * momentum */
+            double s = pre * ef;
+            double t1r = ar * s, t1i = ai * s;
+            double t2r = t1r * chr - t1i * chi, t2i = t1r * chi + t1i * chr;
+            double cr = t2r * mr - t2i * mi, ci = t2r * mi + t2i * mr;
+            if (behaviour & 1) { out[2 * i] += cr; out[2 * i + 1] += ci; }
+            else if (behaviour == 2) {
+                double br = out[2 * i], bi = out[2 * i + 1];
+                out[2 * i] = br * cr - bi * ci; out[2 * i + 1] = br * ci + bi * cr;
+            } else if (behaviour == 4) { out[2 * i] = cr; out[2 * i + 1] = ci; }
+        }
+}

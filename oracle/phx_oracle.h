@@ -1,0 +1,114 @@
+/*
+ * phx_oracle.h -- CPU restatement of PHOENIX's explicit Runge-Kutta time step
+ * (driven-dissipative Gross-Pitaevskii + reservoir), FP64.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * path in phoenix_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product never does.
+ *
+ * Layout conventions at this interface (identical to the product C-ABI):
+ *   full grid, row-major, N_r rows x N_c columns, index = row * N_c + col;
+ *   complex fields interleaved {re, im} (psi), real fields plain (n, P, V).
+ *
+ * Internally the oracle follows the reference structure literally: the grid
+ * is split into subgrids_columns x subgrids_rows padded subgrids with a halo
+ * of width k_max, halos are synchronised once per step (periodic copy or
+ * zero), and stage s is evaluated on the extent (halo - s)
+ * (reference: include/cuda/cuda_macro.cuh:43-49,110-264,410-440).
+ */
+#ifndef PHX_ORACLE_H
+#define PHX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    /* geometry */
+    int N_c, N_r;
+    int subgrids_columns, subgrids_rows;
+    int periodic_x, periodic_y;
+    double dx, dy, dV;
+    /* derived constants (reference: source/system/system.cpp:116-160) */
+    double one_over_dx2, one_over_dy2, m2_over_dx2_p_dy2;
+    double m_eff_scaled, one_over_h_bar_s, h_bar_s, m_eff;
+    /* model parameters (reference: include/system/system_parameters.hpp:25-64) */
+    double gamma_c, gamma_r, g_c, g_r, R, g_pm, delta_LT;
+    /* feature switches (reference: gp_scalar template flags) */
+    int tetm, use_reservoir, use_pump, use_potential, use_stochastic;
+    double stochastic_amplitude;
+    /* temporal amplitudes of pump / potential at t and t+dt
+       (reference: kernel_gp_compute.cuh:79,93 amp*(1-w) + w*amp_next) */
+    double pump_amp, pump_amp_next, pot_amp, pot_amp_next;
+    /* noise stream (own Philox4x32-10 design, see orc_noise_normal2) */
+    uint64_t noise_seed;
+} orc_params;
+
+/* Fill derived constants exactly as SystemParameters::calculateAuto does.
+   h_bar_s / m_eff < 0 mean "derive from h_bar, e_e, m_e".  Returns the
+   magic time step 0.5*dx*dy/(m_e/e*1e8/m_eff).  (system.cpp:116-160) */
+double orc_derive(orc_params* p, double L_x, double L_y, double h_bar, double e_e, double m_e, double h_bar_s, double m_eff);
+
+/* Default physical parameters (system.cpp:18-72). */
+void orc_defaults(orc_params* p);
+
+/* Fixed-step RK4: `steps` full RungeKutta4::step() calls on the state.
+   psi_plus/psi_minus: 2*N doubles; n_plus/n_minus: N doubles (NULL without reservoir);
+   pump_*, pot_*: N doubles or NULL.  t0 is the time at the first step start
+   (only used by the noise stream).  Returns 0 on success.
+   (reference: runge_kutta.cu:74-94, solver.cu:22-70) */
+int orc_rk4_run(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                double dt, int steps, int64_t step0);
+
+/* One Dormand-Prince 5(4) attempt with fixed dt (dormand_prince.cu:37-95, adaptive branch).
+   On return the state holds the 5th-order solution (what swapBuffers would install),
+   *err_sum = sum over interior cells of |dt*sum_j (b_j-b*_j) k_j|^2 (psi+ error counted
+   twice with -tetm, as ERROR_K reuses the psi+ slots for psi-: cuda_macro.cuh:244),
+   *norm = sum over interior cells of |psi+_new|^2.  Both accumulated in FP64. */
+int orc_dp45_attempt(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                     const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                     double dt, double noise_dt, int64_t step_index, double* err_sum, double* norm);
+
+/* Step-size controller of Solver::adaptTimeStep (solver.cu:72-105) on given sums.
+   Returns accept flag, writes the new dt. */
+int orc_adapt_dt(double err_sum, double norm, double tolerance, double dt, double dt_min, double dt_max, double* new_dt, double* final_error);
+
+/* Adaptive DP45 driver: iterate() until t >= t_max (solver.cu:22-70 + main.cu loop).
+   Returns number of accepted steps or a negative error. *t and *dt are in/out. */
+int64_t orc_dp45_run(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                     const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                     double* t, double* dt, double t_max, double tolerance, double dt_min, double dt_max,
+                     int max_rejections, int64_t* n_rejected);
+
+/* k-space filter (solver_fft.cu:67-100, kernel_fft.cuh:15-39):
+   psi <- IDFT( DFT(psi) / (N_c*N_r) * fftshift(mask) ), unnormalised transforms,
+   forward sign e^{-2 pi i ...}.  mask is the unshifted N_r x N_c real mask. */
+int orc_fft_filter(double* psi, const double* mask, int N_c, int N_r);
+
+/* Unnormalised 2D DFT in place; sign = -1 forward, +1 inverse. */
+int orc_fft2d(double* data, int N_c, int N_r, int sign);
+
+/* Envelope spatial shape (envelope.cpp:332-412) for one generated component.
+   Flag bits are the reference enums (envelope.hpp:37-61): type 1 gauss, 2 outerExponent,
+   4 ring, 8 noDivide, 16 local; behaviour 1 add, 2 multiply, 4 replace, 8 adaptive,
+   16 complex.  out is complex interleaved and updated in place. */
+void orc_envelope_add(double* out, int N_c, int N_r, double L_x, double L_y, double dx, double dy,
+                      double amp, double width_x, double width_y, double pos_x, double pos_y,
+                      double exponent, int type_flags, double charge_m, double k0_x, double k0_y, int behaviour);
+
+/* Noise: complex standard normal pair for (seed, step, stage-independent cell index)
+   via Philox4x32-10 (Salmon et al., SC'11) + Box-Muller.  Same stream as the HIP kernels. */
+void orc_noise_normal2(uint64_t seed, uint64_t step, uint32_t cell, int component, double* re, double* im);
+void orc_philox4x32_10(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* Initial random state: 0.01*(N(0,1)+iN(0,1)) per cell from the Philox stream,
+   row-major (the seeded complex Gaussian noise of the benchmark workloads). */
+void orc_gaussian_field(double* psi, int N, double amplitude, uint64_t seed);
+
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,212 @@
+"""ctypes binding of libphx.so (the C ABI declared in include/phx.h).
+
+The HIP library is mandatory: importing this module raises if libphx.so has not
+been built (python -m phoenix_amd.build).  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libphx.so")
+
+PHX_FP32, PHX_FP64 = 0, 1
+PHX_OK, PHX_ERR_INVALID, PHX_ERR_HIP, PHX_ERR_UNSUPPORTED, PHX_ERR_IO, PHX_ERR_STEP, PHX_ERR_STATE = 0, -1, -2, -3, -4, -5, -6
+FIELDS = {
+    "psi_plus": 0, "psi_minus": 1, "n_plus": 2, "n_minus": 3, "pump_plus": 4, "pump_minus": 5,
+    "potential_plus": 6, "potential_minus": 7, "fft_mask_plus": 8, "fft_mask_minus": 9,
+}
+COMPLEX_FIELDS = {0, 1}
+
+# Every entry point declared in include/phx.h (checked by tests/test_capi.py).
+EXPORTED = [
+    "phx_create", "phx_initialize", "phx_step", "phx_run", "phx_apply_fft_filter", "phx_get_field", "phx_set_field",
+    "phx_device_ptr", "phx_get_time", "phx_set_time", "phx_get_info", "phx_set_stream", "phx_sync", "phx_destroy",
+    "phx_last_error", "phx_version", "phx_physics_defaults", "phx_rk4_scalar_device",
+]
+
+
+class PhxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[phx {code}] {msg}")
+        self.code = code
+
+
+class PhxInfo(C.Structure):
+    _fields_ = [
+        ("N_c", C.c_int32), ("N_r", C.c_int32), ("precision", C.c_int32), ("tetm", C.c_int32),
+        ("use_reservoir", C.c_int32), ("use_pump", C.c_int32), ("use_potential", C.c_int32),
+        ("use_stochastic", C.c_int32), ("use_fft_mask", C.c_int32), ("adaptive", C.c_int32),
+        ("periodic_x", C.c_int32), ("periodic_y", C.c_int32), ("halo", C.c_int32), ("iterator", C.c_char * 16),
+        ("dx", C.c_double), ("dy", C.c_double), ("dt", C.c_double), ("t", C.c_double), ("t_max", C.c_double),
+        ("fft_every", C.c_double), ("tolerance", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+        ("iteration", C.c_int64), ("rejected", C.c_int64),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["iterator"] = d["iterator"].decode()
+        return d
+
+
+class PhxPhysics(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("dx", "dy", "dV", "m_eff_scaled", "one_over_h_bar_s", "gamma_c", "gamma_r",
+                                          "g_c", "g_r", "R", "g_pm", "delta_LT")]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m phoenix_amd.build` (no CPU fallback)")
+        # One HIP runtime per process: PyTorch ships its own libamdhip64 (SONAME libamdhip64.so.7) and
+        # loads it by an unversioned name.  Loading torch first lets libphx.so bind to that same runtime;
+        # loading libphx.so first would pull in a second copy and leave torch without devices.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        L = C.CDLL(LIB_PATH)
+        vp, i64 = C.c_void_p, C.c_int64
+        L.phx_create.argtypes = [C.POINTER(vp), C.c_int, C.c_int, C.POINTER(C.c_char_p)]
+        L.phx_initialize.argtypes = [vp]
+        L.phx_step.argtypes = [vp, i64, C.POINTER(i64)]
+        L.phx_run.argtypes = [vp, C.POINTER(i64)]
+        L.phx_apply_fft_filter.argtypes = [vp]
+        L.phx_get_field.argtypes = [vp, C.c_int, vp, C.c_size_t]
+        L.phx_set_field.argtypes = [vp, C.c_int, vp, C.c_size_t]
+        L.phx_device_ptr.argtypes = [vp, C.c_int, C.POINTER(vp)]
+        L.phx_get_time.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.phx_set_time.argtypes = [vp, C.c_double, C.c_double]
+        L.phx_get_info.argtypes = [vp, C.POINTER(PhxInfo)]
+        L.phx_set_stream.argtypes = [vp, vp]
+        L.phx_sync.argtypes = [vp]
+        L.phx_destroy.argtypes = [vp]
+        L.phx_destroy.restype = None
+        L.phx_last_error.restype = C.c_char_p
+        L.phx_version.restype = C.c_char_p
+        L.phx_physics_defaults.argtypes = [C.POINTER(PhxPhysics), C.c_int, C.c_int, C.c_double, C.c_double]
+        L.phx_physics_defaults.restype = C.c_double
+        L.phx_rk4_scalar_device.argtypes = [C.c_int, C.POINTER(PhxPhysics), C.c_int, C.c_int, C.c_int, C.c_int,
+                                            vp, vp, vp, vp, vp, vp, C.c_double, i64, vp]
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != PHX_OK:
+        raise PhxError(rc, lib().phx_last_error().decode())
+    return rc
+
+
+class Solver:
+    """One configured PHOENIX system on the GPU (handle of the C ABI).
+
+    `args` are PHOENIX command line tokens, e.g. ["--N", "256", "256", "--tmax", "10"].
+    """
+
+    def __init__(self, args: Sequence[str], precision: str = "fp64", initialize: bool = True):
+        self.precision = PHX_FP32 if precision == "fp32" else PHX_FP64
+        self.real = np.float32 if self.precision == PHX_FP32 else np.float64
+        self.cplx = np.complex64 if self.precision == PHX_FP32 else np.complex128
+        argv = ["phoenix"] + [str(a) for a in args]
+        arr = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+        h = C.c_void_p()
+        check(lib().phx_create(C.byref(h), self.precision, len(argv), arr))
+        self._h = h
+        if initialize:
+            self.initialize()
+
+    def initialize(self):
+        check(lib().phx_initialize(self._h))
+
+    @property
+    def info(self) -> dict:
+        i = PhxInfo()
+        check(lib().phx_get_info(self._h, C.byref(i)))
+        return i.as_dict()
+
+    def step(self, n: int = 1) -> int:
+        done = C.c_int64()
+        check(lib().phx_step(self._h, n, C.byref(done)))
+        return done.value
+
+    def run(self) -> int:
+        done = C.c_int64()
+        check(lib().phx_run(self._h, C.byref(done)))
+        return done.value
+
+    def apply_fft_filter(self):
+        check(lib().phx_apply_fft_filter(self._h))
+
+    def _shape(self):
+        i = self.info
+        return i["N_r"], i["N_c"]
+
+    def get(self, field: str) -> np.ndarray:
+        fid = FIELDS[field]
+        out = np.empty(self._shape(), self.cplx if fid in COMPLEX_FIELDS else self.real)
+        check(lib().phx_get_field(self._h, fid, out.ctypes.data, out.nbytes))
+        return out
+
+    def set(self, field: str, value: np.ndarray):
+        fid = FIELDS[field]
+        v = np.ascontiguousarray(value, self.cplx if fid in COMPLEX_FIELDS else self.real)
+        if v.shape != self._shape():
+            raise ValueError(f"{field}: expected shape {self._shape()}, got {v.shape}")
+        check(lib().phx_set_field(self._h, fid, v.ctypes.data, v.nbytes))
+
+    def device_ptr(self, field: str) -> int:
+        p = C.c_void_p()
+        check(lib().phx_device_ptr(self._h, FIELDS[field], C.byref(p)))
+        return p.value
+
+    def time(self):
+        t, dt = C.c_double(), C.c_double()
+        check(lib().phx_get_time(self._h, C.byref(t), C.byref(dt)))
+        return t.value, dt.value
+
+    def set_time(self, t: float, dt: float):
+        check(lib().phx_set_time(self._h, t, dt))
+
+    def set_stream(self, stream_ptr: Optional[int]):
+        check(lib().phx_set_stream(self._h, stream_ptr))
+
+    def sync(self):
+        check(lib().phx_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().phx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def physics_defaults(N_c: int, N_r: int, L_x: float, L_y: float):
+    ph = PhxPhysics()
+    magic = lib().phx_physics_defaults(C.byref(ph), N_c, N_r, L_x, L_y)
+    return ph, magic
+
+
+def rk4_scalar_device(precision: int, ph: PhxPhysics, N_c: int, N_r: int, periodic: tuple, psi_a: int, n_a: int,
+                      psi_b: int, n_b: int, pump: int, pot: int, dt: float, n_steps: int, stream: int = 0):
+    check(lib().phx_rk4_scalar_device(precision, C.byref(ph), N_c, N_r, int(periodic[0]), int(periodic[1]), psi_a,
+                                      n_a, psi_b, n_b, pump, pot, dt, n_steps, stream))

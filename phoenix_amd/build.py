@@ -1,0 +1,77 @@
+"""Build libphx.so and the PHOENIX-compatible executables for gfx950.
+
+Usage: python -m phoenix_amd.build   (or phoenix_amd.build.build() from Python)
+
+Outputs (in-tree, so they travel with the source tree):
+  phoenix_amd/lib/libphx.so            C ABI of include/phx.h
+  phoenix_amd/bin/PHOENIX_gpu_fp32     CLI, same flags as the reference executables
+  phoenix_amd/bin/PHOENIX_gpu_fp64
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "lib")
+BIN = os.path.join(PKG, "bin")
+
+ARCH = os.environ.get("PHX_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+
+# Packed FP32 VALU ops stay disabled on every device pass.
+HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-result",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include")]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include")]
+
+HIP_SOURCES = ["phx_rk4.hip", "phx_solver.hip"]
+CXX_SOURCES = ["phx_config.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIB, exist_ok=True)
+    os.makedirs(BIN, exist_ok=True)
+    jobs = []
+    objs = []
+    for s in HIP_SOURCES:
+        o = os.path.join(BUILD, s.replace(".hip", ".o"))
+        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    for s in CXX_SOURCES:
+        o = os.path.join(BUILD, s.replace(".cpp", ".o"))
+        jobs.append([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        objs.append(o)
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
+        for r in ex.map(_run, jobs):
+            if verbose and r.stderr:
+                print(r.stderr, file=sys.stderr)
+    lib = os.path.join(LIB, "libphx.so")
+    tmp = lib + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, lib)
+    main_o = os.path.join(BUILD, "phx_main.o")
+    _run([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, "phx_main.cpp"), "-o", main_o])
+    for prec in ("fp32", "fp64"):
+        exe = os.path.join(BIN, f"PHOENIX_gpu_{prec}")
+        _run([CXX, f"-DPHX_MAIN_PRECISION={0 if prec == 'fp32' else 1}", *CXX_FLAGS,
+              os.path.join(CSRC, "phx_main.cpp"), "-o", exe, f"-L{LIB}", "-lphx", f"-Wl,-rpath,$ORIGIN/../lib"])
+    return lib
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,542 @@
+/*
+ * phx_solver.hip -- the solver object behind the C ABI of include/phx.h.
+ *
+ * Owns the device fields (full grid, no halos: the fused kernels resolve the
+ * boundaries), drives iterate() with the reference's time bookkeeping
+ * (solver.cu:22-70) and the main-loop output landing (main.cu:80-117), and
+ * evaluates the envelopes on the host once (solver_initialization.cu:37-174).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/phx.h"
+#include "phx_config.hpp"
+#include "phx_kernels.h"
+
+namespace {
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define PHX_HIP(call)                                                                     \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) return fail(PHX_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+    } while (0)
+} // namespace
+
+struct phx_solver {
+    phx::SystemConfig cfg;
+    int precision = PHX_FP64;
+    size_t rsize = 8; /* sizeof(real) */
+    hipStream_t own_stream = nullptr, stream = nullptr;
+    bool initialized = false;
+    /* device fields: [component][buffer] */
+    void* psi[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    void* n[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    int cur = 0; /* current buffer */
+    void* pump[2] = {nullptr, nullptr};
+    void* pot[2] = {nullptr, nullptr};
+    void* mask[2] = {nullptr, nullptr};
+    /* time bookkeeping in the build's precision (system_parameters.hpp:33) */
+    double t = 0, dt = 0, fft_cached_t = 0;
+    int64_t iteration = 0, rejected = 0;
+    int ty = 16; /* RK4 tile rows per wavefront */
+    std::vector<void*> allocs;
+
+    size_t cells() const { return (size_t)cfg.N_c * cfg.N_r; }
+    double rnd(double x) const { return precision == PHX_FP32 ? (double)(float)x : x; }
+};
+
+namespace {
+
+int dev_alloc(phx_solver* s, void** p, size_t bytes) {
+    PHX_HIP(hipMalloc(p, bytes));
+    PHX_HIP(hipMemsetAsync(*p, 0, bytes, s->stream));
+    s->allocs.push_back(*p);
+    return PHX_OK;
+}
+
+template <typename real>
+int upload_real(phx_solver* s, void* dst, const std::vector<std::complex<double>>& src) {
+    std::vector<real> h(src.size());
+    for (size_t i = 0; i < src.size(); i++) h[i] = (real)src[i].real();
+    PHX_HIP(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(real), hipMemcpyHostToDevice, s->stream));
+    PHX_HIP(hipStreamSynchronize(s->stream));
+    return PHX_OK;
+}
+
+template <typename real>
+int upload_complex(phx_solver* s, void* dst, const std::vector<std::complex<double>>& src) {
+    std::vector<real> h(2 * src.size());
+    for (size_t i = 0; i < src.size(); i++) {
+        h[2 * i] = (real)src[i].real();
+        h[2 * i + 1] = (real)src[i].imag();
+    }
+    PHX_HIP(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(real), hipMemcpyHostToDevice, s->stream));
+    PHX_HIP(hipStreamSynchronize(s->stream));
+    return PHX_OK;
+}
+
+/* Sum of all spatial groups, each at temporal amplitude 1 (valid when all groups are
+   constant) or a single group whose amplitude is applied per stage. */
+int env_field(phx_solver* s, const phx::Envelope& e, uint32_t pol, std::vector<std::complex<double>>& out) {
+    const auto& c = s->cfg;
+    out.assign(s->cells(), {0.0, 0.0});
+    std::vector<std::complex<double>> tmp(s->cells());
+    for (int g = 0; g < e.groupSize(); g++) {
+        e.calculate(tmp.data(), g, pol, c.N_c, c.N_r, c.L_x, c.L_y, c.dx, c.dy, 0.0);
+        for (size_t i = 0; i < tmp.size(); i++) out[i] += tmp[i];
+    }
+    return PHX_OK;
+}
+
+template <typename real>
+int initialize_t(phx_solver* s) {
+    auto& c = s->cfg;
+    const size_t N = s->cells();
+    const int ncomp = c.tetm ? 2 : 1;
+    if (c.tetm) return fail(PHX_ERR_UNSUPPORTED, "-tetm is not yet provided by this library");
+    if (c.iterator != "RK4") return fail(PHX_ERR_UNSUPPORTED, "iterator " + c.iterator + " is not yet provided by this library");
+    if (c.use_stochastic) return fail(PHX_ERR_UNSUPPORTED, "--dw is not yet provided by this library");
+    if (c.use_fft_mask && c.fft_every < c.t_max) return fail(PHX_ERR_UNSUPPORTED, "--fftMask is not yet provided by this library");
+    if (c.use_pumps && c.pump.groupSize() > 1 && !c.pump.allConstant())
+        return fail(PHX_ERR_UNSUPPORTED, "more than one temporal pump group");
+    if (c.use_potentials && c.potential.groupSize() > 1 && !c.potential.allConstant())
+        return fail(PHX_ERR_UNSUPPORTED, "more than one temporal potential group");
+
+    for (int k = 0; k < ncomp; k++) {
+        for (int b = 0; b < 2; b++) {
+            if (int r = dev_alloc(s, &s->psi[k][b], 2 * N * sizeof(real))) return r;
+            if (c.use_reservoir)
+                if (int r = dev_alloc(s, &s->n[k][b], N * sizeof(real))) return r;
+        }
+        if (c.use_reservoir && c.use_pumps)
+            if (int r = dev_alloc(s, &s->pump[k], N * sizeof(real))) return r;
+        if (c.use_potentials)
+            if (int r = dev_alloc(s, &s->pot[k], N * sizeof(real))) return r;
+        if (c.use_fft_mask)
+            if (int r = dev_alloc(s, &s->mask[k], N * sizeof(real))) return r;
+    }
+
+    /* initial states (solver_initialization.cu:55-98) */
+    std::vector<std::complex<double>> psi0[2], n0[2];
+    for (int k = 0; k < ncomp; k++) {
+        const uint32_t pol = k == 0 ? phx::POL_PLUS : phx::POL_MINUS;
+        psi0[k].assign(N, {0.0, 0.0});
+        c.initial_state.calculate(psi0[k].data(), -1, pol, c.N_c, c.N_r, c.L_x, c.L_y, c.dx, c.dy, 0.0);
+        n0[k].assign(N, {0.0, 0.0});
+        if (c.use_reservoir) c.initial_reservoir.calculate(n0[k].data(), -1, pol, c.N_c, c.N_r, c.L_x, c.L_y, c.dx, c.dy, 0.0);
+    }
+    if (c.random_init) {
+        /* std::mt19937 + uniform_real_distribution<real>(-a, a), psi+ then psi-, row-major */
+        std::mt19937 gen{c.random_seed};
+        std::uniform_real_distribution<real> dist{(real)-c.random_amp, (real)c.random_amp};
+        for (int k = 0; k < ncomp; k++)
+            for (auto& z : psi0[k]) {
+                real a = dist(gen);
+                real b = dist(gen);
+                z += std::complex<double>((double)a, (double)b);
+            }
+    }
+    for (int k = 0; k < ncomp; k++) {
+        if (int r = upload_complex<real>(s, s->psi[k][0], psi0[k])) return r;
+        if (c.use_reservoir)
+            if (int r = upload_real<real>(s, s->n[k][0], n0[k])) return r;
+        const uint32_t pol = k == 0 ? phx::POL_PLUS : phx::POL_MINUS;
+        std::vector<std::complex<double>> f;
+        if (s->pump[k]) {
+            env_field(s, c.pump, pol, f);
+            if (int r = upload_real<real>(s, s->pump[k], f)) return r;
+        }
+        if (s->pot[k]) {
+            env_field(s, c.potential, pol, f);
+            if (int r = upload_real<real>(s, s->pot[k], f)) return r;
+        }
+        if (s->mask[k]) {
+            f.assign(N, {0.0, 0.0});
+            c.fft_mask.calculate(f.data(), -1, pol, c.N_c, c.N_r, c.L_x, c.L_y, c.dx, c.dy, 1.0);
+            if (int r = upload_real<real>(s, s->mask[k], f)) return r;
+        }
+    }
+    s->t = s->rnd(c.t);
+    s->dt = s->rnd(c.dt);
+    s->fft_cached_t = 0.0;
+    s->cur = 0;
+    s->initialized = true;
+    if (const char* e = std::getenv("PHX_RK4_TY")) s->ty = std::atoi(e);
+    return PHX_OK;
+}
+
+/* temporal amplitude of a single-group envelope, interpolated as the kernels do
+   (kernel_gp_compute.cuh:79): a(t)(1-w) + w a(t+dt) */
+template <typename real>
+void stage_amps(const phx::Envelope& e, double t, double dt, const double* w, int ns, real* out) {
+    real a0 = 1, a1 = 1;
+    if (e.groupSize() == 1 && !e.allConstant()) {
+        a0 = (real)e.temporal(0, t).real();
+        a1 = (real)e.temporal(0, t + dt).real();
+    }
+    for (int i = 0; i < ns; i++) out[i] = a0 * (real(1.0) - (real)w[i]) + (real)w[i] * a1;
+}
+
+template <typename real>
+int rk4_launch(phx_solver* s, double t_pre, double dt) {
+    auto& c = s->cfg;
+    phx::Rk4Args<real> a{};
+    using v2 = typename phx::Vec2<real>::type;
+    const int nxt = s->cur ^ 1;
+    a.psi_in = (const v2*)s->psi[0][s->cur];
+    a.psi_out = (v2*)s->psi[0][nxt];
+    a.n_in = c.use_reservoir ? (const real*)s->n[0][s->cur] : nullptr;
+    a.n_out = c.use_reservoir ? (real*)s->n[0][nxt] : nullptr;
+    a.pump = (const real*)s->pump[0];
+    a.pot = (const real*)s->pot[0];
+    a.Nc = c.N_c;
+    a.Nr = c.N_r;
+    a.periodic_x = c.periodic_x;
+    a.periodic_y = c.periodic_y;
+    a.dt = (real)dt;
+    static const double w[4] = {0.0, 0.5, 0.5, 1.0};
+    stage_amps<real>(c.pump, t_pre, dt, w, 4, a.pamp);
+    stage_amps<real>(c.potential, t_pre, dt, w, 4, a.vamp);
+    a.ph = phx::make_phys<real>(c);
+    PHX_HIP(phx::launch_rk4_scalar<real>(a, s->ty, s->stream));
+    s->cur = nxt;
+    return PHX_OK;
+}
+
+/* Solver::iterate (solver.cu:22-70).  Returns 1 if a step was taken, 0 at t_max, <0 on error. */
+template <typename real>
+int iterate_t(phx_solver* s, bool force_fixed) {
+    (void)force_fixed;
+    auto& c = s->cfg;
+    if (s->t >= c.t_max) return 0;
+    const double t_pre = s->t;
+    s->t = s->rnd(s->t + s->dt); /* t += dt before the step (solver.cu:50) */
+    if (int r = rk4_launch<real>(s, t_pre, s->dt)) return r;
+    s->iteration++;
+    if (s->t - s->fft_cached_t < c.fft_every) return 1; /* FFT guard (solver.cu:62-63) */
+    s->fft_cached_t = s->t;
+    /* without a mask the reference only transforms for output (solver_fft.cu:78-79) */
+    return 1;
+}
+
+int iterate(phx_solver* s, bool force_fixed) {
+    return s->precision == PHX_FP32 ? iterate_t<float>(s, force_fixed) : iterate_t<double>(s, force_fixed);
+}
+
+int field_info(phx_solver* s, int field, void*** slot, size_t* elems) {
+    const size_t N = s->cells();
+    const int k = (field & 1);
+    if (k == 1 && !s->cfg.tetm && field <= PHX_POTENTIAL_MINUS) return fail(PHX_ERR_INVALID, "minus component requires -tetm");
+    static void* none = nullptr;
+    switch (field) {
+        case PHX_PSI_PLUS:
+        case PHX_PSI_MINUS: *slot = &s->psi[k][s->cur]; *elems = 2 * N; break;
+        case PHX_N_PLUS:
+        case PHX_N_MINUS: *slot = &s->n[k][s->cur]; *elems = N; break;
+        case PHX_PUMP_PLUS:
+        case PHX_PUMP_MINUS: *slot = &s->pump[k]; *elems = N; break;
+        case PHX_POTENTIAL_PLUS:
+        case PHX_POTENTIAL_MINUS: *slot = &s->pot[k]; *elems = N; break;
+        case PHX_FFT_MASK_PLUS:
+        case PHX_FFT_MASK_MINUS: *slot = &s->mask[k]; *elems = N; break;
+        default: *slot = &none; return fail(PHX_ERR_INVALID, "unknown field id " + std::to_string(field));
+    }
+    if (**slot == nullptr) return fail(PHX_ERR_INVALID, "field " + std::to_string(field) + " is not allocated in this configuration");
+    return PHX_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* phx_last_error(void) { return g_err.c_str(); }
+
+const char* phx_version(void) { return "phoenix_amd 0.1 (gfx950)"; }
+
+int phx_create(phx_solver** out, int precision, int argc, const char* const* argv) {
+    if (!out) return fail(PHX_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (precision != PHX_FP32 && precision != PHX_FP64) return fail(PHX_ERR_INVALID, "precision must be PHX_FP32 or PHX_FP64");
+    std::vector<std::string> args;
+    for (int i = 0; i < argc; i++) args.push_back(argv[i] ? argv[i] : "");
+    if (args.empty()) args.push_back("phoenix");
+    auto* s = new phx_solver();
+    s->precision = precision;
+    s->rsize = precision == PHX_FP32 ? 4 : 8;
+    std::string e = phx::parse_config(s->cfg, args, precision == PHX_FP32);
+    if (e.empty()) e = phx::finalize_config(s->cfg);
+    if (!e.empty()) {
+        delete s;
+        return fail(e.find("not provided") != std::string::npos || e.find("not yet") != std::string::npos ? PHX_ERR_UNSUPPORTED : PHX_ERR_INVALID, e);
+    }
+    *out = s;
+    g_err.clear();
+    return PHX_OK;
+}
+
+int phx_initialize(phx_solver* s) {
+    if (!s) return fail(PHX_ERR_INVALID, "null handle");
+    if (s->initialized) return fail(PHX_ERR_STATE, "already initialized");
+    if (!s->own_stream) PHX_HIP(hipStreamCreate(&s->own_stream));
+    if (!s->stream) s->stream = s->own_stream;
+    return s->precision == PHX_FP32 ? initialize_t<float>(s) : initialize_t<double>(s);
+}
+
+int phx_step(phx_solver* s, int64_t n_steps, int64_t* n_done) {
+    if (!s || !s->initialized) return fail(PHX_ERR_STATE, "phx_step before phx_initialize");
+    int64_t done = 0;
+    for (int64_t i = 0; i < n_steps; i++) {
+        int r = iterate(s, false);
+        if (r < 0) {
+            if (n_done) *n_done = done;
+            return r;
+        }
+        if (r == 0) break;
+        done++;
+    }
+    if (n_done) *n_done = done;
+    return PHX_OK;
+}
+
+/* main.cu:80-117 (render disabled): iterate in chunks of output_every, shortening the
+   step that would overshoot the next output time. */
+int phx_run(phx_solver* s, int64_t* n_done) {
+    if (!s || !s->initialized) return fail(PHX_ERR_STATE, "phx_run before phx_initialize");
+    auto& c = s->cfg;
+    int64_t done = 0;
+    int64_t out_iter = 1;
+    while (s->t < c.t_max) {
+        while (s->t < out_iter * c.output_every) {
+            const double dt = s->dt;
+            bool forced = false;
+            if (s->t + s->dt > out_iter * c.output_every) {
+                const double next_dt = out_iter * c.output_every - s->t;
+                if (next_dt > 0) {
+                    s->dt = s->rnd(next_dt);
+                    forced = true;
+                }
+            }
+            int r = iterate(s, forced);
+            if (forced) s->dt = dt;
+            if (r < 0) {
+                if (n_done) *n_done = done;
+                return r;
+            }
+            if (r == 0) break;
+            done++;
+        }
+        out_iter++;
+    }
+    if (n_done) *n_done = done;
+    return PHX_OK;
+}
+
+int phx_apply_fft_filter(phx_solver* s) {
+    if (!s || !s->initialized) return fail(PHX_ERR_STATE, "not initialized");
+    return fail(PHX_ERR_UNSUPPORTED, "the FFT filter is not yet provided by this library");
+}
+
+int phx_get_field(phx_solver* s, int field, void* host, size_t bytes) {
+    if (!s || !s->initialized) return fail(PHX_ERR_STATE, "not initialized");
+    void** slot;
+    size_t elems;
+    if (int r = field_info(s, field, &slot, &elems)) return r;
+    if (bytes != elems * s->rsize) return fail(PHX_ERR_INVALID, "byte count mismatch: expected " + std::to_string(elems * s->rsize));
+    PHX_HIP(hipMemcpyAsync(host, *slot, bytes, hipMemcpyDeviceToHost, s->stream));
+    PHX_HIP(hipStreamSynchronize(s->stream));
+    return PHX_OK;
+}
+
+int phx_set_field(phx_solver* s, int field, const void* host, size_t bytes) {
+    if (!s || !s->initialized) return fail(PHX_ERR_STATE, "not initialized");
+    void** slot;
+    size_t elems;
+    if (int r = field_info(s, field, &slot, &elems)) return r;
+    if (bytes != elems * s->rsize) return fail(PHX_ERR_INVALID, "byte count mismatch: expected " + std::to_string(elems * s->rsize));
+    PHX_HIP(hipMemcpyAsync(*slot, host, bytes, hipMemcpyHostToDevice, s->stream));
+    PHX_HIP(hipStreamSynchronize(s->stream));
+    return PHX_OK;
+}
+
+int phx_device_ptr(phx_solver* s, int field, void** dptr) {
+    if (!s || !s->initialized || !dptr) return fail(PHX_ERR_STATE, "not initialized");
+    void** slot;
+    size_t elems;
+    if (int r = field_info(s, field, &slot, &elems)) return r;
+    *dptr = *slot;
+    return PHX_OK;
+}
+
+int phx_get_time(phx_solver* s, double* t, double* dt) {
+    if (!s) return fail(PHX_ERR_INVALID, "null handle");
+    if (t) *t = s->initialized ? s->t : s->cfg.t;
+    if (dt) *dt = s->initialized ? s->dt : s->cfg.dt;
+    return PHX_OK;
+}
+
+int phx_set_time(phx_solver* s, double t, double dt) {
+    if (!s) return fail(PHX_ERR_INVALID, "null handle");
+    if (dt <= 0) return fail(PHX_ERR_INVALID, "dt must be positive");
+    s->t = s->rnd(t);
+    s->dt = s->rnd(dt);
+    s->cfg.t = s->t;
+    s->cfg.dt = s->dt;
+    return PHX_OK;
+}
+
+int phx_get_info(phx_solver* s, phx_info* info) {
+    if (!s || !info) return fail(PHX_ERR_INVALID, "null argument");
+    const auto& c = s->cfg;
+    std::memset(info, 0, sizeof(*info));
+    info->N_c = c.N_c;
+    info->N_r = c.N_r;
+    info->precision = s->precision;
+    info->tetm = c.tetm;
+    info->use_reservoir = c.use_reservoir;
+    info->use_pump = c.use_pumps;
+    info->use_potential = c.use_potentials;
+    info->use_stochastic = c.use_stochastic;
+    info->use_fft_mask = c.use_fft_mask;
+    info->adaptive = c.adaptive;
+    info->periodic_x = c.periodic_x;
+    info->periodic_y = c.periodic_y;
+    info->halo = c.halo;
+    std::snprintf(info->iterator, sizeof(info->iterator), "%s", c.iterator.c_str());
+    info->dx = c.dx;
+    info->dy = c.dy;
+    info->dt = s->initialized ? s->dt : c.dt;
+    info->t = s->initialized ? s->t : c.t;
+    info->t_max = c.t_max;
+    info->fft_every = c.fft_every;
+    info->tolerance = c.tolerance;
+    info->dt_min = c.dt_min;
+    info->dt_max = c.dt_max;
+    info->iteration = s->iteration;
+    info->rejected = s->rejected;
+    return PHX_OK;
+}
+
+int phx_set_stream(phx_solver* s, void* stream) {
+    if (!s) return fail(PHX_ERR_INVALID, "null handle");
+    s->stream = stream ? (hipStream_t)stream : s->own_stream;
+    return PHX_OK;
+}
+
+int phx_sync(phx_solver* s) {
+    if (!s) return fail(PHX_ERR_INVALID, "null handle");
+    if (s->stream) PHX_HIP(hipStreamSynchronize(s->stream));
+    return PHX_OK;
+}
+
+void phx_destroy(phx_solver* s) {
+    if (!s) return;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (void* p : s->allocs) (void)hipFree(p);
+    if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
+    delete s;
+}
+
+double phx_physics_defaults(phx_physics* ph, int N_c, int N_r, double L_x, double L_y) {
+    phx::SystemConfig c;
+    c.N_c = N_c;
+    c.N_r = N_r;
+    c.L_x = L_x;
+    c.L_y = L_y;
+    c.t_max = 1e30;
+    phx::finalize_config(c);
+    ph->dx = c.dx;
+    ph->dy = c.dy;
+    ph->dV = c.dV;
+    ph->m_eff_scaled = c.m_eff_scaled;
+    ph->one_over_h_bar_s = c.one_over_h_bar_s;
+    ph->gamma_c = c.gamma_c;
+    ph->gamma_r = c.gamma_r;
+    ph->g_c = c.g_c;
+    ph->g_r = c.g_r;
+    ph->R = c.R;
+    ph->g_pm = c.g_pm;
+    ph->delta_LT = c.delta_LT;
+    return c.magic_timestep;
+}
+
+} /* extern "C" */
+
+static phx::SystemConfig cfg_from_physics(const phx_physics* ph) {
+    phx::SystemConfig c;
+    c.dx = ph->dx;
+    c.dy = ph->dy;
+    c.dV = ph->dV;
+    c.one_over_dx2 = 1.0 / (ph->dx * ph->dx);
+    c.one_over_dy2 = 1.0 / (ph->dy * ph->dy);
+    c.m2_over_dx2_p_dy2 = -2.0 * (c.one_over_dx2 + c.one_over_dy2);
+    c.m_eff_scaled = ph->m_eff_scaled;
+    c.one_over_h_bar_s = ph->one_over_h_bar_s;
+    c.gamma_c = ph->gamma_c;
+    c.gamma_r = ph->gamma_r;
+    c.g_c = ph->g_c;
+    c.g_r = ph->g_r;
+    c.R = ph->R;
+    c.g_pm = ph->g_pm;
+    c.delta_LT = ph->delta_LT;
+    return c;
+}
+
+template <typename real>
+static int rk4_device_t(const phx_physics* ph, int N_c, int N_r, int px, int py, void* psi_a, void* n_a, void* psi_b, void* n_b,
+                        const void* pump, const void* pot, double dt, int64_t n_steps, hipStream_t st) {
+    using v2 = typename phx::Vec2<real>::type;
+    const phx::SystemConfig c = cfg_from_physics(ph);
+    phx::Rk4Args<real> a{};
+    a.Nc = N_c;
+    a.Nr = N_r;
+    a.periodic_x = px;
+    a.periodic_y = py;
+    a.dt = (real)dt;
+    for (int i = 0; i < 4; i++) a.pamp[i] = a.vamp[i] = 1;
+    a.ph = phx::make_phys<real>(c);
+    a.pump = (const real*)pump;
+    a.pot = (const real*)pot;
+    int ty = 16;
+    if (const char* e = std::getenv("PHX_RK4_TY")) ty = std::atoi(e);
+    for (int64_t k = 0; k < n_steps; k++) {
+        const bool even = (k % 2) == 0;
+        a.psi_in = (const v2*)(even ? psi_a : psi_b);
+        a.psi_out = (v2*)(even ? psi_b : psi_a);
+        a.n_in = n_a ? (const real*)(even ? n_a : n_b) : nullptr;
+        a.n_out = n_a ? (real*)(even ? n_b : n_a) : nullptr;
+        PHX_HIP(phx::launch_rk4_scalar<real>(a, ty, st));
+    }
+    if (n_steps % 2 == 1) {
+        PHX_HIP(hipMemcpyAsync(psi_a, psi_b, 2 * sizeof(real) * (size_t)N_c * N_r, hipMemcpyDeviceToDevice, st));
+        if (n_a) PHX_HIP(hipMemcpyAsync(n_a, n_b, sizeof(real) * (size_t)N_c * N_r, hipMemcpyDeviceToDevice, st));
+    }
+    return PHX_OK;
+}
+
+extern "C" int phx_rk4_scalar_device(int precision, const phx_physics* ph, int N_c, int N_r, int periodic_x, int periodic_y, void* psi_a,
+                          void* n_a, void* psi_b, void* n_b, const void* pump, const void* pot, double dt, int64_t n_steps,
+                          void* stream) {
+    if (!ph || !psi_a || !psi_b || N_c <= 0 || N_r <= 0 || n_steps < 0) return fail(PHX_ERR_INVALID, "invalid argument");
+    if ((n_a == nullptr) != (n_b == nullptr)) return fail(PHX_ERR_INVALID, "n_a and n_b must both be given or both be NULL");
+    if (pump && !n_a) return fail(PHX_ERR_INVALID, "a pump needs a reservoir");
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == PHX_FP32) return rk4_device_t<float>(ph, N_c, N_r, periodic_x, periodic_y, psi_a, n_a, psi_b, n_b, pump, pot, dt, n_steps, st);
+    if (precision == PHX_FP64) return rk4_device_t<double>(ph, N_c, N_r, periodic_x, periodic_y, psi_a, n_a, psi_b, n_b, pump, pot, dt, n_steps, st);
+    return fail(PHX_ERR_INVALID, "precision");
+}
+

@@ -1,0 +1,116 @@
+"""C ABI and host-side logic, CPU only (no device calls).
+
+- libphx.so loads and exports every entry point declared in include/phx.h;
+- phx_create parses PHOENIX flags like the reference (aliases, first occurrence
+  wins, --config files, FP32 rounding of numbers, derived constants, validation
+  errors as status codes instead of exit(1), unknown iterators rejected).
+"""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    txt = open(os.path.join(ROOT, "include", "phx.h")).read()
+    return sorted(set(re.findall(r"\b(phx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from phoenix_amd import _lib
+    L = _lib.lib()
+    syms = header_symbols()
+    assert syms, "no declarations found"
+    for s in syms:
+        assert hasattr(L, s), f"libphx.so does not export {s}"
+    assert sorted(_lib.EXPORTED) == syms
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    for s in syms:
+        assert re.search(rf"\bT {s}\b", out), s
+
+
+def make(args, prec="fp64"):
+    from phoenix_amd._lib import Solver
+    return Solver([str(a) for a in args], prec, initialize=False)
+
+
+def test_defaults_and_derived_constants():
+    s = make(["--N", 100, 60, "--L", 99, 59])
+    i = s.info
+    assert (i["N_c"], i["N_r"]) == (100, 60)
+    assert i["dx"] == pytest.approx(1.0) and i["dy"] == pytest.approx(1.0)  # dx = L/(N-1)
+    assert i["dt"] == pytest.approx(0.5 * 1.0 * 1.0)  # magic time step 0.5 dx dy
+    assert i["iterator"] == "RK4" and i["halo"] == 4
+    assert not i["use_reservoir"]  # no pump, no initial reservoir
+    assert i["t_max"] == 1000 and i["fft_every"] == 1
+
+
+def test_aliases_and_first_occurrence_wins():
+    s = make(["--gridsize", 64, 32, "--N", 16, 16, "--tstep", 0.02, "--dt", 0.5, "--boundary", "periodic", "zero"])
+    i = s.info
+    assert (i["N_c"], i["N_r"]) == (64, 32)
+    assert i["dt"] == 0.02
+    assert i["periodic_x"] == 1 and i["periodic_y"] == 0
+
+
+def test_fp32_rounds_numbers():
+    s = make(["--N", 32, 32, "--tstep", 0.1, "--tmax", 10], prec="fp32")
+    assert s.info["dt"] == float(np.float32(0.1))
+    d = make(["--N", 32, 32, "--tstep", 0.1, "--tmax", 10], prec="fp64")
+    assert d.info["dt"] == 0.1
+
+
+def test_config_file_tokens_append_and_cli_wins(tmp_path):
+    cfg = tmp_path / "run.cfg"
+    cfg.write_text("# comment line\n--N 48 48 # trailing comment\n--tmax 5\n--tstep 0.01\n")
+    s = make(["--config", str(cfg), "--tmax", 2])
+    i = s.info
+    assert (i["N_c"], i["N_r"]) == (48, 48)
+    assert i["t_max"] == 2  # command line comes first, first occurrence wins
+    assert i["dt"] == 0.01
+
+
+def test_envelope_flags_enable_reservoir_and_pump():
+    s = make(["--N", 32, 32, "--pump", 5.0, "add", 10, 10, 0, 0, "both", 1, "none", "gauss+noDivide"])
+    i = s.info
+    assert i["use_reservoir"] and i["use_pump"]
+    s = make(["--N", 32, 32, "--pump", 5.0, "add", "width", 10, 10, "pos", 0, 0, "-noReservoir"])
+    assert not s.info["use_reservoir"]
+
+
+@pytest.mark.parametrize("args,code", [
+    (["--N", 31, 32], -1),               # odd dimension
+    (["--N", 32, 32, "--tstep", 0], -1),  # dt <= 0
+    (["--N", 32, 32, "--tstep", 2, "--tmax", 1], -1),  # dt > t_max
+    (["--iterator", "RK3"], -3),          # not provided by this library
+    (["--imagTime", 1.0], -3),
+    (["--N", "abc", 32], -1),
+    (["--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "notatype"], -1),
+])
+def test_invalid_input_returns_status(args, code):
+    from phoenix_amd._lib import PhxError
+    with pytest.raises(PhxError) as e:
+        make(args)
+    assert e.value.code == code
+
+
+def test_adaptive_flag_only_with_adaptive_iterator():
+    assert make(["--N", 32, 32, "-adaptive"]).info["adaptive"] == 0  # RK4 cannot adapt (solver.cu:23-26)
+    i = make(["--N", 32, 32, "--iterator", "DP45", "-adaptive", "--tol", 1e-6, "--rkvdt", 1e-4, 0.1]).info
+    assert i["adaptive"] == 1 and i["halo"] == 7 and i["tolerance"] == 1e-6 and i["dt_max"] == 0.1
+
+
+def test_physics_defaults_match_reference_constants():
+    from phoenix_amd._lib import physics_defaults
+    ph, magic = physics_defaults(256, 256, 127.5, 127.5)
+    hbar_s = 1.0545718e-34 / 1.60217662e-19 * 1e12
+    m_eff = 9.10938356e-31 / 1.60217662e-19 * 1e8
+    assert ph.one_over_h_bar_s == pytest.approx(1 / hbar_s, rel=1e-15)
+    assert ph.m_eff_scaled == pytest.approx(-0.5 * hbar_s ** 2 / m_eff, rel=1e-15)
+    assert ph.dx == 0.5 and magic == pytest.approx(0.125)
+    # gamma_r = 1.5 * gamma_c and g_r = 2 * g_c as written in system.cpp:29,31
+    assert (ph.gamma_c, ph.gamma_r, ph.g_c, ph.g_r, ph.R) == (0.15, 1.5 * 0.15, 3e-6, 2 * 3e-6, 0.01)
