@@ -26,12 +26,17 @@ ARCH = os.environ.get("PHX_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
-# Packed FP32 VALU ops stay disabled on every device pass.
-HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-result",
+# Packed FP32 VALU ops stay disabled on every device pass.  No automatic FMA contraction:
+# the kernels write their fused multiply-adds explicitly (phx_fma), so results are
+# bitwise independent of the launch shape.
+HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-result", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include")]
+# The streaming kernels keep their row windows in registers only when the row loop is
+# unrolled completely; the 7-stage spinor kernels exceed LLVM's default pragma threshold.
+HIP_FLAGS_FILE = {"phx_rk_generic.hip": ["-mllvm", "-pragma-unroll-threshold=2000000"]}
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["phx_rk4.hip", "phx_solver.hip"]
+HIP_SOURCES = ["phx_rk4.hip", "phx_rk_generic.hip", "phx_fft.hip", "phx_solver.hip"]
 CXX_SOURCES = ["phx_config.cpp"]
 
 
@@ -50,7 +55,7 @@ def build(verbose: bool = False) -> str:
     objs = []
     for s in HIP_SOURCES:
         o = os.path.join(BUILD, s.replace(".hip", ".o"))
-        jobs.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        jobs.append([HIPCC, *HIP_FLAGS, *HIP_FLAGS_FILE.get(s, []), "-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     for s in CXX_SOURCES:
         o = os.path.join(BUILD, s.replace(".cpp", ".o"))

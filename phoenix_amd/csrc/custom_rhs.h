@@ -24,6 +24,11 @@
 
 #define PHX_CUSTOM_PARAMETERS(X) /* X(custom_var, 0.0) */
 
+/* Fused multiply-add written out: the kernels are compiled without automatic
+   contraction, so every cell rounds identically whatever the tile shape. */
+__host__ __device__ __forceinline__ float phx_fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__host__ __device__ __forceinline__ double phx_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 template <typename real>
 struct PhxPhys {
     real m_eff_scaled;      /* -hbar_s^2 / (2 m_eff)                  (system.cpp:151) */
@@ -32,32 +37,54 @@ struct PhxPhys {
     real one_over_dx2, one_over_dy2;
     real dx, dy, dV;
     real gamma_c, gamma_r, g_c, g_r, R, g_pm, delta_LT;
+    /* folded constants, filled on the host from the members above (phx_fold) */
+    real kin_c, kin_x, kin_y; /* (m~/hbar) * {C, 1/dx^2, 1/dy^2}: kinetic stencil weights */
+    real gc_h, gr_h, gpm_h;   /* g_c/hbar, g_r/hbar, g_pm/hbar */
+    real lt_h, lt_hxy;        /* delta_LT/hbar, delta_LT/hbar * 1/2 /dx/dy */
+    real half_R, mhalf_gamma_c; /* R/2, -gamma_c/2 */
 #define PHX_X(name, def) real name;
     PHX_CUSTOM_PARAMETERS(PHX_X)
 #undef PHX_X
 };
 
+/* Derive the folded constants (host side, in double before rounding to real). */
+template <typename real>
+inline void phx_fold(PhxPhys<real>& p, double m_eff_scaled, double one_over_h_bar_s, double m2, double idx2, double idy2,
+                     double dx, double dy) {
+    const double hk = m_eff_scaled * one_over_h_bar_s;
+    p.kin_c = (real)(hk * m2);
+    p.kin_x = (real)(hk * idx2);
+    p.kin_y = (real)(hk * idy2);
+    p.gc_h = (real)((double)p.g_c * one_over_h_bar_s);
+    p.gr_h = (real)((double)p.g_r * one_over_h_bar_s);
+    p.gpm_h = (real)((double)p.g_pm * one_over_h_bar_s);
+    p.lt_h = (real)((double)p.delta_LT * one_over_h_bar_s);
+    p.lt_hxy = (real)((double)p.delta_LT * one_over_h_bar_s * 0.5 / dx / dy);
+    p.half_R = (real)(0.5 * (double)p.R);
+    p.mhalf_gamma_c = (real)(-0.5 * (double)p.gamma_c);
+}
+
 /*
  * Scalar model (kernel_gp_compute.cuh:50-110, non-stochastic branch):
  *   k_psi = -i/hbar [ m~ lap + (g_c|psi|^2 + g_r n + V) psi ] + 1/2 (R n - gamma_c) psi
  *   k_n   = -(gamma_r + R|psi|^2) n + P
- * (pr, pi) = psi, (lr, li) = C psi + (N+S)/dy^2 + (E+W)/dx^2, n reservoir,
- * pump = sum_k P_k a_k(w), pot = sum_k V_k a_k(w).
+ * Inputs: psi = (pr, pi); (tr, ti) = (m~/hbar) lap psi with
+ * lap psi = C psi + (N+S)/dy^2 + (E+W)/dx^2 (the kernel evaluates it with the
+ * kin_* weights); n reservoir; pump = sum_k P_k a_k(w); pot_h = sum_k V_k a_k(w) / hbar.
  * Without a reservoir the caller passes n = 0 and ignores kn.
  */
 template <typename real>
-__host__ __device__ __forceinline__ void phx_rhs_scalar(const PhxPhys<real>& p, real pr, real pi, real lr, real li,
-                                                        real n, real pump, real pot, real& kr, real& ki, real& kn) {
-    const real nrm = pr * pr + pi * pi;
-    const real hk = p.m_eff_scaled * p.one_over_h_bar_s;
-    /* real potential felt by psi, divided by hbar */
-    const real a = p.one_over_h_bar_s * (p.g_c * nrm + p.g_r * n + pot);
+__host__ __device__ __forceinline__ void phx_rhs_scalar(const PhxPhys<real>& p, real pr, real pi, real tr, real ti,
+                                                        real n, real pump, real pot_h, real& kr, real& ki, real& kn) {
+    const real nrm = phx_fma(pr, pr, pi * pi);
+    /* real potential felt by psi, over hbar */
+    const real a = phx_fma(p.gc_h, nrm, phx_fma(p.gr_h, n, pot_h));
     /* gain - loss */
-    const real b = real(0.5) * (p.R * n - p.gamma_c);
-    /* -i (hk*lap + a*psi) + b*psi */
-    kr = hk * li + a * pi + b * pr;
-    ki = -(hk * lr) - a * pr + b * pi;
-    kn = -(p.gamma_r + p.R * nrm) * n + pump;
+    const real b = phx_fma(p.half_R, n, p.mhalf_gamma_c);
+    /* -i (T + a psi) + b psi */
+    kr = phx_fma(b, pr, phx_fma(a, pi, ti));
+    ki = phx_fma(b, pi, phx_fma(-a, pr, -tr));
+    kn = phx_fma(phx_fma(-p.R, nrm, -p.gamma_r), n, pump);
 }
 
 /*
@@ -65,19 +92,19 @@ __host__ __device__ __forceinline__ void phx_rhs_scalar(const PhxPhys<real>& p, 
  *   k_psi = -i/hbar [ m~ lap_s + (g_c|psi_s|^2 + g_r n_s + V_s + g_pm|psi_o|^2) psi_s + delta_LT cross_s ]
  *           + 1/2 (R n_s - gamma_c) psi_s
  *   k_n   = -(gamma_r + R|psi_s|^2) n_s + P_s
- * (cr, ci) = cross_s = (E+W)_o/dx^2 - (N+S)_o/dy^2 +- (i/2)/(dx dy) D_o, computed by the caller
- * (the centre terms cancel only for dx == dy; the reference omits them, and so does this hook).
+ * (cr, ci) = delta_LT/hbar * cross_s with
+ * cross_s = (E+W)_o/dx^2 - (N+S)_o/dy^2 +- (i/2)/(dx dy) D_o (o = other component),
+ * evaluated by the caller; the centre terms cancel only for dx == dy, and the
+ * reference omits them, as does this hook.
  */
 template <typename real>
-__host__ __device__ __forceinline__ void phx_rhs_tetm(const PhxPhys<real>& p, real pr, real pi, real lr, real li,
-                                                      real cr, real ci, real other_norm, real n, real pump, real pot,
+__host__ __device__ __forceinline__ void phx_rhs_tetm(const PhxPhys<real>& p, real pr, real pi, real tr, real ti,
+                                                      real cr, real ci, real other_norm, real n, real pump, real pot_h,
                                                       real& kr, real& ki, real& kn) {
-    const real nrm = pr * pr + pi * pi;
-    const real hk = p.m_eff_scaled * p.one_over_h_bar_s;
-    const real hl = p.delta_LT * p.one_over_h_bar_s;
-    const real a = p.one_over_h_bar_s * (p.g_c * nrm + p.g_r * n + pot + p.g_pm * other_norm);
-    const real b = real(0.5) * (p.R * n - p.gamma_c);
-    kr = hk * li + hl * ci + a * pi + b * pr;
-    ki = -(hk * lr) - hl * cr - a * pr + b * pi;
-    kn = -(p.gamma_r + p.R * nrm) * n + pump;
+    const real nrm = phx_fma(pr, pr, pi * pi);
+    const real a = phx_fma(p.gpm_h, other_norm, phx_fma(p.gc_h, nrm, phx_fma(p.gr_h, n, pot_h)));
+    const real b = phx_fma(p.half_R, n, p.mhalf_gamma_c);
+    kr = phx_fma(b, pr, phx_fma(a, pi, ti + ci));
+    ki = phx_fma(b, pi, phx_fma(-a, pr, -tr - cr));
+    kn = phx_fma(phx_fma(-p.R, nrm, -p.gamma_r), n, pump);
 }

@@ -630,6 +630,7 @@ PhxPhys<real> make_phys(const SystemConfig& c) {
 #define PHX_X(name, def) p.name = (real)c.name;
     PHX_CUSTOM_PARAMETERS(PHX_X)
 #undef PHX_X
+    phx_fold<real>(p, c.m_eff_scaled, c.one_over_h_bar_s, c.m2_over_dx2_p_dy2, c.one_over_dx2, c.one_over_dy2, c.dx, c.dy);
     return p;
 }
 

@@ -34,12 +34,67 @@ struct Rk4Args {
     int periodic_x, periodic_y;
     int nstrips, nbands; /* set by the launcher */
     real dt;
-    real pamp[4], vamp[4]; /* temporal amplitude per stage: a(t)(1-w)+w a(t+dt) */
+    real pamp[4];   /* pump temporal amplitude per stage: a(t)(1-w)+w a(t+dt) */
+    real vamp_h[4]; /* potential temporal amplitude per stage, divided by hbar */
     PhxPhys<real> ph;
 };
 
-/* ty: output rows per wavefront tile (8, 16 or 32). */
+/* Launch shape of the fused RK4 kernel.
+   ty: output rows per wavefront tile (8, 12, 16, 24);
+   ew: east/west exchange (0 = ds_bpermute shuffle, 1 / 2 = DPP wave shift, either lane direction);
+   pf: rows loaded ahead of first use (0 or 2). */
+struct Rk4Tune {
+    int ty = 16, ew = 1, pf = 2;
+};
+
 template <typename real>
-hipError_t launch_rk4_scalar(const Rk4Args<real>& a, int ty, hipStream_t s);
+hipError_t launch_rk4_scalar(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s);
+
+/* Arguments of the generic fused Runge-Kutta kernel (phx_rk_generic.hip).
+   Index 0 = plus component, 1 = minus component (TE-TM only). */
+template <typename real>
+struct RkGenArgs {
+    const typename Vec2<real>::type* psi_in[2];
+    const real* n_in[2]; /* nullptr: no reservoir */
+    typename Vec2<real>::type* psi_out[2];
+    real* n_out[2];
+    const real* pump[2]; /* nullptr: no pump */
+    const real* pot[2];  /* nullptr: no potential */
+    int Nc, Nr;
+    int periodic_x, periodic_y;
+    int nstrips, nbands; /* set by the launcher */
+    int tetm, stochastic;
+    real dt;
+    /* temporal amplitudes at t and t+dt; a(w) = a(t) + w (a(t+dt) - a(t)) */
+    real amp_pump, amp_next_pump, amp_pot_h, amp_next_pot_h;
+    /* noise: xi = noise_amp * N2(seed, step, cell), noise_amp = stochastic amplitude * sqrt(dt) */
+    uint64_t noise_seed, noise_step;
+    real noise_amp;
+    double* partials; /* 2 doubles per wavefront (error, norm), adaptive tableaux only */
+    double* sums;     /* 2 doubles: reduced error and norm */
+    PhxPhys<real> ph;
+};
+
+enum class RkTableau { RK4, DP45_FIXED, DP45_ADAPTIVE };
+
+template <typename real>
+hipError_t launch_rk_generic(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
+
+/* wavefront tiles of the generic kernel for a grid (partials buffer size = 2 * this) */
+inline int rk_generic_waves(RkTableau tab, int Nc, int Nr) {
+    const int S = tab == RkTableau::RK4 ? 4 : tab == RkTableau::DP45_FIXED ? 6 : 7;
+    const int outw = 64 - 2 * S, ty = 16;
+    return ((Nc + outw - 1) / outw) * ((Nr + ty - 1) / ty);
+}
+
+/* k-space filter psi <- IDFT(DFT(psi) / (N_c N_r) * fftshift(mask)), in place.
+   tw_c / tw_r: forward twiddle tables exp(-2 pi i k / N) for k < N (complex, real
+   precision) of the row length N_c and column length N_r. */
+template <typename real>
+hipError_t launch_fft_filter(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, int Nc, int Nr,
+                             hipStream_t s);
+
+/* Power of two in [4, 4096] whose two LDS row buffers fit in 64 KiB. */
+bool fft_supported(int N, size_t real_size);
 
 } // namespace phx

@@ -9,19 +9,25 @@
  * Mapping (one wavefront = one tile, no inter-wave synchronisation):
  *   - a wave owns a strip of 64 extended columns (56 output columns plus a
  *     4-cell halo either side), one lane per column;
- *   - it streams TY+8 extended rows top to bottom; after loading row i it
+ *   - it streams TY+8 extended rows top to bottom; after row i is available it
  *     evaluates stage 1 at row i-1, stage 2 at i-2, stage 3 at i-3 and stage 4
  *     at i-4 and writes output row i-4.  Stage s is therefore evaluated on rows
- *     [s, TY+7-s] and (implicitly, through the lanes that later read it) on
- *     columns [s, 63-s]: the extents of GENERATE_SUBGRID_INDEX with
- *     current_halo = 4 - s (cuda_macro.cuh:43-49,112-113);
+ *     [s, TY+7-s] and is valid on columns [s, 63-s]: the extents that
+ *     GENERATE_SUBGRID_INDEX gives with current_halo = 4 - s
+ *     (cuda_macro.cuh:43-49,112-113);
  *   - north/south neighbours come from the register window of the previous
- *     stage, east/west neighbours from the adjacent lanes (cross-lane shuffle);
- *   - out-of-domain cells of a zero boundary are loaded as 0 (psi, n, P, V),
- *     which is what synchronize_halos writes into the reference's halos
- *     (kernel_halo.cuh:66-69); periodic boundaries wrap the load index.
- * HBM traffic per step and cell: read psi, n, P (, V) once, write psi, n once
- * (plus the 8-row / 8-column halo re-read, served from L2).
+ *     stage, east/west neighbours from the adjacent lanes through DPP wave
+ *     shifts (one VALU op, no LDS round trip);
+ *   - rows are read with raw buffer loads PF rows ahead of their first use.
+ *     Each row gets its own buffer descriptor (scalar registers); a row outside
+ *     a zero-boundary domain gets an empty descriptor and a column outside it an
+ *     out-of-range offset, so the hardware returns 0 for psi, n, P, V -- which is
+ *     what synchronize_halos writes into the reference's halos
+ *     (kernel_halo.cuh:66-69).  Periodic boundaries wrap the row / column index.
+ * Arithmetic uses explicit fused multiply-adds and no automatic contraction, so
+ * a cell's result does not depend on the tile shape or its place in the tile.
+ * HBM traffic per step and cell: read psi, n, P (, V) once, write psi, n once;
+ * the 8-row / 8-column halo re-reads are served from L2.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -36,9 +42,19 @@ constexpr int kHalo = 4;
 constexpr int kOutW = kWave - 2 * kHalo; /* 56 */
 constexpr int kWavesPerBlock = 4;
 
+/* DPP controls of the GFX9 wavefront shifts: lane l of the result reads lane
+   l+1 for wave_shl:1 and lane l-1 for wave_shr:1; the edge lane reads 0. */
+constexpr int kDppWaveShl1 = 0x130;
+constexpr int kDppWaveShr1 = 0x138;
+/* buffer descriptor word 3 for GFX9-family raw buffers */
+constexpr int kBufferFlags = 0x00020000;
+/* byte offset beyond any row: raw buffer loads there return 0 */
+constexpr uint32_t kOutOfRange = 0x80000000u;
+
 /* Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
    Remap so that each XCD receives a contiguous range of tiles: horizontally
-   adjacent tiles then share one L2 for their halo columns. */
+   adjacent tiles then share one L2 for their halo columns, and a tile stays on
+   the same XCD from step to step. */
 __device__ __forceinline__ int xcd_contiguous(int b, int nblocks) {
     const int q = nblocks >> 3, r = nblocks & 7;
     const int x = b & 7;
@@ -46,16 +62,64 @@ __device__ __forceinline__ int xcd_contiguous(int b, int nblocks) {
     return start + (b >> 3);
 }
 
-template <typename real>
-__device__ __forceinline__ real lane_from(real v, int src) {
-    return __shfl(v, src, kWave);
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    int2 x = __builtin_bit_cast(int2, v);
+    x.x = __builtin_amdgcn_mov_dpp(x.x, CTRL, 0xF, 0xF, true);
+    x.y = __builtin_amdgcn_mov_dpp(x.y, CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, x);
 }
 
-template <typename real, int TY, bool RES, bool PUMP, bool POT>
+/* sum of the east (lane+1) and west (lane-1) neighbours.
+   EW = 0: ds_bpermute shuffles; EW = 1: DPP wave shifts. */
+template <int EW, typename real>
+__device__ __forceinline__ real east_plus_west(real v, int lane) {
+    if constexpr (EW == 0)
+        return __shfl(v, lane + 1, kWave) + __shfl(v, lane - 1, kWave);
+    else
+        return dpp_move<kDppWaveShl1>(v) + dpp_move<kDppWaveShr1>(v);
+}
+
+/* keep a uniform value in a vector register (an FMA may read one scalar operand only) */
+template <typename T>
+__device__ __forceinline__ T in_vgpr(T x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t row_rsrc(const void* row, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), 0, (int)bytes, kBufferFlags);
+}
+__device__ __forceinline__ void ld_real(rsrc_t r, uint32_t off, float& x) {
+    x = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void ld_real(rsrc_t r, uint32_t off, double& x) {
+    x = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+/* The vector results of the raw buffer builtins are reinterpreted as a whole:
+   element access on them (v[1]) compiles to a single-dword load on this
+   toolchain and returns the first element twice. */
+__device__ __forceinline__ void ld_cplx(rsrc_t r, uint32_t off, float& re, float& im) {
+    const float2 v = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    re = v.x;
+    im = v.y;
+}
+__device__ __forceinline__ void ld_cplx(rsrc_t r, uint32_t off, double& re, double& im) {
+    const double2 v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    re = v.x;
+    im = v.y;
+}
+
+template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>
 __global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
     using v2 = typename Vec2<real>::type;
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
     const int strip = tile % a.nstrips;
     const int band = (tile / a.nstrips) * kWavesPerBlock + wave;
@@ -63,18 +127,22 @@ __global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
     const int x0 = strip * kOutW;
     const int y0 = band * TY;
     const PhxPhys<real>& ph = a.ph;
+    const int Nc = a.Nc, Nr = a.Nr;
 
+    /* column of this lane (wrapped for periodic x); out-of-range offset outside a zero-BC domain */
     int gcol = x0 - kHalo + lane;
     bool col_in = true;
-    if (gcol < 0 || gcol >= a.Nc) {
+    if (gcol < 0 || gcol >= Nc) {
         if (a.periodic_x)
-            gcol = ((gcol % a.Nc) + a.Nc) % a.Nc;
-        else {
+            gcol = ((gcol % Nc) + Nc) % Nc;
+        else
             col_in = false;
-            gcol = 0;
-        }
     }
-    const bool out_lane = lane >= kHalo && lane < kWave - kHalo && (x0 - kHalo + lane) < a.Nc;
+    const uint32_t off_r = col_in ? (uint32_t)gcol * (uint32_t)sizeof(real) : kOutOfRange;
+    const uint32_t off_c = col_in ? (uint32_t)gcol * (uint32_t)sizeof(v2) : kOutOfRange;
+    const bool out_lane = lane >= kHalo && lane < kWave - kHalo && (x0 - kHalo + lane) < Nc;
+    const uint32_t rbytes = (uint32_t)Nc * (uint32_t)sizeof(real);
+    const uint32_t cbytes = (uint32_t)Nc * (uint32_t)sizeof(v2);
 
     constexpr int E = TY + 2 * kHalo;
     real p0r[E], p0i[E], n0[E], pp[E], vv[E];
@@ -84,58 +152,55 @@ __global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
 
     const real dth = a.dt * real(0.5);
     const real dt1 = a.dt;
-    const real w16 = real(1.0 / 6.0), w13 = real(1.0 / 3.0);
+    const real dt6 = a.dt / real(6);
+    PhxPhys<real> phv = ph;
+    phv.mhalf_gamma_c = in_vgpr(ph.mhalf_gamma_c);
+    phv.gamma_r = in_vgpr(ph.gamma_r);
+    const real kin_c = ph.kin_c, kin_x = ph.kin_x, kin_y = ph.kin_y;
 
-    /* stage evaluation: centre (cr,ci), north+south sum, reservoir value */
-    auto rhs = [&](real cr, real ci, real nsr, real nsi, real nv, real pump, real pot, real& kr, real& ki, real& kn) {
-        const real er = lane_from(cr, lane + 1), ei = lane_from(ci, lane + 1);
-        const real wr = lane_from(cr, lane - 1), wi = lane_from(ci, lane - 1);
-        const real lr = ph.m2_over_dx2_p_dy2 * cr + nsr * ph.one_over_dy2 + (er + wr) * ph.one_over_dx2;
-        const real li = ph.m2_over_dx2_p_dy2 * ci + nsi * ph.one_over_dy2 + (ei + wi) * ph.one_over_dx2;
-        phx_rhs_scalar<real>(ph, cr, ci, lr, li, RES ? nv : real(0), PUMP ? pump : real(0), POT ? pot : real(0), kr, ki, kn);
+    auto load_row = [&](int r) {
+        int grow = y0 - kHalo + r; /* wave-uniform */
+        bool in = true;
+        if (grow < 0 || grow >= Nr) {
+            if (a.periodic_y)
+                grow = ((grow % Nr) + Nr) % Nr;
+            else
+                in = false;
+        }
+        const size_t row = (size_t)(in ? grow : 0) * (size_t)Nc;
+        ld_cplx(row_rsrc(a.psi_in + row, in ? cbytes : 0u), off_c, p0r[r], p0i[r]);
+        if (RES) ld_real(row_rsrc(a.n_in + row, in ? rbytes : 0u), off_r, n0[r]);
+        if (PUMP) ld_real(row_rsrc(a.pump + row, in ? rbytes : 0u), off_r, pp[r]);
+        if (POT) ld_real(row_rsrc(a.pot + row, in ? rbytes : 0u), off_r, vv[r]);
+    };
+
+    /* one RHS evaluation: centre (cr, ci), north+south sums, reservoir, pump*amp, V*amp/hbar */
+    auto rhs = [&](real cr, real ci, real nsr, real nsi, real nv, real pmp, real poth, real& kr, real& ki, real& kn) {
+        const real ewr = east_plus_west<EW>(cr, lane);
+        const real ewi = east_plus_west<EW>(ci, lane);
+        const real tr = phx_fma(kin_x, ewr, phx_fma(kin_y, nsr, kin_c * cr));
+        const real ti = phx_fma(kin_x, ewi, phx_fma(kin_y, nsi, kin_c * ci));
+        phx_rhs_scalar<real>(phv, cr, ci, tr, ti, RES ? nv : real(0), PUMP ? pmp : real(0), POT ? poth : real(0), kr, ki, kn);
     };
 
 #pragma unroll
-    for (int it = 0; it < E; ++it) {
-        /* ---- load extended row `it` */
-        {
-            int grow = y0 - kHalo + it;
-            bool in = col_in;
-            if (grow < 0 || grow >= a.Nr) {
-                if (a.periodic_y)
-                    grow = ((grow % a.Nr) + a.Nr) % a.Nr;
-                else
-                    in = false;
-            }
-            real lr = 0, li = 0, ln = 0, lp = 0, lv = 0;
-            if (in) {
-                const size_t idx = (size_t)grow * (size_t)a.Nc + (size_t)gcol;
-                const v2 v = a.psi_in[idx];
-                lr = v.x;
-                li = v.y;
-                if (RES) ln = a.n_in[idx];
-                if (PUMP) lp = a.pump[idx];
-                if (POT) lv = a.pot[idx];
-            }
-            p0r[it] = lr;
-            p0i[it] = li;
-            n0[it] = ln;
-            pp[it] = lp;
-            vv[it] = lv;
-        }
+    for (int it = -PF; it < E; ++it) {
+        if (it + PF < E) load_row(it + PF);
+        if (it < 0) continue;
         /* ---- stage 1 at row e = it-1, input psi0 */
         {
             const int e = it - 1;
             if (e >= 1 && e <= E - 2) {
                 real kr, ki, kn;
-                rhs(p0r[e], p0i[e], p0r[e - 1] + p0r[e + 1], p0i[e - 1] + p0i[e + 1], n0[e], pp[e] * a.pamp[0], vv[e] * a.vamp[0], kr, ki, kn);
-                y1r[e] = p0r[e] + dth * kr;
-                y1i[e] = p0i[e] + dth * ki;
-                acr[e] = w16 * kr;
-                aci[e] = w16 * ki;
+                rhs(p0r[e], p0i[e], p0r[e - 1] + p0r[e + 1], p0i[e - 1] + p0i[e + 1], n0[e], pp[e] * a.pamp[0],
+                    vv[e] * a.vamp_h[0], kr, ki, kn);
+                y1r[e] = phx_fma(dth, kr, p0r[e]);
+                y1i[e] = phx_fma(dth, ki, p0i[e]);
+                acr[e] = kr;
+                aci[e] = ki;
                 if (RES) {
-                    yn1[e] = n0[e] + dth * kn;
-                    acn[e] = w16 * kn;
+                    yn1[e] = phx_fma(dth, kn, n0[e]);
+                    acn[e] = kn;
                 }
             }
         }
@@ -144,14 +209,15 @@ __global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
             const int e = it - 2;
             if (e >= 2 && e <= E - 3) {
                 real kr, ki, kn;
-                rhs(y1r[e], y1i[e], y1r[e - 1] + y1r[e + 1], y1i[e - 1] + y1i[e + 1], RES ? yn1[e] : real(0), pp[e] * a.pamp[1], vv[e] * a.vamp[1], kr, ki, kn);
-                y2r[e] = p0r[e] + dth * kr;
-                y2i[e] = p0i[e] + dth * ki;
-                acr[e] += w13 * kr;
-                aci[e] += w13 * ki;
+                rhs(y1r[e], y1i[e], y1r[e - 1] + y1r[e + 1], y1i[e - 1] + y1i[e + 1], RES ? yn1[e] : real(0),
+                    pp[e] * a.pamp[1], vv[e] * a.vamp_h[1], kr, ki, kn);
+                y2r[e] = phx_fma(dth, kr, p0r[e]);
+                y2i[e] = phx_fma(dth, ki, p0i[e]);
+                acr[e] = phx_fma(real(2), kr, acr[e]);
+                aci[e] = phx_fma(real(2), ki, aci[e]);
                 if (RES) {
-                    yn2[e] = n0[e] + dth * kn;
-                    acn[e] += w13 * kn;
+                    yn2[e] = phx_fma(dth, kn, n0[e]);
+                    acn[e] = phx_fma(real(2), kn, acn[e]);
                 }
             }
         }
@@ -160,68 +226,93 @@ __global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
             const int e = it - 3;
             if (e >= 3 && e <= E - 4) {
                 real kr, ki, kn;
-                rhs(y2r[e], y2i[e], y2r[e - 1] + y2r[e + 1], y2i[e - 1] + y2i[e + 1], RES ? yn2[e] : real(0), pp[e] * a.pamp[2], vv[e] * a.vamp[2], kr, ki, kn);
-                y3r[e] = p0r[e] + dt1 * kr;
-                y3i[e] = p0i[e] + dt1 * ki;
-                acr[e] += w13 * kr;
-                aci[e] += w13 * ki;
+                rhs(y2r[e], y2i[e], y2r[e - 1] + y2r[e + 1], y2i[e - 1] + y2i[e + 1], RES ? yn2[e] : real(0),
+                    pp[e] * a.pamp[2], vv[e] * a.vamp_h[2], kr, ki, kn);
+                y3r[e] = phx_fma(dt1, kr, p0r[e]);
+                y3i[e] = phx_fma(dt1, ki, p0i[e]);
+                acr[e] = phx_fma(real(2), kr, acr[e]);
+                aci[e] = phx_fma(real(2), ki, aci[e]);
                 if (RES) {
-                    yn3[e] = n0[e] + dt1 * kn;
-                    acn[e] += w13 * kn;
+                    yn3[e] = phx_fma(dt1, kn, n0[e]);
+                    acn[e] = phx_fma(real(2), kn, acn[e]);
                 }
             }
         }
-        /* ---- stage 4 at row e = it-4, input y3, final sum and store */
+        /* ---- stage 4 at row e = it-4, input y3; psi += dt/6 (k1 + 2k2 + 2k3 + k4) */
         {
             const int e = it - 4;
             if (e >= 4 && e <= E - 5) {
                 real kr, ki, kn;
-                rhs(y3r[e], y3i[e], y3r[e - 1] + y3r[e + 1], y3i[e - 1] + y3i[e + 1], RES ? yn3[e] : real(0), pp[e] * a.pamp[3], vv[e] * a.vamp[3], kr, ki, kn);
+                rhs(y3r[e], y3i[e], y3r[e - 1] + y3r[e + 1], y3i[e - 1] + y3i[e + 1], RES ? yn3[e] : real(0),
+                    pp[e] * a.pamp[3], vv[e] * a.vamp_h[3], kr, ki, kn);
                 const int grow = y0 + e - kHalo;
-                if (out_lane && grow < a.Nr) {
-                    const size_t idx = (size_t)grow * (size_t)a.Nc + (size_t)(x0 - kHalo + lane);
+                if (out_lane && grow < Nr) {
+                    const size_t idx = (size_t)grow * (size_t)Nc + (size_t)(x0 - kHalo + lane);
                     v2 o;
-                    o.x = p0r[e] + dt1 * (acr[e] + w16 * kr);
-                    o.y = p0i[e] + dt1 * (aci[e] + w16 * ki);
+                    o.x = phx_fma(dt6, acr[e] + kr, p0r[e]);
+                    o.y = phx_fma(dt6, aci[e] + ki, p0i[e]);
                     a.psi_out[idx] = o;
-                    if (RES) a.n_out[idx] = n0[e] + dt1 * (acn[e] + w16 * kn);
+                    if (RES) a.n_out[idx] = phx_fma(dt6, acn[e] + kn, n0[e]);
                 }
             }
         }
     }
 }
 
-template <typename real, int TY, bool RES, bool PUMP, bool POT>
+template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>
 static hipError_t launch_ty(const Rk4Args<real>& a0, hipStream_t s) {
     Rk4Args<real> a = a0;
     a.nstrips = (a.Nc + kOutW - 1) / kOutW;
     a.nbands = (a.Nr + TY - 1) / TY;
     const int bandgroups = (a.nbands + kWavesPerBlock - 1) / kWavesPerBlock;
     const int nblocks = a.nstrips * bandgroups;
-    hipLaunchKernelGGL((rk4_scalar_fused<real, TY, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, a);
+    hipLaunchKernelGGL((rk4_scalar_fused<real, TY, EW, PF, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, a);
     return hipGetLastError();
 }
 
+/* Tunables: TY in {8, 10, 12, 16, 20, 24, 32}, EW in {0, 1}, PF in {2, 4}.
+   All shapes are instantiated for the pumped-reservoir configurations (the
+   benchmark workloads); the others use DPP with PF = 2 and TY in {8, 12, 16, 24}.
+   The shuffle variant (EW = 0) is kept as a cross-check of the DPP lane layout. */
+template <typename real, bool RES, bool PUMP, bool POT, int EW, int PF>
+static hipError_t launch_tys(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
+    if constexpr (RES && PUMP && EW == 1) {
+        switch (t.ty) {
+            case 10: return launch_ty<real, 10, EW, PF, RES, PUMP, POT>(a, s);
+            case 20: return launch_ty<real, 20, EW, PF, RES, PUMP, POT>(a, s);
+            case 32: return launch_ty<real, 32, EW, PF, RES, PUMP, POT>(a, s);
+            default: break;
+        }
+    }
+    switch (t.ty) {
+        case 8: return launch_ty<real, 8, EW, PF, RES, PUMP, POT>(a, s);
+        case 12: return launch_ty<real, 12, EW, PF, RES, PUMP, POT>(a, s);
+        case 24: return launch_ty<real, 24, EW, PF, RES, PUMP, POT>(a, s);
+        default: return launch_ty<real, 16, EW, PF, RES, PUMP, POT>(a, s);
+    }
+}
+
 template <typename real, bool RES, bool PUMP, bool POT>
-static hipError_t launch_flags(const Rk4Args<real>& a, int ty, hipStream_t s) {
-    switch (ty) {
-        case 8: return launch_ty<real, 8, RES, PUMP, POT>(a, s);
-        case 32: return launch_ty<real, 32, RES, PUMP, POT>(a, s);
-        default: return launch_ty<real, 16, RES, PUMP, POT>(a, s);
+static hipError_t launch_flags(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
+    if constexpr (RES && PUMP) {
+        if (t.ew == 0) return launch_tys<real, RES, PUMP, POT, 0, 2>(a, t, s);
+        return t.pf >= 4 ? launch_tys<real, RES, PUMP, POT, 1, 4>(a, t, s) : launch_tys<real, RES, PUMP, POT, 1, 2>(a, t, s);
+    } else {
+        return launch_tys<real, RES, PUMP, POT, 1, 2>(a, t, s);
     }
 }
 
 template <typename real>
-hipError_t launch_rk4_scalar(const Rk4Args<real>& a, int ty, hipStream_t s) {
+hipError_t launch_rk4_scalar(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
     const bool res = a.n_in != nullptr;
     const bool pump = res && a.pump != nullptr;
     const bool pot = a.pot != nullptr;
-    if (!res) return pot ? launch_flags<real, false, false, true>(a, ty, s) : launch_flags<real, false, false, false>(a, ty, s);
-    if (pump) return pot ? launch_flags<real, true, true, true>(a, ty, s) : launch_flags<real, true, true, false>(a, ty, s);
-    return pot ? launch_flags<real, true, false, true>(a, ty, s) : launch_flags<real, true, false, false>(a, ty, s);
+    if (!res) return pot ? launch_flags<real, false, false, true>(a, t, s) : launch_flags<real, false, false, false>(a, t, s);
+    if (pump) return pot ? launch_flags<real, true, true, true>(a, t, s) : launch_flags<real, true, true, false>(a, t, s);
+    return pot ? launch_flags<real, true, false, true>(a, t, s) : launch_flags<real, true, false, false>(a, t, s);
 }
 
-template hipError_t launch_rk4_scalar<float>(const Rk4Args<float>&, int, hipStream_t);
-template hipError_t launch_rk4_scalar<double>(const Rk4Args<double>&, int, hipStream_t);
+template hipError_t launch_rk4_scalar<float>(const Rk4Args<float>&, const Rk4Tune&, hipStream_t);
+template hipError_t launch_rk4_scalar<double>(const Rk4Args<double>&, const Rk4Tune&, hipStream_t);
 
 } // namespace phx

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -35,6 +36,14 @@ int fail(int code, const std::string& msg) {
         hipError_t e_ = (call);                                                           \
         if (e_ != hipSuccess) return fail(PHX_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
     } while (0)
+
+/* PHX_RK4_TY / PHX_RK4_EW / PHX_RK4_PF override the launch shape (used by the tuning sweep). */
+phx::Rk4Tune rk4_tune_from_env(phx::Rk4Tune t) {
+    if (const char* e = std::getenv("PHX_RK4_TY")) t.ty = std::atoi(e);
+    if (const char* e = std::getenv("PHX_RK4_EW")) t.ew = std::atoi(e);
+    if (const char* e = std::getenv("PHX_RK4_PF")) t.pf = std::atoi(e);
+    return t;
+}
 } // namespace
 
 struct phx_solver {
@@ -50,10 +59,17 @@ struct phx_solver {
     void* pump[2] = {nullptr, nullptr};
     void* pot[2] = {nullptr, nullptr};
     void* mask[2] = {nullptr, nullptr};
+    void* tw_c = nullptr; /* forward FFT twiddles of the row length N_c */
+    void* tw_r = nullptr; /* ... and of the column length N_r */
+    double* partials = nullptr; /* per-wavefront (error, norm) of the adaptive kernel */
+    double* sums = nullptr;     /* reduced (error, norm), device */
+    double* sums_host = nullptr; /* pinned host copy */
+    int max_rejections = 10000;
+    double last_error = 0; /* E of the last adaptive attempt */
     /* time bookkeeping in the build's precision (system_parameters.hpp:33) */
     double t = 0, dt = 0, fft_cached_t = 0;
     int64_t iteration = 0, rejected = 0;
-    int ty = 16; /* RK4 tile rows per wavefront */
+    phx::Rk4Tune tune; /* launch shape of the fused RK4 kernel */
     std::vector<void*> allocs;
 
     size_t cells() const { return (size_t)cfg.N_c * cfg.N_r; }
@@ -108,10 +124,18 @@ int initialize_t(phx_solver* s) {
     auto& c = s->cfg;
     const size_t N = s->cells();
     const int ncomp = c.tetm ? 2 : 1;
-    if (c.tetm) return fail(PHX_ERR_UNSUPPORTED, "-tetm is not yet provided by this library");
-    if (c.iterator != "RK4") return fail(PHX_ERR_UNSUPPORTED, "iterator " + c.iterator + " is not yet provided by this library");
-    if (c.use_stochastic) return fail(PHX_ERR_UNSUPPORTED, "--dw is not yet provided by this library");
-    if (c.use_fft_mask && c.fft_every < c.t_max) return fail(PHX_ERR_UNSUPPORTED, "--fftMask is not yet provided by this library");
+    if (c.iterator == "DP45" && c.adaptive) {
+        const int nw = phx::rk_generic_waves(phx::RkTableau::DP45_ADAPTIVE, c.N_c, c.N_r);
+        PHX_HIP(hipMalloc((void**)&s->partials, 2 * sizeof(double) * (size_t)nw));
+        s->allocs.push_back(s->partials);
+        PHX_HIP(hipMalloc((void**)&s->sums, 2 * sizeof(double)));
+        s->allocs.push_back(s->sums);
+        PHX_HIP(hipHostMalloc((void**)&s->sums_host, 2 * sizeof(double), hipHostMallocDefault));
+    }
+    if (const char* e = std::getenv("PHX_MAX_REJECTIONS")) s->max_rejections = std::atoi(e);
+    if (c.use_fft_mask && (!phx::fft_supported(c.N_c, sizeof(real)) || !phx::fft_supported(c.N_r, sizeof(real))))
+        return fail(PHX_ERR_UNSUPPORTED, "--fftMask needs power-of-two grid sizes between 4 and " +
+                                             std::to_string(sizeof(real) == 4 ? 4096 : 2048));
     if (c.use_pumps && c.pump.groupSize() > 1 && !c.pump.allConstant())
         return fail(PHX_ERR_UNSUPPORTED, "more than one temporal pump group");
     if (c.use_potentials && c.potential.groupSize() > 1 && !c.potential.allConstant())
@@ -129,6 +153,16 @@ int initialize_t(phx_solver* s) {
             if (int r = dev_alloc(s, &s->pot[k], N * sizeof(real))) return r;
         if (c.use_fft_mask)
             if (int r = dev_alloc(s, &s->mask[k], N * sizeof(real))) return r;
+    }
+    if (c.use_fft_mask) {
+        for (int d = 0; d < 2; d++) {
+            const int L = d == 0 ? c.N_c : c.N_r;
+            std::vector<std::complex<double>> w(L);
+            for (int k = 0; k < L; k++) w[k] = std::polar(1.0, -2.0 * M_PI * (double)k / (double)L);
+            void** slot = d == 0 ? &s->tw_c : &s->tw_r;
+            if (int r = dev_alloc(s, slot, 2 * L * sizeof(real))) return r;
+            if (int r = upload_complex<real>(s, *slot, w)) return r;
+        }
     }
 
     /* initial states (solver_initialization.cu:55-98) */
@@ -176,7 +210,7 @@ int initialize_t(phx_solver* s) {
     s->fft_cached_t = 0.0;
     s->cur = 0;
     s->initialized = true;
-    if (const char* e = std::getenv("PHX_RK4_TY")) s->ty = std::atoi(e);
+    s->tune = rk4_tune_from_env(s->tune);
     return PHX_OK;
 }
 
@@ -211,26 +245,125 @@ int rk4_launch(phx_solver* s, double t_pre, double dt) {
     a.dt = (real)dt;
     static const double w[4] = {0.0, 0.5, 0.5, 1.0};
     stage_amps<real>(c.pump, t_pre, dt, w, 4, a.pamp);
-    stage_amps<real>(c.potential, t_pre, dt, w, 4, a.vamp);
+    stage_amps<real>(c.potential, t_pre, dt, w, 4, a.vamp_h);
+    for (int i = 0; i < 4; i++) a.vamp_h[i] = (real)((double)a.vamp_h[i] * c.one_over_h_bar_s);
     a.ph = phx::make_phys<real>(c);
-    PHX_HIP(phx::launch_rk4_scalar<real>(a, s->ty, s->stream));
+    PHX_HIP(phx::launch_rk4_scalar<real>(a, s->tune, s->stream));
     s->cur = nxt;
+    return PHX_OK;
+}
+
+/* Solver::adaptTimeStep (solver.cu:72-105), power 1/4, on FP64 sums over the grid:
+   E = |err / norm|, dh = (tol / 2 / max(E, DBL_MIN))^(1/4), NaN -> dh = 0.9 and reject,
+   dt' = max(min(dt dh, dt_max), dt_min), accept iff E < tol. */
+bool adapt_dt(double err, double norm, double tol, double dt, double dt_min, double dt_max, double* new_dt, double* final_error) {
+    double fe = std::fabs(err / norm);
+    double dh = std::pow(tol / 2.0 / std::max(fe, DBL_MIN), 0.25);
+    if (std::isnan(dh) || std::isnan(fe)) {
+        dh = 0.9;
+        fe = DBL_MAX;
+    }
+    double nd = std::min(dt * dh, dt_max);
+    if (nd < dt_min) nd = dt_min;
+    *new_dt = nd;
+    *final_error = fe;
+    return fe < tol;
+}
+
+/* One fused generic-tableau step from the current buffers into the other ones. */
+template <typename real>
+int generic_launch(phx_solver* s, phx::RkTableau tab, double t_k, double dt, double noise_dt) {
+    auto& c = s->cfg;
+    using v2 = typename phx::Vec2<real>::type;
+    phx::RkGenArgs<real> a{};
+    const int nxt = s->cur ^ 1;
+    for (int k = 0; k < (c.tetm ? 2 : 1); k++) {
+        a.psi_in[k] = (const v2*)s->psi[k][s->cur];
+        a.psi_out[k] = (v2*)s->psi[k][nxt];
+        a.n_in[k] = c.use_reservoir ? (const real*)s->n[k][s->cur] : nullptr;
+        a.n_out[k] = c.use_reservoir ? (real*)s->n[k][nxt] : nullptr;
+        a.pump[k] = (const real*)s->pump[k];
+        a.pot[k] = (const real*)s->pot[k];
+    }
+    a.Nc = c.N_c;
+    a.Nr = c.N_r;
+    a.periodic_x = c.periodic_x;
+    a.periodic_y = c.periodic_y;
+    a.tetm = c.tetm;
+    a.stochastic = c.use_stochastic;
+    a.dt = (real)dt;
+    static const double w01[2] = {0.0, 1.0};
+    real pa[2], va[2];
+    stage_amps<real>(c.pump, t_k, dt, w01, 2, pa);
+    stage_amps<real>(c.potential, t_k, dt, w01, 2, va);
+    a.amp_pump = pa[0];
+    a.amp_next_pump = pa[1];
+    a.amp_pot_h = (real)((double)va[0] * c.one_over_h_bar_s);
+    a.amp_next_pot_h = (real)((double)va[1] * c.one_over_h_bar_s);
+    a.noise_seed = c.random_seed;
+    a.noise_step = (uint64_t)s->iteration;
+    a.noise_amp = (real)(c.stochastic_amplitude * std::sqrt(noise_dt));
+    a.partials = s->partials;
+    a.sums = s->sums;
+    a.ph = phx::make_phys<real>(c);
+    PHX_HIP(phx::launch_rk_generic<real>(a, tab, s->stream));
+    return PHX_OK;
+}
+
+/* Solver::applyFFTFilter(true) (solver_fft.cu:67-100) on psi+ and, with -tetm, psi-. */
+template <typename real>
+int fft_filter_t(phx_solver* s) {
+    auto& c = s->cfg;
+    if (!s->tw_c || !s->tw_r) return fail(PHX_ERR_STATE, "no FFT mask in this configuration");
+    using v2 = typename phx::Vec2<real>::type;
+    for (int k = 0; k < (c.tetm ? 2 : 1); k++)
+        PHX_HIP(phx::launch_fft_filter<real>((v2*)s->psi[k][s->cur], s->tw_c, s->tw_r, (const real*)s->mask[k], c.N_c, c.N_r,
+                                             s->stream));
     return PHX_OK;
 }
 
 /* Solver::iterate (solver.cu:22-70).  Returns 1 if a step was taken, 0 at t_max, <0 on error. */
 template <typename real>
 int iterate_t(phx_solver* s, bool force_fixed) {
-    (void)force_fixed;
     auto& c = s->cfg;
     if (s->t >= c.t_max) return 0;
     const double t_pre = s->t;
-    s->t = s->rnd(s->t + s->dt); /* t += dt before the step (solver.cu:50) */
-    if (int r = rk4_launch<real>(s, t_pre, s->dt)) return r;
+    const double noise_dt = s->dt; /* the noise of this step is drawn once with sqrt(dt) (solver.cu:36-45) */
+    s->t = s->rnd(s->t + s->dt);   /* t += dt before the step (solver.cu:50) */
+    const bool adaptive = c.iterator == "DP45" && c.adaptive && !force_fixed;
+    if (c.iterator == "RK4" && !c.tetm && !c.use_stochastic) {
+        if (int r = rk4_launch<real>(s, t_pre, s->dt)) return r;
+    } else if (!adaptive) {
+        const auto tab = c.iterator == "RK4" ? phx::RkTableau::RK4 : phx::RkTableau::DP45_FIXED;
+        if (int r = generic_launch<real>(s, tab, t_pre, s->dt, noise_dt)) return r;
+        s->cur ^= 1;
+    } else {
+        /* DormandPrince45::step(true) + Solver::adaptTimeStep (dormand_prince.cu:37-95, solver.cu:72-105) */
+        int rejections = 0;
+        for (;;) {
+            /* retries see the kernel time already advanced (updateKernelTime in adaptTimeStep, solver.cu:97) */
+            const double t_k = rejections == 0 ? t_pre : s->t;
+            if (int r = generic_launch<real>(s, phx::RkTableau::DP45_ADAPTIVE, t_k, s->dt, noise_dt)) return r;
+            PHX_HIP(hipMemcpyAsync(s->sums_host, s->sums, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+            PHX_HIP(hipStreamSynchronize(s->stream));
+            double new_dt = 0, final_error = 0;
+            const bool accept = adapt_dt(s->sums_host[0], s->sums_host[1], c.tolerance, s->dt, c.dt_min, c.dt_max, &new_dt, &final_error);
+            s->last_error = final_error;
+            s->dt = s->rnd(new_dt);
+            if (accept) break;
+            s->rejected++;
+            if (++rejections > s->max_rejections)
+                return fail(PHX_ERR_STEP, "adaptive step rejected " + std::to_string(rejections) + " times (error " +
+                                              std::to_string(final_error) + " >= tolerance at dt = " + std::to_string(s->dt) + ")");
+        }
+        s->cur ^= 1; /* swapBuffers (solver.cu:107-114) */
+    }
     s->iteration++;
-    if (s->t - s->fft_cached_t < c.fft_every) return 1; /* FFT guard (solver.cu:62-63) */
+    if (s->rnd(s->t - s->fft_cached_t) < c.fft_every) return 1; /* FFT guard in the build's real (solver.cu:62-63) */
     s->fft_cached_t = s->t;
-    /* without a mask the reference only transforms for output (solver_fft.cu:78-79) */
+    /* without a mask the reference only transforms for output (solver_fft.cu:78-79): no state change */
+    if (c.use_fft_mask)
+        if (int r = fft_filter_t<real>(s)) return r;
     return 1;
 }
 
@@ -348,7 +481,7 @@ int phx_run(phx_solver* s, int64_t* n_done) {
 
 int phx_apply_fft_filter(phx_solver* s) {
     if (!s || !s->initialized) return fail(PHX_ERR_STATE, "not initialized");
-    return fail(PHX_ERR_UNSUPPORTED, "the FFT filter is not yet provided by this library");
+    return s->precision == PHX_FP32 ? fft_filter_t<float>(s) : fft_filter_t<double>(s);
 }
 
 int phx_get_field(phx_solver* s, int field, void* host, size_t bytes) {
@@ -447,6 +580,7 @@ void phx_destroy(phx_solver* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (void* p : s->allocs) (void)hipFree(p);
+    if (s->sums_host) (void)hipHostFree(s->sums_host);
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);
     delete s;
 }
@@ -507,19 +641,21 @@ static int rk4_device_t(const phx_physics* ph, int N_c, int N_r, int px, int py,
     a.periodic_x = px;
     a.periodic_y = py;
     a.dt = (real)dt;
-    for (int i = 0; i < 4; i++) a.pamp[i] = a.vamp[i] = 1;
+    for (int i = 0; i < 4; i++) {
+        a.pamp[i] = 1;
+        a.vamp_h[i] = (real)c.one_over_h_bar_s;
+    }
     a.ph = phx::make_phys<real>(c);
     a.pump = (const real*)pump;
     a.pot = (const real*)pot;
-    int ty = 16;
-    if (const char* e = std::getenv("PHX_RK4_TY")) ty = std::atoi(e);
+    const phx::Rk4Tune tune = rk4_tune_from_env(phx::Rk4Tune{});
     for (int64_t k = 0; k < n_steps; k++) {
         const bool even = (k % 2) == 0;
         a.psi_in = (const v2*)(even ? psi_a : psi_b);
         a.psi_out = (v2*)(even ? psi_b : psi_a);
         a.n_in = n_a ? (const real*)(even ? n_a : n_b) : nullptr;
         a.n_out = n_a ? (real*)(even ? n_b : n_a) : nullptr;
-        PHX_HIP(phx::launch_rk4_scalar<real>(a, ty, st));
+        PHX_HIP(phx::launch_rk4_scalar<real>(a, tune, st));
     }
     if (n_steps % 2 == 1) {
         PHX_HIP(hipMemcpyAsync(psi_a, psi_b, 2 * sizeof(real) * (size_t)N_c * N_r, hipMemcpyDeviceToDevice, st));
