@@ -295,6 +295,7 @@ static hipError_t launch_tys(const Rk4Args<real>& a, const Rk4Tune& t, hipStream
 template <typename real, bool RES, bool PUMP, bool POT>
 static hipError_t launch_flags(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
     if constexpr (RES && PUMP) {
+        /* deeper prefetch (8, 16 rows) was measured slower at every size: it only adds registers */
         if (t.ew == 0) return launch_tys<real, RES, PUMP, POT, 0, 2>(a, t, s);
         return t.pf >= 4 ? launch_tys<real, RES, PUMP, POT, 1, 4>(a, t, s) : launch_tys<real, RES, PUMP, POT, 1, 2>(a, t, s);
     } else {

@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <random>
 #include <string>
 #include <vector>
@@ -70,6 +71,7 @@ struct phx_solver {
     double t = 0, dt = 0, fft_cached_t = 0;
     int64_t iteration = 0, rejected = 0;
     phx::Rk4Tune tune; /* launch shape of the fused RK4 kernel */
+    bool tuned = false;
     std::vector<void*> allocs;
 
     size_t cells() const { return (size_t)cfg.N_c * cfg.N_r; }
@@ -322,6 +324,56 @@ int fft_filter_t(phx_solver* s) {
     return PHX_OK;
 }
 
+/* Autotuner of the fused RK4 launch shape: time each candidate tile height on this
+   grid (writing into the spare buffers, so the state is untouched) and keep the
+   fastest.  Results are cached per (precision, grid, feature set) for the process.
+   PHX_RK4_TY fixes the shape; PHX_AUTOTUNE=0 keeps the default. */
+template <typename real>
+int autotune_rk4(phx_solver* s) {
+    static std::map<std::string, phx::Rk4Tune> cache;
+    auto& c = s->cfg;
+    if (std::getenv("PHX_RK4_TY")) return PHX_OK;
+    if (const char* e = std::getenv("PHX_AUTOTUNE"))
+        if (std::atoi(e) == 0) return PHX_OK;
+    const std::string key = std::to_string(sizeof(real)) + ":" + std::to_string(c.N_c) + "x" + std::to_string(c.N_r) + ":" +
+                            std::to_string(c.use_reservoir) + std::to_string(c.use_pumps) + std::to_string(c.use_potentials);
+    if (auto it = cache.find(key); it != cache.end()) {
+        s->tune = it->second;
+        return PHX_OK;
+    }
+    hipEvent_t e0, e1;
+    PHX_HIP(hipEventCreate(&e0));
+    PHX_HIP(hipEventCreate(&e1));
+    const int cands[] = {8, 10, 12, 16, 20, 24};
+    float best = 1e30f;
+    phx::Rk4Tune best_t = s->tune;
+    for (int ty : cands) {
+        phx::Rk4Tune t = s->tune;
+        t.ty = ty;
+        const phx::Rk4Tune keep = s->tune;
+        s->tune = t;
+        int rc = PHX_OK;
+        for (int w = 0; w < 2 && !rc; w++) rc = rk4_launch<real>(s, s->t, s->dt), s->cur ^= 1;
+        PHX_HIP(hipEventRecord(e0, s->stream));
+        for (int r = 0; r < 6 && !rc; r++) rc = rk4_launch<real>(s, s->t, s->dt), s->cur ^= 1;
+        PHX_HIP(hipEventRecord(e1, s->stream));
+        PHX_HIP(hipEventSynchronize(e1));
+        s->tune = keep;
+        if (rc) return rc;
+        float ms = 0;
+        PHX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best) {
+            best = ms;
+            best_t = t;
+        }
+    }
+    PHX_HIP(hipEventDestroy(e0));
+    PHX_HIP(hipEventDestroy(e1));
+    s->tune = best_t;
+    cache[key] = best_t;
+    return PHX_OK;
+}
+
 /* Solver::iterate (solver.cu:22-70).  Returns 1 if a step was taken, 0 at t_max, <0 on error. */
 template <typename real>
 int iterate_t(phx_solver* s, bool force_fixed) {
@@ -332,6 +384,10 @@ int iterate_t(phx_solver* s, bool force_fixed) {
     s->t = s->rnd(s->t + s->dt);   /* t += dt before the step (solver.cu:50) */
     const bool adaptive = c.iterator == "DP45" && c.adaptive && !force_fixed;
     if (c.iterator == "RK4" && !c.tetm && !c.use_stochastic) {
+        if (!s->tuned) {
+            if (int r = autotune_rk4<real>(s)) return r;
+            s->tuned = true;
+        }
         if (int r = rk4_launch<real>(s, t_pre, s->dt)) return r;
     } else if (!adaptive) {
         const auto tab = c.iterator == "RK4" ? phx::RkTableau::RK4 : phx::RkTableau::DP45_FIXED;
