@@ -88,6 +88,72 @@ def time_rk4(prec: str, N: int, L: float, steps: int, warmup: int, kernel_sample
                 finite=finite)
 
 
+def ensemble_inputs(i: int, N: int, amp: float, seed: int):
+    """Instance i of the ensemble workload: dx = 0.5 um, Gaussian pump of width 10 um at amp * P_th,
+    psi0 = 0.01 complex Gaussian noise from the instance's own seed, n0 = 0."""
+    L = 0.5 * (N - 1)
+    x = -L / 2 + 0.5 * np.arange(N)
+    X, Y = np.meshgrid(x, x)
+    P = (amp * PTH * np.exp(-(X ** 2 + Y ** 2) / (2 * 10.0 ** 2))).astype(np.float32)
+    rng = np.random.default_rng(seed)
+    psi = (0.01 * (rng.standard_normal((N, N)) + 1j * rng.standard_normal((N, N)))).astype(np.complex64)
+    return psi, np.zeros((N, N), np.float32), P
+
+
+def time_batched(instances: int, steps: int, seed: int = 99, graph_steps: int = 100, check: int = 8):
+    """Ensemble of `instances` FP32 grids with sides drawn from {128,192,256,384,512} and pump amplitudes
+    from U[0.8, 3] P_th: `steps` RK4 steps as one fused launch per step (hipGraph replay) against the same
+    instances stepped one after the other through the plain single-grid entry point."""
+    import torch
+    from phoenix_amd import _lib as Lb
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([128, 192, 256, 384, 512], size=instances)
+    amps = rng.uniform(0.8, 3.0, size=instances)
+    ph, _ = Lb.physics_defaults(512, 512, 255.5, 255.5)
+    dt = 0.04
+    stream = torch.cuda.Stream()
+    ens = Lb.Ensemble("fp32", ph, [(int(N), int(N)) for N in sides], [(0.5, 0.5)] * instances)
+    ens.set_stream(stream.cuda_stream)
+    dev = torch.device("cuda:0")
+    seq = []
+    for i, N in enumerate(sides):
+        psi, n, P = ensemble_inputs(i, int(N), float(amps[i]), seed * 1000 + i)
+        ens.set(i, "psi_plus", psi)
+        ens.set(i, "n_plus", n)
+        ens.set(i, "pump_plus", P)
+        t = [torch.from_numpy(a).to(dev) for a in (psi, n, P)]
+        seq.append((int(N), t[0], t[1], t[2], torch.empty_like(t[0]), torch.empty_like(t[1])))
+    cells = int(sum(int(N) ** 2 for N in sides))
+    torch.cuda.synchronize()
+    # warm-up: autotune of the tile height and graph capture, without advancing the timed state twice
+    ens.step(dt, graph_steps, graph_steps=graph_steps)
+    ens.step(dt, graph_steps, graph_steps=graph_steps)
+    ens.sync()
+    for N, psi, n, P, bpsi, bn in seq:
+        Lb.rk4_scalar_device(Lb.PHX_FP32, ph, N, N, (True, True), psi.data_ptr(), n.data_ptr(), bpsi.data_ptr(),
+                             bn.data_ptr(), P.data_ptr(), None, dt, 2 * graph_steps, stream.cuda_stream)
+    torch.cuda.synchronize()
+    e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+    e0.record(stream)
+    ens.step(dt, steps, graph_steps=graph_steps)
+    e1.record(stream)
+    for N, psi, n, P, bpsi, bn in seq:
+        Lb.rk4_scalar_device(Lb.PHX_FP32, ph, N, N, (True, True), psi.data_ptr(), n.data_ptr(), bpsi.data_ptr(),
+                             bn.data_ptr(), P.data_ptr(), None, dt, steps, stream.cuda_stream)
+    e2.record(stream)
+    torch.cuda.synchronize()
+    ms_b, ms_s = e0.elapsed_time(e1), e1.elapsed_time(e2)
+    same = all(np.array_equal(ens.get(i, "psi_plus"), seq[i][1].cpu().numpy()) and
+               np.array_equal(ens.get(i, "n_plus"), seq[i][2].cpu().numpy()) for i in range(min(check, instances)))
+    ens.close()
+    achieved = 28 * cells / (1e-3 * ms_b / steps) / 1e9
+    return {"instances": instances, "steps": steps, "cells": cells, "sides": "{128,192,256,384,512} seed %d" % seed,
+            "ms_batched": round(ms_b, 3), "ms_sequential": round(ms_s, 3), "speedup": round(ms_s / ms_b, 3),
+            "us_per_step_batched": round(1e3 * ms_b / steps, 3),
+            "gcell_steps_per_s": round(cells * steps / (1e-3 * ms_b) / 1e9, 2),
+            "hbm_frac": round(achieved / HBM_PEAK_GBS, 4), "bitwise_equal_first_%d" % min(check, instances): same}
+
+
 def cpu_baseline(N: int, L: float, steps: int):
     """The FP64 oracle (single-threaded C restatement of the reference CPU path) on a bounded sample."""
     from oracle import oracle as O
@@ -112,6 +178,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None, help="time one precision only (profiling)")
     ap.add_argument("--kernel-samples", type=int, default=200, help="single-step launches timed with event pairs")
+    ap.add_argument("--batch-instances", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=1000)
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB 10 times (calibrates rocprof byte counters)")
     a = ap.parse_args()
@@ -168,6 +237,8 @@ def main():
     for k, v in res.items():
         out["precisions"][k]["roofline_frac"] = round(
             bytes_per_cell[k] * cells / (v["us_per_step"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+    if not a.no_batched and a.only is None:
+        out["batched"] = time_batched(a.batch_instances, a.batch_steps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, L, a.cpu_steps)
     print(json.dumps(out), flush=True)

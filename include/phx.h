@@ -157,6 +157,48 @@ int phx_rk4_scalar_device(int precision, const phx_physics* ph, int N_c, int N_r
                           void* psi_a, void* n_a, void* psi_b, void* n_b, const void* pump, const void* pot,
                           double dt, int64_t n_steps, void* stream);
 
+/* ---------------------------------------------------------------------------
+ * Batched ensembles: many independent scalar grids of different sizes stepped
+ * by fixed RK4 in one launch per step.  Each instance is the system that one
+ * reference process integrates with RungeKutta4::step (runge_kutta.cu:74-94,
+ * scalar path only); the reference runs one instance per process, so this
+ * has no reference entry point of its own.  Every instance shares the physics
+ * constants (except dx, dy), the boundary flags, dt and the feature flags, and
+ * its result is bitwise that of stepping it alone with phx_rk4_scalar_device.
+ * ------------------------------------------------------------------------- */
+typedef struct phx_ensemble phx_ensemble;
+
+enum { PHX_ENS_RESERVOIR = 1, PHX_ENS_PUMP = 2, PHX_ENS_POTENTIAL = 4 };
+
+/* n instances of N_c[i] x N_r[i] cells with grid spacings dx[i], dy[i]
+   (NULL: ph->dx / ph->dy for all).  flags: PHX_ENS_* (a pump needs the
+   reservoir).  All fields start at zero. */
+int phx_ensemble_create(phx_ensemble** out, int precision, const phx_physics* ph, int n, const int* N_c, const int* N_r,
+                        const double* dx, const double* dy, int periodic_x, int periodic_y, int flags);
+
+/* Copy one field of instance `inst` (PHX_PSI_PLUS, PHX_N_PLUS, PHX_PUMP_PLUS,
+   PHX_POTENTIAL_PLUS) from / to host memory; bytes must equal the field size. */
+int phx_ensemble_set_field(phx_ensemble* e, int inst, int field, const void* host, size_t bytes);
+int phx_ensemble_get_field(phx_ensemble* e, int inst, int field, void* host, size_t bytes);
+
+/* Device address of the current buffer of a field (changes parity with each
+   odd step count). */
+int phx_ensemble_device_ptr(phx_ensemble* e, int inst, int field, void** dptr);
+
+/* Output rows per wavefront tile of the batched kernel: 8, 10, 12, 16 or 20.
+   Without this call the first step times every height and keeps the fastest. */
+int phx_ensemble_set_tile_rows(phx_ensemble* e, int ty);
+
+/* n_steps RK4 steps of every instance.  graph_steps >= 2 captures that many
+   (rounded down to even) launches in a hipGraph once and replays it; 0 or 1
+   launches step by step.  Asynchronous on the ensemble's stream. */
+int phx_ensemble_step(phx_ensemble* e, double dt, int64_t n_steps, int graph_steps);
+
+/* Run on a caller-owned hipStream_t (NULL: the ensemble's own stream). */
+int phx_ensemble_set_stream(phx_ensemble* e, void* stream);
+int phx_ensemble_sync(phx_ensemble* e);
+void phx_ensemble_destroy(phx_ensemble* e);
+
 #ifdef __cplusplus
 }
 #endif

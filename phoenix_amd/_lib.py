@@ -27,7 +27,11 @@ EXPORTED = [
     "phx_create", "phx_initialize", "phx_step", "phx_run", "phx_apply_fft_filter", "phx_get_field", "phx_set_field",
     "phx_device_ptr", "phx_get_time", "phx_set_time", "phx_get_info", "phx_set_stream", "phx_sync", "phx_destroy",
     "phx_last_error", "phx_version", "phx_physics_defaults", "phx_rk4_scalar_device",
+    "phx_ensemble_create", "phx_ensemble_set_field", "phx_ensemble_get_field", "phx_ensemble_device_ptr",
+    "phx_ensemble_set_tile_rows", "phx_ensemble_step", "phx_ensemble_set_stream", "phx_ensemble_sync",
+    "phx_ensemble_destroy",
 ]
+PHX_ENS_RESERVOIR, PHX_ENS_PUMP, PHX_ENS_POTENTIAL = 1, 2, 4
 
 
 class PhxError(RuntimeError):
@@ -96,6 +100,18 @@ def lib():
         L.phx_physics_defaults.restype = C.c_double
         L.phx_rk4_scalar_device.argtypes = [C.c_int, C.POINTER(PhxPhysics), C.c_int, C.c_int, C.c_int, C.c_int,
                                             vp, vp, vp, vp, vp, vp, C.c_double, i64, vp]
+        ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
+        L.phx_ensemble_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(PhxPhysics), C.c_int, ip, ip, dp, dp,
+                                          C.c_int, C.c_int, C.c_int]
+        L.phx_ensemble_set_field.argtypes = [vp, C.c_int, C.c_int, vp, C.c_size_t]
+        L.phx_ensemble_get_field.argtypes = [vp, C.c_int, C.c_int, vp, C.c_size_t]
+        L.phx_ensemble_device_ptr.argtypes = [vp, C.c_int, C.c_int, C.POINTER(vp)]
+        L.phx_ensemble_set_tile_rows.argtypes = [vp, C.c_int]
+        L.phx_ensemble_step.argtypes = [vp, C.c_double, i64, C.c_int]
+        L.phx_ensemble_set_stream.argtypes = [vp, vp]
+        L.phx_ensemble_sync.argtypes = [vp]
+        L.phx_ensemble_destroy.argtypes = [vp]
+        L.phx_ensemble_destroy.restype = None
         _lib = L
     return _lib
 
@@ -210,3 +226,77 @@ def rk4_scalar_device(precision: int, ph: PhxPhysics, N_c: int, N_r: int, period
                       psi_b: int, n_b: int, pump: int, pot: int, dt: float, n_steps: int, stream: int = 0):
     check(lib().phx_rk4_scalar_device(precision, C.byref(ph), N_c, N_r, int(periodic[0]), int(periodic[1]), psi_a,
                                       n_a, psi_b, n_b, pump, pot, dt, n_steps, stream))
+
+
+class Ensemble:
+    """Independent scalar grids of different sizes stepped together by fixed RK4
+    (phx_ensemble_* of include/phx.h): one fused launch per step for all of them.
+
+    `sizes` is a list of (N_c, N_r); `spacing` an optional list of (dx, dy).
+    """
+
+    def __init__(self, precision: str, ph: PhxPhysics, sizes: Sequence[tuple], spacing: Optional[Sequence[tuple]] = None,
+                 periodic=(True, True), reservoir: bool = True, pump: bool = True, potential: bool = False):
+        self.precision = PHX_FP32 if precision == "fp32" else PHX_FP64
+        self.real = np.float32 if self.precision == PHX_FP32 else np.float64
+        self.cplx = np.complex64 if self.precision == PHX_FP32 else np.complex128
+        self.sizes = [(int(a), int(b)) for a, b in sizes]
+        n = len(self.sizes)
+        Nc = (C.c_int * n)(*[a for a, _ in self.sizes])
+        Nr = (C.c_int * n)(*[b for _, b in self.sizes])
+        dx = dy = None
+        if spacing is not None:
+            dx = (C.c_double * n)(*[float(a) for a, _ in spacing])
+            dy = (C.c_double * n)(*[float(b) for _, b in spacing])
+        flags = (PHX_ENS_RESERVOIR if reservoir else 0) | (PHX_ENS_PUMP if pump else 0) | (PHX_ENS_POTENTIAL if potential else 0)
+        h = C.c_void_p()
+        check(lib().phx_ensemble_create(C.byref(h), self.precision, C.byref(ph), n, Nc, Nr, dx, dy, int(periodic[0]),
+                                        int(periodic[1]), flags))
+        self._h = h
+
+    def __len__(self):
+        return len(self.sizes)
+
+    def _shape(self, i):
+        return self.sizes[i][1], self.sizes[i][0]
+
+    def get(self, i: int, field: str) -> np.ndarray:
+        fid = FIELDS[field]
+        out = np.empty(self._shape(i), self.cplx if fid in COMPLEX_FIELDS else self.real)
+        check(lib().phx_ensemble_get_field(self._h, i, fid, out.ctypes.data, out.nbytes))
+        return out
+
+    def set(self, i: int, field: str, value: np.ndarray):
+        fid = FIELDS[field]
+        v = np.ascontiguousarray(value, self.cplx if fid in COMPLEX_FIELDS else self.real)
+        if v.shape != self._shape(i):
+            raise ValueError(f"{field}[{i}]: expected shape {self._shape(i)}, got {v.shape}")
+        check(lib().phx_ensemble_set_field(self._h, i, fid, v.ctypes.data, v.nbytes))
+
+    def device_ptr(self, i: int, field: str) -> int:
+        p = C.c_void_p()
+        check(lib().phx_ensemble_device_ptr(self._h, i, FIELDS[field], C.byref(p)))
+        return p.value
+
+    def set_tile_rows(self, ty: int):
+        check(lib().phx_ensemble_set_tile_rows(self._h, ty))
+
+    def step(self, dt: float, n: int = 1, graph_steps: int = 0):
+        check(lib().phx_ensemble_step(self._h, dt, n, graph_steps))
+
+    def set_stream(self, stream_ptr: Optional[int]):
+        check(lib().phx_ensemble_set_stream(self._h, stream_ptr))
+
+    def sync(self):
+        check(lib().phx_ensemble_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().phx_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,6 +1,7 @@
 """Build libphx.so and the PHOENIX-compatible executables for gfx950.
 
-Usage: python -m phoenix_amd.build   (or phoenix_amd.build.build() from Python)
+Usage: python -m phoenix_amd.build [-i] [-v]   (or phoenix_amd.build.build() from Python)
+  -i  recompile only objects older than their source or a header
 
 Outputs (in-tree, so they travel with the source tree):
   phoenix_amd/lib/libphx.so            C ABI of include/phx.h
@@ -36,7 +37,7 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unuse
 HIP_FLAGS_FILE = {"phx_rk_generic.hip": ["-mllvm", "-pragma-unroll-threshold=2000000"]}
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["phx_rk4.hip", "phx_rk_generic.hip", "phx_fft.hip", "phx_solver.hip"]
+HIP_SOURCES = ["phx_rk4.hip", "phx_rk_generic.hip", "phx_fft.hip", "phx_solver.hip", "phx_ensemble.hip"]
 CXX_SOURCES = ["phx_config.cpp"]
 
 
@@ -47,7 +48,18 @@ def _run(cmd):
     return r
 
 
-def build(verbose: bool = False) -> str:
+def _stale(obj: str, src: str) -> bool:
+    """Object missing or older than its source or any header of the library."""
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    deps = [src, __file__] + [os.path.join(d, f) for d in (CSRC, os.path.join(ROOT, "include"))
+                              for f in os.listdir(d) if f.endswith((".h", ".hpp"))]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every source (incremental=True: only objects older than their inputs)."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
     os.makedirs(BIN, exist_ok=True)
@@ -55,12 +67,16 @@ def build(verbose: bool = False) -> str:
     objs = []
     for s in HIP_SOURCES:
         o = os.path.join(BUILD, s.replace(".hip", ".o"))
-        jobs.append([HIPCC, *HIP_FLAGS, *HIP_FLAGS_FILE.get(s, []), "-c", os.path.join(CSRC, s), "-o", o])
+        if not incremental or _stale(o, os.path.join(CSRC, s)):
+            jobs.append([HIPCC, *HIP_FLAGS, *HIP_FLAGS_FILE.get(s, []), "-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
     for s in CXX_SOURCES:
         o = os.path.join(BUILD, s.replace(".cpp", ".o"))
-        jobs.append([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+        if not incremental or _stale(o, os.path.join(CSRC, s)):
+            jobs.append([CXX, *CXX_FLAGS, "-c", os.path.join(CSRC, s), "-o", o])
         objs.append(o)
+    if not jobs:
+        jobs.append(["true"])
     with cf.ThreadPoolExecutor(max_workers=min(8, len(jobs))) as ex:
         for r in ex.map(_run, jobs):
             if verbose and r.stderr:
@@ -79,4 +95,4 @@ def build(verbose: bool = False) -> str:
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="-i" in sys.argv))

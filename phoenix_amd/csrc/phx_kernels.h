@@ -50,6 +50,36 @@ struct Rk4Tune {
 template <typename real>
 hipError_t launch_rk4_scalar(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s);
 
+/* One instance of an ensemble: its buffers, its size, and its first wavefront tile
+   in the flattened tile list.  The kinetic weights carry the instance's dx, dy. */
+template <typename real>
+struct Rk4BatchEntry {
+    const typename Vec2<real>::type* psi_in;
+    const real* n_in;
+    typename Vec2<real>::type* psi_out;
+    real* n_out;
+    const real* pump;
+    const real* pot;
+    int Nc, Nr, nstrips, tile0;
+    real kin_c, kin_x, kin_y;
+};
+
+template <typename real>
+struct Rk4BatchArgs {
+    const Rk4BatchEntry<real>* entries; /* device array, one per instance */
+    const int* tile_inst;               /* instance of each wavefront tile */
+    int ntiles;
+    Rk4Args<real> common;               /* dt, amplitudes, physics, boundary flags */
+};
+
+/* All instances share the feature set (reservoir / pump / potential) and dt. */
+template <typename real>
+hipError_t launch_rk4_batched(const Rk4BatchArgs<real>& b, int ty, bool res, bool pump, bool pot, hipStream_t s);
+
+/* wavefront tiles of one grid for the fused RK4 kernel with tile height ty */
+inline int rk4_tiles(int Nc, int Nr, int ty) { return ((Nc + 55) / 56) * ((Nr + ty - 1) / ty); }
+inline int rk4_strips(int Nc) { return (Nc + 55) / 56; }
+
 /* Arguments of the generic fused Runge-Kutta kernel (phx_rk_generic.hip).
    Index 0 = plus component, 1 = minus component (TE-TM only). */
 template <typename real>

@@ -116,14 +116,9 @@ __device__ __forceinline__ void ld_cplx(rsrc_t r, uint32_t off, double& re, doub
 }
 
 template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>
-__global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
+__device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int band) {
     using v2 = typename Vec2<real>::type;
     const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
-    const int strip = tile % a.nstrips;
-    const int band = (tile / a.nstrips) * kWavesPerBlock + wave;
-    if (band >= a.nbands) return;
     const int x0 = strip * kOutW;
     const int y0 = band * TY;
     const PhxPhys<real>& ph = a.ph;
@@ -259,6 +254,43 @@ __global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
     }
 }
 
+/* one grid: tile (strip, band) from the block index, 4 vertically stacked bands per block */
+template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>
+__global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int strip = tile % a.nstrips;
+    const int band = (tile / a.nstrips) * kWavesPerBlock + wave;
+    if (band >= a.nbands) return;
+    rk4_tile<real, TY, EW, PF, RES, PUMP, POT>(a, strip, band);
+}
+
+/* ensemble: one flattened list of wavefront tiles over all instances; each wave looks up
+   its instance and steps one tile of it (instances of any size in the same launch) */
+template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>
+__global__ __launch_bounds__(256) void rk4_batched_fused(Rk4BatchArgs<real> b) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int gw = xcd_contiguous(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+    if (gw >= b.ntiles) return;
+    const int inst = __builtin_amdgcn_readfirstlane(b.tile_inst[gw]);
+    const Rk4BatchEntry<real>& en = b.entries[inst];
+    Rk4Args<real> a = b.common;
+    a.psi_in = en.psi_in;
+    a.n_in = en.n_in;
+    a.psi_out = en.psi_out;
+    a.n_out = en.n_out;
+    a.pump = en.pump;
+    a.pot = en.pot;
+    a.Nc = en.Nc;
+    a.Nr = en.Nr;
+    a.nstrips = en.nstrips;
+    a.ph.kin_c = en.kin_c;
+    a.ph.kin_x = en.kin_x;
+    a.ph.kin_y = en.kin_y;
+    const int local = gw - en.tile0;
+    rk4_tile<real, TY, EW, PF, RES, PUMP, POT>(a, local % en.nstrips, local / en.nstrips);
+}
+
 template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>
 static hipError_t launch_ty(const Rk4Args<real>& a0, hipStream_t s) {
     Rk4Args<real> a = a0;
@@ -315,5 +347,34 @@ hipError_t launch_rk4_scalar(const Rk4Args<real>& a, const Rk4Tune& t, hipStream
 
 template hipError_t launch_rk4_scalar<float>(const Rk4Args<float>&, const Rk4Tune&, hipStream_t);
 template hipError_t launch_rk4_scalar<double>(const Rk4Args<double>&, const Rk4Tune&, hipStream_t);
+
+/* ---------------------------------------------------------------- ensembles */
+template <typename real, int TY, bool RES, bool PUMP, bool POT>
+static hipError_t launch_batch_ty(const Rk4BatchArgs<real>& b, hipStream_t s) {
+    const int nblocks = (b.ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
+    hipLaunchKernelGGL((rk4_batched_fused<real, TY, 1, 2, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, b);
+    return hipGetLastError();
+}
+
+template <typename real, bool RES, bool PUMP, bool POT>
+static hipError_t launch_batch_flags(const Rk4BatchArgs<real>& b, int ty, hipStream_t s) {
+    switch (ty) {
+        case 8: return launch_batch_ty<real, 8, RES, PUMP, POT>(b, s);
+        case 10: return launch_batch_ty<real, 10, RES, PUMP, POT>(b, s);
+        case 16: return launch_batch_ty<real, 16, RES, PUMP, POT>(b, s);
+        case 20: return launch_batch_ty<real, 20, RES, PUMP, POT>(b, s);
+        default: return launch_batch_ty<real, 12, RES, PUMP, POT>(b, s);
+    }
+}
+
+template <typename real>
+hipError_t launch_rk4_batched(const Rk4BatchArgs<real>& b, int ty, bool res, bool pump, bool pot, hipStream_t s) {
+    if (!res) return pot ? launch_batch_flags<real, false, false, true>(b, ty, s) : launch_batch_flags<real, false, false, false>(b, ty, s);
+    if (pump) return pot ? launch_batch_flags<real, true, true, true>(b, ty, s) : launch_batch_flags<real, true, true, false>(b, ty, s);
+    return pot ? launch_batch_flags<real, true, false, true>(b, ty, s) : launch_batch_flags<real, true, false, false>(b, ty, s);
+}
+
+template hipError_t launch_rk4_batched<float>(const Rk4BatchArgs<float>&, int, bool, bool, bool, hipStream_t);
+template hipError_t launch_rk4_batched<double>(const Rk4BatchArgs<double>&, int, bool, bool, bool, hipStream_t);
 
 } // namespace phx

@@ -26,11 +26,18 @@
 
 namespace {
 thread_local std::string g_err;
+}
 
-int fail(int code, const std::string& msg) {
+namespace phx {
+/* shared by every entry point of the library: phx_last_error() reports it */
+int set_error(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
+} // namespace phx
+
+namespace {
+int fail(int code, const std::string& msg) { return phx::set_error(code, msg); }
 
 #define PHX_HIP(call)                                                                     \
     do {                                                                                  \

@@ -114,3 +114,25 @@ def test_physics_defaults_match_reference_constants():
     assert ph.dx == 0.5 and magic == pytest.approx(0.125)
     # gamma_r = 1.5 * gamma_c and g_r = 2 * g_c as written in system.cpp:29,31
     assert (ph.gamma_c, ph.gamma_r, ph.g_c, ph.g_r, ph.R) == (0.15, 1.5 * 0.15, 3e-6, 2 * 3e-6, 0.01)
+
+
+def test_ensemble_rejects_invalid_arguments_before_touching_the_device():
+    import ctypes as C
+
+    from phoenix_amd import _lib as L
+    ph, _ = L.physics_defaults(64, 64, 31.5, 31.5)
+    lib = L.lib()
+    h = C.c_void_p()
+
+    def create(prec, sizes, flags=3):
+        n = len(sizes)
+        Nc = (C.c_int * max(n, 1))(*[a for a, _ in sizes])
+        Nr = (C.c_int * max(n, 1))(*[b for _, b in sizes])
+        return lib.phx_ensemble_create(C.byref(h), prec, C.byref(ph), n, Nc, Nr, None, None, 1, 1, flags)
+
+    assert create(L.PHX_FP32, []) == L.PHX_ERR_INVALID
+    assert create(7, [(64, 64)]) == L.PHX_ERR_INVALID
+    assert create(L.PHX_FP32, [(64, 64), (0, 12)]) == L.PHX_ERR_INVALID
+    assert "instance 1" in lib.phx_last_error().decode()
+    assert lib.phx_ensemble_step(None, 0.04, 1, 0) == L.PHX_ERR_INVALID
+    assert lib.phx_ensemble_set_tile_rows(None, 12) == L.PHX_ERR_INVALID
