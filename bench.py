@@ -154,6 +154,85 @@ def time_batched(instances: int, steps: int, seed: int = 99, graph_steps: int = 
             "hbm_frac": round(achieved / HBM_PEAK_GBS, 4), "bitwise_equal_first_%d" % min(check, instances): same}
 
 
+def smooth_random_field(rng, N: int, L: float, count: int, amp: float, width: tuple):
+    """Sum of `count` Gaussians with random centres, widths in `width` and amplitudes in [0, amp]."""
+    x = -L / 2 + (L / (N - 1)) * np.arange(N)
+    X, Y = np.meshgrid(x, x)
+    f = np.zeros((N, N))
+    for _ in range(count):
+        cx, cy = rng.uniform(-L / 3, L / 3, 2)
+        w = rng.uniform(*width)
+        f += rng.uniform(0, amp) * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * w * w))
+    return f
+
+
+def time_adjoint(N: int = 512, steps: int = 2000, K: int = 50, batch: int = 16, seed: int = 7):
+    """Gradient of J = sum_seeds int_target |psi(T)|^2 dV (target: 20 um disk 60 um off centre) with respect to
+    P(x,y) and V(x,y) (smooth random fields, 32 Gaussians each), FP32, `batch` noise seeds: forward with a
+    checkpoint every K steps, then the adjoint backward.  Compared with the plain forward of the same batch."""
+    import torch
+    from phoenix_amd import _lib as Lb
+    from phoenix_amd.adjoint import release_engines, rk4_evolve
+    L = 0.5 * (N - 1)
+    dt = 0.04
+    rng = np.random.default_rng(seed)
+    P = smooth_random_field(rng, N, L, 32, 0.5 * PTH, (8.0, 30.0))
+    V = smooth_random_field(rng, N, L, 32, 4e-4, (5.0, 20.0))
+    psi0 = np.stack([ensemble_inputs(0, N, 0.0, seed * 100 + b)[0] for b in range(batch)])
+    x = -L / 2 + 0.5 * np.arange(N)
+    X, Y = np.meshgrid(x, x)
+    target = ((X - 60.0) ** 2 + Y ** 2 < 10.0 ** 2).astype(np.float32)
+    ph, _ = Lb.physics_defaults(N, N, L, L)
+    dev = torch.device("cuda:0")
+    tP = torch.tensor(P, dtype=torch.float32, device=dev, requires_grad=True)
+    tV = torch.tensor(V, dtype=torch.float32, device=dev, requires_grad=True)
+    tpsi = torch.tensor(psi0, dtype=torch.complex64, device=dev)
+    tn = torch.zeros((batch, N, N), dtype=torch.float32, device=dev)
+    tt = torch.tensor(target, dtype=torch.float32, device=dev)
+    dV = 0.25
+
+    def run():
+        psiT, _ = rk4_evolve(tP, tV, tpsi, tn, physics=ph, dt=dt, steps=steps, checkpoint_every=K)
+        return (psiT.abs() ** 2 * tt).sum() * dV
+
+    J = run()  # warm-up: allocation of the checkpoint store
+    J.backward()  # warm-up: tile autotune
+    tP.grad = tV.grad = None
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record()
+    J = run()
+    ev[1].record()
+    J.backward()
+    ev[2].record()
+    torch.cuda.synchronize()
+    ms_f, ms_b = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+    gP, gV = tP.grad, tV.grad
+    finite = bool(torch.isfinite(gP).all() and torch.isfinite(gV).all())
+    # plain forward of the same batch: one uniform-batch RK4 launch per step, no checkpoints
+    ens = Lb.Ensemble("fp32", ph, [(N, N)] * batch, [(0.5, 0.5)] * batch, reservoir=True, pump=True, potential=True)
+    stream = torch.cuda.current_stream()
+    ens.set_stream(stream.cuda_stream)
+    for b in range(batch):
+        ens.set(b, "psi_plus", psi0[b])
+        ens.set(b, "n_plus", np.zeros((N, N), np.float32))
+        ens.set(b, "pump_plus", P.astype(np.float32))
+        ens.set(b, "potential_plus", V.astype(np.float32))
+    ens.step(dt, 100, graph_steps=100)
+    torch.cuda.synchronize()
+    ev[0].record()
+    ens.step(dt, steps, graph_steps=100)
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms_plain = ev[0].elapsed_time(ev[1])
+    ens.close()
+    release_engines()
+    return {"grid": f"{N}x{N}", "steps": steps, "checkpoint_every": K, "batch": batch, "objective": float(J),
+            "ms_forward_checkpointed": round(ms_f, 2), "ms_backward": round(ms_b, 2), "ms_plain_forward": round(ms_plain, 2),
+            "backward_over_forward": round(ms_b / ms_plain, 3),
+            "gradient_over_forward": round((ms_f + ms_b) / ms_plain, 3), "finite": finite}
+
+
 def cpu_baseline(N: int, L: float, steps: int):
     """The FP64 oracle (single-threaded C restatement of the reference CPU path) on a bounded sample."""
     from oracle import oracle as O
@@ -181,6 +260,8 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB 10 times (calibrates rocprof byte counters)")
     a = ap.parse_args()
@@ -239,6 +320,8 @@ def main():
             bytes_per_cell[k] * cells / (v["us_per_step"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
     if not a.no_batched and a.only is None:
         out["batched"] = time_batched(a.batch_instances, a.batch_steps)
+    if not a.no_adjoint and a.only is None:
+        out["adjoint"] = time_adjoint(steps=a.adjoint_steps)
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, L, a.cpu_steps)
     print(json.dumps(out), flush=True)

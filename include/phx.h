@@ -199,6 +199,40 @@ int phx_ensemble_set_stream(phx_ensemble* e, void* stream);
 int phx_ensemble_sync(phx_ensemble* e);
 void phx_ensemble_destroy(phx_ensemble* e);
 
+/* ---------------------------------------------------------------------------
+ * Differentiation: n_steps fixed RK4 steps of `batch` scalar grids of one size
+ * forward with a checkpoint every `checkpoint_every` steps, and the discrete
+ * adjoint backward (stage states recomputed from the checkpoints), giving the
+ * gradient of any objective of the final state with respect to the pump P,
+ * the potential V and the initial state.  The forward map is RungeKutta4::step
+ * (runge_kutta.cu:74-94) with time-independent pump and potential; the
+ * reference itself has no adjoint.  All fields are device pointers, batch
+ * major: psi (batch x N_r x N_c, interleaved complex), n (batch x N_r x N_c).
+ * P, V: one grid shared by the batch (shared_params != 0; their gradients are
+ * summed over the batch) or one grid per item.  Gradients of complex fields
+ * are dJ/dRe + i dJ/dIm.  Asynchronous on the handle's stream.
+ * ------------------------------------------------------------------------- */
+typedef struct phx_adjoint phx_adjoint;
+
+/* flags: PHX_ENS_RESERVOIR / PHX_ENS_PUMP / PHX_ENS_POTENTIAL.  Allocates
+   ceil(n_steps / checkpoint_every) + checkpoint_every + 1 batched states. */
+int phx_adjoint_create(phx_adjoint** out, int precision, const phx_physics* ph, int N_c, int N_r, int batch, int periodic_x,
+                       int periodic_y, int flags, int shared_params, double dt, int64_t n_steps, int checkpoint_every);
+
+/* Run forward from (psi0, n0) to (psi_T, n_T), keeping checkpoints.  pump and
+   pot must stay valid and unchanged until the last phx_adjoint_backward. */
+int phx_adjoint_forward(phx_adjoint* a, const void* psi0, const void* n0, const void* pump, const void* pot, void* psi_T,
+                        void* n_T);
+
+/* Given dJ/d(psi_T) and dJ/d(n_T) (NULL: zero), write dJ/dP, dJ/dV, dJ/d(psi0),
+   dJ/d(n0) (any output may be NULL).  Needs a preceding phx_adjoint_forward. */
+int phx_adjoint_backward(phx_adjoint* a, const void* g_psi_T, const void* g_n_T, void* g_pump, void* g_pot, void* g_psi0,
+                         void* g_n0);
+
+int phx_adjoint_set_stream(phx_adjoint* a, void* stream);
+int phx_adjoint_sync(phx_adjoint* a);
+void phx_adjoint_destroy(phx_adjoint* a);
+
 #ifdef __cplusplus
 }
 #endif

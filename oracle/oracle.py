@@ -69,6 +69,7 @@ def lib():
         L.orc_noise_normal2.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_int, dp, dp]
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_gaussian_field.argtypes = [dp, C.c_int, C.c_double, C.c_uint64]
+        L.orc_rk4_adjoint.argtypes = [P, dp, dp, dp, dp, C.c_double, C.c_int, dp, dp, dp, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -131,6 +132,24 @@ def rk4_run(p: OrcParams, psi, n=None, pump=None, pot=None, dt=0.04, steps=1, ps
     if psi_minus is not None:
         return psi, n, pm, nm
     return psi, n
+
+
+def rk4_adjoint(p: OrcParams, psi0, n0, pump, pot, dt, steps, lam_psi_T, lam_n_T=None):
+    """Discrete adjoint of `steps` RK4 steps (orc_rk4_adjoint).  lam_psi_T: complex cotangent
+    (dJ/dRe + i dJ/dIm) of the final psi, lam_n_T of the final n.  Returns
+    (lam_psi_0, lam_n_0, dJ/dP, dJ/dV, psi_T, n_T)."""
+    shape = np.asarray(psi0).shape
+    psi0 = _c128(psi0)
+    n0 = _f64(n0) if n0 is not None else np.zeros(shape)
+    lp = _c128(lam_psi_T)
+    ln = _f64(lam_n_T) if lam_n_T is not None else np.zeros(shape)
+    gP, gV = np.zeros(shape), np.zeros(shape)
+    psiT, nT = np.zeros(shape, np.complex128), np.zeros(shape)
+    rc = lib().orc_rk4_adjoint(C.byref(p), _dp(psi0.view(np.float64)), _dp(n0), _dp(_f64(pump)), _dp(_f64(pot)), dt, steps,
+                               _dp(lp.view(np.float64)), _dp(ln), _dp(gP), _dp(gV), _dp(psiT.view(np.float64)), _dp(nT))
+    if rc:
+        raise RuntimeError(f"orc_rk4_adjoint failed: {rc}")
+    return lp, ln, gP, gV, psiT, nT
 
 
 def dp45_attempt(p, psi, n, pump=None, pot=None, dt=0.04, noise_dt=None, step=0, psi_minus=None, n_minus=None,

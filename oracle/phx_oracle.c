@@ -752,3 +752,220 @@ void orc_envelope_add(double* out, int N_c, int N_r, double L_x, double L_y, dou
             } else if (behaviour == 4) { out[2 * i] = cr; out[2 * i + 1] = ci; }
         }
 }
+
+/* ======================================================================
+ * Discrete adjoint of fixed-step RK4 (scalar model), FP64.
+ *
+ * Not a reference function (the reference has no adjoint): the gradient of
+ * the forward map that orc_rk4_run restates (RungeKutta4::step,
+ * runge_kutta.cu:74-94, RHS kernel_gp_compute.cuh:50-110), for checking the
+ * HIP adjoint.  Written in complex form on a grid padded by 4 ghost cells in
+ * every zero-boundary direction: the reference evaluates stage s on the halo
+ * of width 4 - s, so stage states exist on ghost cells (from a zero input
+ * halo) and feed the boundary cells; a padded global evaluation reproduces
+ * exactly that map.  Periodic directions wrap instead.
+ *
+ * RHS (complex psi, real n):
+ *   k_psi = -(i/hbar)[m~ L psi + (g_c|psi|^2 + g_r n + V~) psi] + (R n - gamma_c)/2 psi
+ *   k_n   = -(gamma_r + R|psi|^2) n + P~
+ * Cotangents use the real inner product <a, b> = Re(conj(a) b); for a
+ * cotangent (mu, nu) of (k_psi, k_n) the pullback is
+ *   psi_bar = (i/hbar)[m~ L mu + (g_c|psi|^2 + g_r n + V~) mu] + (i/hbar) g_c |psi|^2 mu
+ *             - (i/hbar) g_c psi^2 conj(mu) + (R n - gamma_c)/2 mu - 2 R n nu psi
+ *   n_bar   = Re(conj(mu) (R/2 - i g_r/hbar) psi) - (gamma_r + R|psi|^2) nu
+ *   V_bar  += vamp Re(conj(mu) (-i/hbar) psi),   P_bar += pamp nu
+ * (L is symmetric under both boundary conditions, so it is its own adjoint).
+ * ====================================================================== */
+#include <complex.h>
+
+typedef struct {
+    int W, H;    /* padded extents */
+    int gx, gy;  /* ghost widths (0 for periodic directions) */
+    int px, py;
+} adj_grid_t;
+
+static long adj_nb(const adj_grid_t* g, int r, int c) {
+    if (g->py) r = ((r % g->H) + g->H) % g->H;
+    else if (r < 0 || r >= g->H) return -1;
+    if (g->px) c = ((c % g->W) + g->W) % g->W;
+    else if (c < 0 || c >= g->W) return -1;
+    return (long)r * g->W + c;
+}
+
+/* m~ L f on the padded grid (zero beyond it) */
+static void adj_lap(const adj_grid_t* g, const orc_params* p, const double complex* f, double complex* out) {
+    for (int r = 0; r < g->H; r++)
+        for (int c = 0; c < g->W; c++) {
+            long n = adj_nb(g, r - 1, c), s = adj_nb(g, r + 1, c), w = adj_nb(g, r, c - 1), e = adj_nb(g, r, c + 1);
+            double complex ns = (n >= 0 ? f[n] : 0) + (s >= 0 ? f[s] : 0);
+            double complex ew = (w >= 0 ? f[w] : 0) + (e >= 0 ? f[e] : 0);
+            long i = (long)r * g->W + c;
+            out[i] = p->m_eff_scaled * (p->m2_over_dx2_p_dy2 * f[i] + ns * p->one_over_dy2 + ew * p->one_over_dx2);
+        }
+}
+
+static void adj_rhs(const adj_grid_t* g, const orc_params* p, const double complex* psi, const double* n, const double* P,
+                    const double* V, double pamp, double vamp, double complex* kpsi, double* kn, double complex* tmp) {
+    const long M = (long)g->W * g->H;
+    const double ohb = p->one_over_h_bar_s;
+    adj_lap(g, p, psi, tmp);
+    for (long i = 0; i < M; i++) {
+        const double nv = p->use_reservoir ? n[i] : 0.0;
+        const double a2 = creal(psi[i]) * creal(psi[i]) + cimag(psi[i]) * cimag(psi[i]);
+        const double pot = (p->use_potential ? V[i] * vamp : 0.0) + p->g_c * a2 + p->g_r * nv;
+        kpsi[i] = -I * ohb * (tmp[i] + pot * psi[i]) + 0.5 * (p->R * nv - p->gamma_c) * psi[i];
+        if (p->use_reservoir) kn[i] = -(p->gamma_r + p->R * a2) * nv + (p->use_pump ? P[i] * pamp : 0.0);
+    }
+}
+
+/* pullback of (mu, nu) through the RHS at (psi, n); accumulates into (opsi, on), gP, gV */
+static void adj_vjp(const adj_grid_t* g, const orc_params* p, const double complex* psi, const double* n, const double* V,
+                    double pamp, double vamp, const double complex* mu, const double* nu, double complex* opsi, double* on,
+                    double* gP, double* gV, double complex* tmp) {
+    const long M = (long)g->W * g->H;
+    const double ohb = p->one_over_h_bar_s;
+    adj_lap(g, p, mu, tmp);
+    for (long i = 0; i < M; i++) {
+        const double nv = p->use_reservoir ? n[i] : 0.0;
+        const double nuv = p->use_reservoir ? nu[i] : 0.0;
+        const double complex z = psi[i];
+        const double a2 = creal(z) * creal(z) + cimag(z) * cimag(z);
+        const double pot = (p->use_potential ? V[i] * vamp : 0.0) + p->g_c * a2 + p->g_r * nv;
+        opsi[i] += I * ohb * (tmp[i] + pot * mu[i]) + I * ohb * p->g_c * a2 * mu[i] - I * ohb * p->g_c * z * z * conj(mu[i]) +
+                   0.5 * (p->R * nv - p->gamma_c) * mu[i] - 2.0 * p->R * nv * nuv * z;
+        if (p->use_reservoir) on[i] += creal(conj(mu[i]) * (0.5 * p->R - I * ohb * p->g_r) * z) - (p->gamma_r + p->R * a2) * nuv;
+        if (gV) gV[i] += vamp * creal(conj(mu[i]) * (-I * ohb) * z);
+        if (gP && p->use_reservoir) gP[i] += pamp * nuv;
+    }
+}
+
+int orc_rk4_adjoint(const orc_params* p, const double* psi0, const double* n0, const double* pump, const double* pot, double dt,
+                    int steps, double* lam_psi, double* lam_n, double* g_pump, double* g_pot, double* psi_T, double* n_T) {
+    if (p->tetm || (p->use_stochastic && p->stochastic_amplitude > 0.0)) return -2;
+    adj_grid_t g;
+    g.gx = p->periodic_x ? 0 : 4;
+    g.gy = p->periodic_y ? 0 : 4;
+    g.W = p->N_c + 2 * g.gx;
+    g.H = p->N_r + 2 * g.gy;
+    g.px = p->periodic_x;
+    g.py = p->periodic_y;
+    const long M = (long)g.W * g.H;
+    const long NT = (long)p->N_c * p->N_r;
+    const double w[4] = {0.0, 0.5, 0.5, 1.0};
+    double pamp[4], vamp[4];
+    for (int s = 0; s < 4; s++) {
+        pamp[s] = p->pump_amp * (1.0 - w[s]) + w[s] * p->pump_amp_next;
+        vamp[s] = p->pot_amp * (1.0 - w[s]) + w[s] * p->pot_amp_next;
+    }
+    /* padded fields; states x_0 .. x_steps */
+    double complex* X = xcalloc((size_t)(steps + 1) * M, sizeof(double complex));
+    double* Xn = xcalloc((size_t)(steps + 1) * M, sizeof(double));
+    double *P = xcalloc(M, sizeof(double)), *V = xcalloc(M, sizeof(double));
+    double complex *Y[4], *K[4], *tmp = xcalloc(M, sizeof(double complex));
+    double *Yn[4], *Kn[4];
+    for (int s = 0; s < 4; s++) {
+        Y[s] = xcalloc(M, sizeof(double complex));
+        K[s] = xcalloc(M, sizeof(double complex));
+        Yn[s] = xcalloc(M, sizeof(double));
+        Kn[s] = xcalloc(M, sizeof(double));
+    }
+    double complex *lam = xcalloc(M, sizeof(double complex)), *cs = xcalloc(M, sizeof(double complex));
+    double complex* yb = xcalloc(M, sizeof(double complex));
+    double *lamn = xcalloc(M, sizeof(double)), *csn = xcalloc(M, sizeof(double)), *ybn = xcalloc(M, sizeof(double));
+    double complex* acc = xcalloc(M, sizeof(double complex));
+    double* accn = xcalloc(M, sizeof(double));
+    double *gPp = xcalloc(M, sizeof(double)), *gVp = xcalloc(M, sizeof(double));
+#define PAD(rr, cc) ((long)((rr) + g.gy) * g.W + (cc) + g.gx)
+    for (int r = 0; r < p->N_r; r++)
+        for (int c = 0; c < p->N_c; c++) {
+            long i = (long)r * p->N_c + c, q = PAD(r, c);
+            X[q] = psi0[2 * i] + I * psi0[2 * i + 1];
+            Xn[q] = (p->use_reservoir && n0) ? n0[i] : 0.0;
+            P[q] = (p->use_pump && pump) ? pump[i] : 0.0;
+            V[q] = (p->use_potential && pot) ? pot[i] : 0.0;
+            lam[q] = lam_psi[2 * i] + I * lam_psi[2 * i + 1];
+            lamn[q] = (p->use_reservoir && lam_n) ? lam_n[i] : 0.0;
+        }
+    const double cY[4] = {0.0, 0.5 * dt, 0.5 * dt, dt};   /* Y_s = x + cY[s] k_{s-1} */
+    const double bK[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
+    /* stages of one step from state x (padded, ghosts zero) */
+#define STAGES(x, xn)                                                                                     \
+    for (int s = 0; s < 4; s++) {                                                                         \
+        for (long i = 0; i < M; i++) {                                                                    \
+            Y[s][i] = s ? (x)[i] + cY[s] * K[s - 1][i] : (x)[i];                                          \
+            Yn[s][i] = s ? (xn)[i] + cY[s] * Kn[s - 1][i] : (xn)[i];                                      \
+        }                                                                                                 \
+        adj_rhs(&g, p, Y[s], Yn[s], P, V, pamp[s], vamp[s], K[s], Kn[s], tmp);                            \
+    }
+    for (int k = 0; k < steps; k++) {
+        double complex* x = X + (size_t)k * M;
+        double* xn = Xn + (size_t)k * M;
+        STAGES(x, xn);
+        double complex* x1 = X + (size_t)(k + 1) * M;
+        double* x1n = Xn + (size_t)(k + 1) * M;
+        for (int r = 0; r < p->N_r; r++)
+            for (int c = 0; c < p->N_c; c++) {
+                long q = PAD(r, c);
+                x1[q] = x[q] + bK[0] * K[0][q] + bK[1] * K[1][q] + bK[2] * K[2][q] + bK[3] * K[3][q];
+                x1n[q] = p->use_reservoir ? xn[q] + bK[0] * Kn[0][q] + bK[1] * Kn[1][q] + bK[2] * Kn[2][q] + bK[3] * Kn[3][q] : 0.0;
+            }
+    }
+    /* reverse sweep */
+    for (int k = steps - 1; k >= 0; k--) {
+        double complex* x = X + (size_t)k * M;
+        double* xn = Xn + (size_t)k * M;
+        STAGES(x, xn);
+        memset(acc, 0, M * sizeof(double complex));
+        memset(accn, 0, M * sizeof(double));
+        memset(yb, 0, M * sizeof(double complex));
+        memset(ybn, 0, M * sizeof(double));
+        for (int s = 3; s >= 0; s--) {
+            /* c_s = b_s l' + cY[s+1] Yb_{s+1} */
+            for (long i = 0; i < M; i++) {
+                cs[i] = bK[s] * lam[i] + (s < 3 ? cY[s + 1] * yb[i] : 0.0);
+                csn[i] = bK[s] * lamn[i] + (s < 3 ? cY[s + 1] * ybn[i] : 0.0);
+            }
+            memset(yb, 0, M * sizeof(double complex));
+            memset(ybn, 0, M * sizeof(double));
+            adj_vjp(&g, p, Y[s], Yn[s], V, pamp[s], vamp[s], cs, csn, yb, ybn, gPp, gVp, tmp);
+            for (long i = 0; i < M; i++) {
+                acc[i] += yb[i];
+                accn[i] += ybn[i];
+            }
+        }
+        /* l = l' + sum_s Yb_s on the grid; ghost cells are not variables (zero input halo) */
+        for (long i = 0; i < M; i++) {
+            lam[i] += acc[i];
+            lamn[i] += accn[i];
+        }
+        for (int r = 0; r < g.H; r++)
+            for (int c = 0; c < g.W; c++) {
+                int rr = r - g.gy, cc = c - g.gx;
+                if (rr < 0 || rr >= p->N_r || cc < 0 || cc >= p->N_c) {
+                    lam[(long)r * g.W + c] = 0;
+                    lamn[(long)r * g.W + c] = 0;
+                }
+            }
+    }
+    for (int r = 0; r < p->N_r; r++)
+        for (int c = 0; c < p->N_c; c++) {
+            long i = (long)r * p->N_c + c, q = PAD(r, c);
+            lam_psi[2 * i] = creal(lam[q]);
+            lam_psi[2 * i + 1] = cimag(lam[q]);
+            if (lam_n) lam_n[i] = p->use_reservoir ? lamn[q] : 0.0;
+            if (g_pump) g_pump[i] += gPp[q];
+            if (g_pot) g_pot[i] += gVp[q];
+            if (psi_T) {
+                psi_T[2 * i] = creal(X[(size_t)steps * M + q]);
+                psi_T[2 * i + 1] = cimag(X[(size_t)steps * M + q]);
+            }
+            if (n_T) n_T[i] = Xn[(size_t)steps * M + q];
+        }
+#undef PAD
+#undef STAGES
+    free(X); free(Xn); free(P); free(V); free(tmp);
+    for (int s = 0; s < 4; s++) { free(Y[s]); free(K[s]); free(Yn[s]); free(Kn[s]); }
+    free(lam); free(cs); free(yb); free(lamn); free(csn); free(ybn); free(acc); free(accn); free(gPp); free(gVp);
+    (void)NT;
+    return 0;
+}

@@ -107,6 +107,16 @@ void orc_philox4x32_10(uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
    row-major (the seeded complex Gaussian noise of the benchmark workloads). */
 void orc_gaussian_field(double* psi, int N, double amplitude, uint64_t seed);
 
+/* Discrete adjoint of `steps` fixed RK4 steps of the scalar model (no TE-TM, no
+   noise; pump / potential amplitudes from p as in orc_rk4_run).  The forward
+   map is that of orc_rk4_run, evaluated on a padded grid; psi_T / n_T (may be
+   NULL) receive its final state.  On entry lam_psi (2N, d/dRe + i d/dIm) and
+   lam_n (N, may be NULL) hold the cotangent of the final state; on return the
+   cotangent of the initial state.  g_pump / g_pot (N each, may be NULL)
+   accumulate the cotangents of P and V.  Returns 0, or -2 for unsupported
+   settings.  Not a reference function: the check for the HIP adjoint. */
+int orc_rk4_adjoint(const orc_params* p, const double* psi0, const double* n0, const double* pump, const double* pot, double dt,
+                    int steps, double* lam_psi, double* lam_n, double* g_pump, double* g_pot, double* psi_T, double* n_T);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,8 @@ EXPORTED = [
     "phx_ensemble_create", "phx_ensemble_set_field", "phx_ensemble_get_field", "phx_ensemble_device_ptr",
     "phx_ensemble_set_tile_rows", "phx_ensemble_step", "phx_ensemble_set_stream", "phx_ensemble_sync",
     "phx_ensemble_destroy",
+    "phx_adjoint_create", "phx_adjoint_forward", "phx_adjoint_backward", "phx_adjoint_set_stream", "phx_adjoint_sync",
+    "phx_adjoint_destroy",
 ]
 PHX_ENS_RESERVOIR, PHX_ENS_PUMP, PHX_ENS_POTENTIAL = 1, 2, 4
 
@@ -112,6 +114,14 @@ def lib():
         L.phx_ensemble_sync.argtypes = [vp]
         L.phx_ensemble_destroy.argtypes = [vp]
         L.phx_ensemble_destroy.restype = None
+        L.phx_adjoint_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(PhxPhysics), C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_int, C.c_int, C.c_int, C.c_double, i64, C.c_int]
+        L.phx_adjoint_forward.argtypes = [vp] * 7
+        L.phx_adjoint_backward.argtypes = [vp] * 7
+        L.phx_adjoint_set_stream.argtypes = [vp, vp]
+        L.phx_adjoint_sync.argtypes = [vp]
+        L.phx_adjoint_destroy.argtypes = [vp]
+        L.phx_adjoint_destroy.restype = None
         _lib = L
     return _lib
 
@@ -293,6 +303,46 @@ class Ensemble:
     def close(self):
         if getattr(self, "_h", None):
             lib().phx_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Adjoint:
+    """Checkpointed RK4 forward / discrete-adjoint backward of `batch` scalar grids of one size
+    (phx_adjoint_* of include/phx.h).  Fields are device pointers (e.g. tensor.data_ptr()), batch-major:
+    psi (batch, N_r, N_c) complex, n (batch, N_r, N_c) real; P and V (N_r, N_c) if shared_params,
+    else (batch, N_r, N_c)."""
+
+    def __init__(self, precision: str, ph: PhxPhysics, N_c: int, N_r: int, batch: int, dt: float, steps: int,
+                 checkpoint_every: int = 50, periodic=(True, True), reservoir: bool = True, pump: bool = True,
+                 potential: bool = True, shared_params: bool = True):
+        self.precision = PHX_FP32 if precision == "fp32" else PHX_FP64
+        flags = (PHX_ENS_RESERVOIR if reservoir else 0) | (PHX_ENS_PUMP if pump else 0) | (PHX_ENS_POTENTIAL if potential else 0)
+        h = C.c_void_p()
+        check(lib().phx_adjoint_create(C.byref(h), self.precision, C.byref(ph), N_c, N_r, batch, int(periodic[0]),
+                                       int(periodic[1]), flags, int(shared_params), dt, steps, checkpoint_every))
+        self._h = h
+
+    def forward(self, psi0: int, n0: int, pump: int, pot: int, psi_T: int, n_T: int):
+        check(lib().phx_adjoint_forward(self._h, psi0, n0, pump, pot, psi_T, n_T))
+
+    def backward(self, g_psi_T: int, g_n_T: int, g_pump: int, g_pot: int, g_psi0: int, g_n0: int):
+        check(lib().phx_adjoint_backward(self._h, g_psi_T, g_n_T, g_pump, g_pot, g_psi0, g_n0))
+
+    def set_stream(self, stream_ptr: Optional[int]):
+        check(lib().phx_adjoint_set_stream(self._h, stream_ptr))
+
+    def sync(self):
+        check(lib().phx_adjoint_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().phx_adjoint_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -37,7 +37,7 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unuse
 HIP_FLAGS_FILE = {"phx_rk_generic.hip": ["-mllvm", "-pragma-unroll-threshold=2000000"]}
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["phx_rk4.hip", "phx_rk_generic.hip", "phx_fft.hip", "phx_solver.hip", "phx_ensemble.hip"]
+HIP_SOURCES = ["phx_rk4.hip", "phx_rk_generic.hip", "phx_fft.hip", "phx_solver.hip", "phx_ensemble.hip", "phx_adjoint.hip"]
 CXX_SOURCES = ["phx_config.cpp"]
 
 

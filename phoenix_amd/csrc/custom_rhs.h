@@ -88,6 +88,42 @@ __host__ __device__ __forceinline__ void phx_rhs_scalar(const PhxPhys<real>& p, 
 }
 
 /*
+ * Transpose of the Jacobian of phx_rhs_scalar (the adjoint RK4 kernel calls it
+ * once per cell and stage; an edit of phx_rhs_scalar needs the matching edit
+ * here).  Given the cotangent (cr, ci, cn) of (kr, ki, kn) at the state
+ * (pr, pi, n) and the kinetic stencil applied to the cotangent,
+ * (Kr, Ki) = (m~/hbar) lap c (the 5-point stencil is symmetric, so it is its
+ * own transpose), returns the cotangent (gr, gi, gn) of (pr, pi, n):
+ *   gr = cr (b + 2 g uv) - ci (a + 2 g u^2) - cn 2 R n u - Ki
+ *   gi = cr (a + 2 g v^2) + ci (b - 2 g uv) - cn 2 R n v + Kr
+ *   gn = cr (R/2 u + g_r/hbar v) + ci (R/2 v - g_r/hbar u) - cn (gamma_r + R |psi|^2)
+ * with u = pr, v = pi, g = g_c/hbar and a, b as in phx_rhs_scalar.
+ */
+template <typename real>
+__host__ __device__ __forceinline__ void phx_rhs_scalar_vjp(const PhxPhys<real>& p, real pr, real pi, real n, real pot_h,
+                                                            real cr, real ci, real cn, real Kr, real Ki, real& gr,
+                                                            real& gi, real& gn) {
+    const real nrm = phx_fma(pr, pr, pi * pi);
+    const real a = phx_fma(p.gc_h, nrm, phx_fma(p.gr_h, n, pot_h));
+    const real b = phx_fma(p.half_R, n, p.mhalf_gamma_c);
+    const real g2 = p.gc_h + p.gc_h;
+    const real uv2 = g2 * (pr * pi);
+    const real rn2 = (p.R + p.R) * n;
+    gr = phx_fma(cr, b + uv2, phx_fma(-ci, phx_fma(g2 * pr, pr, a), phx_fma(-cn * rn2, pr, -Ki)));
+    gi = phx_fma(cr, phx_fma(g2 * pi, pi, a), phx_fma(ci, b - uv2, phx_fma(-cn * rn2, pi, Kr)));
+    gn = phx_fma(cr, phx_fma(p.half_R, pr, p.gr_h * pi), phx_fma(ci, phx_fma(p.half_R, pi, -(p.gr_h * pr)), -(cn * phx_fma(p.R, nrm, p.gamma_r))));
+}
+
+/* Cotangents of the pump and potential values of one RHS evaluation:
+   dk/dP = pump amplitude on kn, dk/dV = amplitude/hbar * (pi, -pr) on (kr, ki). */
+template <typename real>
+__host__ __device__ __forceinline__ void phx_rhs_scalar_param_vjp(real pr, real pi, real cr, real ci, real cn, real pamp,
+                                                                  real vamp_h, real& gp, real& gv) {
+    gp = pamp * cn;
+    gv = vamp_h * phx_fma(cr, pi, -(ci * pr));
+}
+
+/*
  * TE-TM spinor model, one component (kernel_gp_compute.cuh:137-234):
  *   k_psi = -i/hbar [ m~ lap_s + (g_c|psi_s|^2 + g_r n_s + V_s + g_pm|psi_o|^2) psi_s + delta_LT cross_s ]
  *           + 1/2 (R n_s - gamma_c) psi_s

@@ -1,0 +1,97 @@
+/*
+ * phx_device.h -- gfx950 device helpers shared by the streaming stencil kernels:
+ * XCD-aware tile order, DPP wavefront shifts for east/west neighbours, raw
+ * buffer loads whose out-of-range offsets read as zero.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace phx {
+
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+
+/* DPP controls of the GFX9 wavefront shifts: lane l of the result reads lane
+   l+1 for wave_shl:1 and lane l-1 for wave_shr:1; the edge lane reads 0. */
+constexpr int kDppWaveShl1 = 0x130;
+constexpr int kDppWaveShr1 = 0x138;
+/* buffer descriptor word 3 for GFX9-family raw buffers */
+constexpr int kBufferFlags = 0x00020000;
+/* byte offset beyond any row: raw buffer loads there return 0 */
+constexpr uint32_t kOutOfRange = 0x80000000u;
+
+/* Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
+   Remap so that each XCD receives a contiguous range of tiles: horizontally
+   adjacent tiles then share one L2 for their halo columns, and a tile stays on
+   the same XCD from step to step. */
+__device__ __forceinline__ int xcd_contiguous(int b, int nblocks) {
+    const int q = nblocks >> 3, r = nblocks & 7;
+    const int x = b & 7;
+    const int start = x * q + (x < r ? x : r);
+    return start + (b >> 3);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    int2 x = __builtin_bit_cast(int2, v);
+    x.x = __builtin_amdgcn_mov_dpp(x.x, CTRL, 0xF, 0xF, true);
+    x.y = __builtin_amdgcn_mov_dpp(x.y, CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, x);
+}
+template <typename real>
+__device__ __forceinline__ real east(real v) { return dpp_move<kDppWaveShl1>(v); }
+template <typename real>
+__device__ __forceinline__ real west(real v) { return dpp_move<kDppWaveShr1>(v); }
+
+/* keep a uniform value in a vector register (an FMA may read one scalar operand only) */
+template <typename T>
+__device__ __forceinline__ T in_vgpr(T x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t row_rsrc(const void* row, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), 0, (int)bytes, kBufferFlags);
+}
+
+/* The vector results of the raw buffer builtins are reinterpreted as a whole:
+   element access on them (v[1]) compiles to a single-dword load on this
+   toolchain and returns the first element twice. */
+template <typename real>
+__device__ __forceinline__ real load_real(rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ float load_real<float>(rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double load_real<double>(rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <typename real>
+struct Cplx;
+template <>
+struct Cplx<float> {
+    using type = float2;
+};
+template <>
+struct Cplx<double> {
+    using type = double2;
+};
+template <typename real>
+__device__ __forceinline__ typename Cplx<real>::type load_cplx(rsrc_t r, uint32_t off);
+template <>
+__device__ __forceinline__ float2 load_cplx<float>(rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double2 load_cplx<double>(rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+}
+
+} // namespace phx

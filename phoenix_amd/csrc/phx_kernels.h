@@ -66,10 +66,15 @@ struct Rk4BatchEntry {
 
 template <typename real>
 struct Rk4BatchArgs {
-    const Rk4BatchEntry<real>* entries; /* device array, one per instance */
+    const Rk4BatchEntry<real>* entries; /* device array, one per instance; nullptr: uniform batch */
     const int* tile_inst;               /* instance of each wavefront tile */
     int ntiles;
     Rk4Args<real> common;               /* dt, amplitudes, physics, boundary flags */
+    /* uniform batch (entries == nullptr): grids of size common.Nc x common.Nr whose
+       fields start at common.<field> + item * item_stride (pump / pot: param_stride,
+       0 = shared by all items); ntiles = items * tiles_per_item */
+    int tiles_per_item;
+    size_t item_stride, param_stride;
 };
 
 /* All instances share the feature set (reservoir / pump / potential) and dt. */
@@ -77,8 +82,46 @@ template <typename real>
 hipError_t launch_rk4_batched(const Rk4BatchArgs<real>& b, int ty, bool res, bool pump, bool pot, hipStream_t s);
 
 /* wavefront tiles of one grid for the fused RK4 kernel with tile height ty */
-inline int rk4_tiles(int Nc, int Nr, int ty) { return ((Nc + 55) / 56) * ((Nr + ty - 1) / ty); }
-inline int rk4_strips(int Nc) { return (Nc + 55) / 56; }
+__host__ __device__ inline int rk4_tiles(int Nc, int Nr, int ty) { return ((Nc + 55) / 56) * ((Nr + ty - 1) / ty); }
+__host__ __device__ inline int rk4_strips(int Nc) { return (Nc + 55) / 56; }
+
+/* Discrete adjoint of one RK4 step of the scalar model (phx_adjoint.hip), for
+   `ntiles / tiles_per_item` grids of one size at a fixed element stride.
+   lam_*_in is the cotangent of the step's output state, lam_*_out receives the
+   cotangent of its input state; g_pump / g_pot (one grid per item) accumulate
+   the cotangents of P and V. */
+template <typename real>
+struct Rk4AdjArgs {
+    const typename Vec2<real>::type* psi; /* input state of the step */
+    const real* n;
+    const typename Vec2<real>::type* lam_psi_in;
+    const real* lam_n_in;
+    typename Vec2<real>::type* lam_psi_out;
+    real* lam_n_out;
+    const real* pump; /* nullptr: no pump */
+    const real* pot;  /* nullptr: V = 0 (its cotangent is still accumulated) */
+    real* g_pump;     /* nullptr: not accumulated */
+    real* g_pot;
+    int Nc, Nr;
+    int periodic_x, periodic_y;
+    int ty, nstrips, tiles_per_item, ntiles; /* ty: output rows per wavefront tile */
+    size_t item_stride, param_stride;       /* elements; param_stride 0: P, V shared */
+    real dt;
+    real pamp[4], vamp_h[4];
+    PhxPhys<real> ph;
+};
+
+/* halo of the adjoint tile: the state is needed 6 cells beyond the output */
+constexpr int kAdjHalo = 6;
+__host__ __device__ inline int adj_strips(int Nc) { return (Nc + 51) / 52; }
+__host__ __device__ inline int adj_tiles(int Nc, int Nr, int ty) { return adj_strips(Nc) * ((Nr + ty - 1) / ty); }
+
+template <typename real>
+hipError_t launch_rk4_adjoint(const Rk4AdjArgs<real>& a, bool res, bool pump, bool pot, hipStream_t s);
+
+/* out[i] = sum_b in[b * N + i], b = 0 .. items-1 in order */
+template <typename real>
+hipError_t launch_sum_items(const real* in, real* out, size_t N, int items, hipStream_t s);
 
 /* Arguments of the generic fused Runge-Kutta kernel (phx_rk_generic.hip).
    Index 0 = plus component, 1 = minus component (TE-TM only). */

@@ -33,46 +33,13 @@
 #include <stdint.h>
 
 #include "custom_rhs.h"
+#include "phx_device.h"
 #include "phx_kernels.h"
 
 namespace phx {
 
-constexpr int kWave = 64;
 constexpr int kHalo = 4;
 constexpr int kOutW = kWave - 2 * kHalo; /* 56 */
-constexpr int kWavesPerBlock = 4;
-
-/* DPP controls of the GFX9 wavefront shifts: lane l of the result reads lane
-   l+1 for wave_shl:1 and lane l-1 for wave_shr:1; the edge lane reads 0. */
-constexpr int kDppWaveShl1 = 0x130;
-constexpr int kDppWaveShr1 = 0x138;
-/* buffer descriptor word 3 for GFX9-family raw buffers */
-constexpr int kBufferFlags = 0x00020000;
-/* byte offset beyond any row: raw buffer loads there return 0 */
-constexpr uint32_t kOutOfRange = 0x80000000u;
-
-/* Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b%8).
-   Remap so that each XCD receives a contiguous range of tiles: horizontally
-   adjacent tiles then share one L2 for their halo columns, and a tile stays on
-   the same XCD from step to step. */
-__device__ __forceinline__ int xcd_contiguous(int b, int nblocks) {
-    const int q = nblocks >> 3, r = nblocks & 7;
-    const int x = b & 7;
-    const int start = x * q + (x < r ? x : r);
-    return start + (b >> 3);
-}
-
-template <int CTRL>
-__device__ __forceinline__ float dpp_move(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double v) {
-    int2 x = __builtin_bit_cast(int2, v);
-    x.x = __builtin_amdgcn_mov_dpp(x.x, CTRL, 0xF, 0xF, true);
-    x.y = __builtin_amdgcn_mov_dpp(x.y, CTRL, 0xF, 0xF, true);
-    return __builtin_bit_cast(double, x);
-}
 
 /* sum of the east (lane+1) and west (lane-1) neighbours.
    EW = 0: ds_bpermute shuffles; EW = 1: DPP wave shifts. */
@@ -84,33 +51,13 @@ __device__ __forceinline__ real east_plus_west(real v, int lane) {
         return dpp_move<kDppWaveShl1>(v) + dpp_move<kDppWaveShr1>(v);
 }
 
-/* keep a uniform value in a vector register (an FMA may read one scalar operand only) */
-template <typename T>
-__device__ __forceinline__ T in_vgpr(T x) {
-    asm volatile("" : "+v"(x));
-    return x;
+template <typename real>
+__device__ __forceinline__ void ld_real(rsrc_t r, uint32_t off, real& x) {
+    x = load_real<real>(r, off);
 }
-
-using rsrc_t = __amdgpu_buffer_rsrc_t;
-__device__ __forceinline__ rsrc_t row_rsrc(const void* row, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), 0, (int)bytes, kBufferFlags);
-}
-__device__ __forceinline__ void ld_real(rsrc_t r, uint32_t off, float& x) {
-    x = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
-__device__ __forceinline__ void ld_real(rsrc_t r, uint32_t off, double& x) {
-    x = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-/* The vector results of the raw buffer builtins are reinterpreted as a whole:
-   element access on them (v[1]) compiles to a single-dword load on this
-   toolchain and returns the first element twice. */
-__device__ __forceinline__ void ld_cplx(rsrc_t r, uint32_t off, float& re, float& im) {
-    const float2 v = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-    re = v.x;
-    im = v.y;
-}
-__device__ __forceinline__ void ld_cplx(rsrc_t r, uint32_t off, double& re, double& im) {
-    const double2 v = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+template <typename real>
+__device__ __forceinline__ void ld_cplx(rsrc_t r, uint32_t off, real& re, real& im) {
+    const auto v = load_cplx<real>(r, off);
     re = v.x;
     im = v.y;
 }
@@ -266,29 +213,46 @@ __global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
 }
 
 /* ensemble: one flattened list of wavefront tiles over all instances; each wave looks up
-   its instance and steps one tile of it (instances of any size in the same launch) */
+   its instance and steps one tile of it (instances of any size in the same launch).
+   Without an instance table: `items` grids of the common size at a fixed element stride. */
 template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>
 __global__ __launch_bounds__(256) void rk4_batched_fused(Rk4BatchArgs<real> b) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int gw = xcd_contiguous(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
     if (gw >= b.ntiles) return;
-    const int inst = __builtin_amdgcn_readfirstlane(b.tile_inst[gw]);
-    const Rk4BatchEntry<real>& en = b.entries[inst];
     Rk4Args<real> a = b.common;
-    a.psi_in = en.psi_in;
-    a.n_in = en.n_in;
-    a.psi_out = en.psi_out;
-    a.n_out = en.n_out;
-    a.pump = en.pump;
-    a.pot = en.pot;
-    a.Nc = en.Nc;
-    a.Nr = en.Nr;
-    a.nstrips = en.nstrips;
-    a.ph.kin_c = en.kin_c;
-    a.ph.kin_x = en.kin_x;
-    a.ph.kin_y = en.kin_y;
-    const int local = gw - en.tile0;
-    rk4_tile<real, TY, EW, PF, RES, PUMP, POT>(a, local % en.nstrips, local / en.nstrips);
+    int local;
+    if (b.entries) {
+        const int inst = __builtin_amdgcn_readfirstlane(b.tile_inst[gw]);
+        const Rk4BatchEntry<real>& en = b.entries[inst];
+        a.psi_in = en.psi_in;
+        a.n_in = en.n_in;
+        a.psi_out = en.psi_out;
+        a.n_out = en.n_out;
+        a.pump = en.pump;
+        a.pot = en.pot;
+        a.Nc = en.Nc;
+        a.Nr = en.Nr;
+        a.nstrips = en.nstrips;
+        a.ph.kin_c = en.kin_c;
+        a.ph.kin_x = en.kin_x;
+        a.ph.kin_y = en.kin_y;
+        local = gw - en.tile0;
+    } else {
+        const int inst = gw / b.tiles_per_item;
+        const size_t so = (size_t)inst * b.item_stride, po = (size_t)inst * b.param_stride;
+        a.psi_in += so;
+        a.psi_out += so;
+        if (RES) {
+            a.n_in += so;
+            a.n_out += so;
+        }
+        if (PUMP) a.pump += po;
+        if (POT) a.pot += po;
+        a.nstrips = rk4_strips(a.Nc);
+        local = gw - inst * b.tiles_per_item;
+    }
+    rk4_tile<real, TY, EW, PF, RES, PUMP, POT>(a, local % a.nstrips, local / a.nstrips);
 }
 
 template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>

@@ -26,66 +26,13 @@
 #include <stdint.h>
 
 #include "custom_rhs.h"
+#include "phx_device.h"
 #include "phx_kernels.h"
 #include "phx_noise.h"
 
 namespace phx {
 
 namespace {
-constexpr int kWave = 64;
-constexpr int kWavesPerBlock = 4;
-constexpr int kDppWaveShl1 = 0x130;
-constexpr int kDppWaveShr1 = 0x138;
-constexpr int kBufferFlags = 0x00020000;
-constexpr uint32_t kOutOfRange = 0x80000000u;
-
-__device__ __forceinline__ int xcd_contiguous(int b, int nblocks) {
-    const int q = nblocks >> 3, r = nblocks & 7;
-    const int x = b & 7;
-    const int start = x * q + (x < r ? x : r);
-    return start + (b >> 3);
-}
-
-template <int CTRL>
-__device__ __forceinline__ float dpp_move(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));
-}
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double v) {
-    int2 x = __builtin_bit_cast(int2, v);
-    x.x = __builtin_amdgcn_mov_dpp(x.x, CTRL, 0xF, 0xF, true);
-    x.y = __builtin_amdgcn_mov_dpp(x.y, CTRL, 0xF, 0xF, true);
-    return __builtin_bit_cast(double, x);
-}
-template <typename real>
-__device__ __forceinline__ real east(real v) { return dpp_move<kDppWaveShl1>(v); }
-template <typename real>
-__device__ __forceinline__ real west(real v) { return dpp_move<kDppWaveShr1>(v); }
-
-template <typename T>
-__device__ __forceinline__ T in_vgpr(T x) {
-    asm volatile("" : "+v"(x));
-    return x;
-}
-
-using rsrc_t = __amdgpu_buffer_rsrc_t;
-__device__ __forceinline__ rsrc_t row_rsrc(const void* row, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(row), 0, (int)bytes, kBufferFlags);
-}
-__device__ __forceinline__ float ld_real(rsrc_t r, uint32_t off, float) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
-}
-__device__ __forceinline__ double ld_real(rsrc_t r, uint32_t off, double) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-/* whole-vector reinterpretation (element access on these builtin results miscompiles) */
-__device__ __forceinline__ float2 ld_cplx(rsrc_t r, uint32_t off, float) {
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-__device__ __forceinline__ double2 ld_cplx(rsrc_t r, uint32_t off, double) {
-    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
-}
-
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
     return v;
@@ -186,12 +133,12 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
         const size_t row = (size_t)(in ? grow : 0) * (size_t)Nc;
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            const v2 v = ld_cplx(row_rsrc(a.psi_in[k] + row, in ? cbytes : 0u), off_c, real(0));
+            const v2 v = load_cplx<real>(row_rsrc(a.psi_in[k] + row, in ? cbytes : 0u), off_c);
             p0r[k][r] = v.x;
             p0i[k][r] = v.y;
-            n0[k][r] = RES ? ld_real(row_rsrc(a.n_in[k] + row, in ? rbytes : 0u), off_r, real(0)) : real(0);
-            pp[k][r] = PUMP ? ld_real(row_rsrc(a.pump[k] + row, in ? rbytes : 0u), off_r, real(0)) : real(0);
-            vv[k][r] = POT ? ld_real(row_rsrc(a.pot[k] + row, in ? rbytes : 0u), off_r, real(0)) : real(0);
+            n0[k][r] = RES ? load_real<real>(row_rsrc(a.n_in[k] + row, in ? rbytes : 0u), off_r) : real(0);
+            pp[k][r] = PUMP ? load_real<real>(row_rsrc(a.pump[k] + row, in ? rbytes : 0u), off_r) : real(0);
+            vv[k][r] = POT ? load_real<real>(row_rsrc(a.pot[k] + row, in ? rbytes : 0u), off_r) : real(0);
         }
         if (STOCH) {
             /* own noise stream: Philox4x32-10 keyed by the seed, counter (cell, step); 0 outside a zero-BC domain */
