@@ -39,6 +39,7 @@ struct PhxPhys {
     real gamma_c, gamma_r, g_c, g_r, R, g_pm, delta_LT;
     /* folded constants, filled on the host from the members above (phx_fold) */
     real kin_c, kin_x, kin_y; /* (m~/hbar) * {C, 1/dx^2, 1/dy^2}: kinetic stencil weights */
+    real kin_yx;              /* kin_y / kin_x = dx^2 / dy^2 (exactly 1 for dx == dy) */
     real gc_h, gr_h, gpm_h;   /* g_c/hbar, g_r/hbar, g_pm/hbar */
     real lt_h, lt_hxy;        /* delta_LT/hbar, delta_LT/hbar * 1/2 /dx/dy */
     real half_R, mhalf_gamma_c; /* R/2, -gamma_c/2 */
@@ -55,6 +56,7 @@ inline void phx_fold(PhxPhys<real>& p, double m_eff_scaled, double one_over_h_ba
     p.kin_c = (real)(hk * m2);
     p.kin_x = (real)(hk * idx2);
     p.kin_y = (real)(hk * idy2);
+    p.kin_yx = (real)(idy2 / idx2);
     p.gc_h = (real)((double)p.g_c * one_over_h_bar_s);
     p.gr_h = (real)((double)p.g_r * one_over_h_bar_s);
     p.gpm_h = (real)((double)p.g_pm * one_over_h_bar_s);
@@ -68,20 +70,22 @@ inline void phx_fold(PhxPhys<real>& p, double m_eff_scaled, double one_over_h_ba
  * Scalar model (kernel_gp_compute.cuh:50-110, non-stochastic branch):
  *   k_psi = -i/hbar [ m~ lap + (g_c|psi|^2 + g_r n + V) psi ] + 1/2 (R n - gamma_c) psi
  *   k_n   = -(gamma_r + R|psi|^2) n + P
- * Inputs: psi = (pr, pi); (tr, ti) = (m~/hbar) lap psi with
- * lap psi = C psi + (N+S)/dy^2 + (E+W)/dx^2 (the kernel evaluates it with the
- * kin_* weights); n reservoir; pump = sum_k P_k a_k(w); pot_h = sum_k V_k a_k(w) / hbar.
+ * Inputs: psi = (pr, pi); (tr, ti) = the neighbour part of (m~/hbar) lap psi,
+ * kin_x (E+W) + kin_y (N+S) with lap psi = C psi + (N+S)/dy^2 + (E+W)/dx^2
+ * (the centre term kin_c psi is added here, folded into the potential:
+ * -i (kin_c psi + T_nb + a psi) = -i ((a + kin_c) psi + T_nb)); n reservoir;
+ * pump = sum_k P_k a_k(w); pot_h = sum_k V_k a_k(w) / hbar.
  * Without a reservoir the caller passes n = 0 and ignores kn.
  */
 template <typename real>
 __host__ __device__ __forceinline__ void phx_rhs_scalar(const PhxPhys<real>& p, real pr, real pi, real tr, real ti,
                                                         real n, real pump, real pot_h, real& kr, real& ki, real& kn) {
     const real nrm = phx_fma(pr, pr, pi * pi);
-    /* real potential felt by psi, over hbar */
-    const real a = phx_fma(p.gc_h, nrm, phx_fma(p.gr_h, n, pot_h));
+    /* real potential felt by psi, over hbar, plus the stencil centre */
+    const real a = phx_fma(p.gc_h, nrm, phx_fma(p.gr_h, n, pot_h + p.kin_c));
     /* gain - loss */
     const real b = phx_fma(p.half_R, n, p.mhalf_gamma_c);
-    /* -i (T + a psi) + b psi */
+    /* -i (T_nb + a psi) + b psi */
     kr = phx_fma(b, pr, phx_fma(a, pi, ti));
     ki = phx_fma(b, pi, phx_fma(-a, pr, -tr));
     kn = phx_fma(phx_fma(-p.R, nrm, -p.gamma_r), n, pump);

@@ -145,8 +145,10 @@ __global__ __launch_bounds__(256) void rk4_adjoint_fused(Rk4AdjArgs<real> a) {
         const real ew = east(c) + west(c);
         return phx_fma(kin_x, ew, phx_fma(kin_y, ns, kin_c * c));
     };
+    /* neighbour part kin_x (E+W) + kin_y (N+S) of the stencil (the RHS hook adds the centre) */
+    auto nbr = [&](real c, real ns) { return phx_fma(kin_x, east(c) + west(c), kin_y * ns); };
     auto rhs = [&](real u, real v, real nsu, real nsv, real n, real p, real vh, real& ku, real& kv, real& kn) {
-        phx_rhs_scalar<real>(ph, u, v, kin(u, nsu), kin(v, nsv), RES ? n : real(0), PUMP ? p : real(0), POT ? vh : real(0), ku,
+        phx_rhs_scalar<real>(ph, u, v, nbr(u, nsu), nbr(v, nsv), RES ? n : real(0), PUMP ? p : real(0), POT ? vh : real(0), ku,
                              kv, kn);
     };
 
@@ -412,6 +414,7 @@ int fwd_step(phx_adjoint* a, const void* psi_in, const void* n_in, void* psi_out
         c.vamp_h[k] = (real)a->ph.one_over_h_bar_s;
     }
     c.ph = adj_phys<real>(a);
+    b.iso = c.ph.kin_yx == real(1);
     ADJ_HIP(phx::launch_rk4_batched<real>(b, ty, a->res, a->pump, a->pot, a->stream));
     return PHX_OK;
 }

@@ -54,6 +54,7 @@ struct phx_ensemble {
     int* tile_inst = nullptr;
     int ntiles = 0;
     int tables_ty = -1;
+    bool iso = false; /* all instances have dx == dy (set with the tables) */
     hipStream_t own_stream = nullptr, stream = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
@@ -95,8 +96,10 @@ int build_tables(phx_ensemble* e, int ty) {
     using v2 = typename phx::Vec2<real>::type;
     std::vector<phx::Rk4BatchEntry<real>> ent[2];
     std::vector<int> inst;
+    e->iso = true;
     for (int i = 0; i < e->n; i++) {
         const PhxPhys<real> p = phys_of<real>(e, i);
+        e->iso = e->iso && p.kin_yx == real(1);
         const int t0 = (int)inst.size();
         const int nt = phx::rk4_tiles(e->Nc[i], e->Nr[i], ty);
         for (int k = 0; k < nt; k++) inst.push_back(i);
@@ -115,6 +118,7 @@ int build_tables(phx_ensemble* e, int ty) {
             en.kin_c = p.kin_c;
             en.kin_x = p.kin_x;
             en.kin_y = p.kin_y;
+            en.kin_yx = p.kin_yx;
             ent[par].push_back(en);
         }
     }
@@ -137,6 +141,7 @@ int launch_step(phx_ensemble* e, double dt, int parity) {
     b.entries = (const phx::Rk4BatchEntry<real>*)e->entries[parity];
     b.tile_inst = e->tile_inst;
     b.ntiles = e->ntiles;
+    b.iso = e->iso;
     b.common.periodic_x = e->periodic_x;
     b.common.periodic_y = e->periodic_y;
     b.common.dt = (real)dt;

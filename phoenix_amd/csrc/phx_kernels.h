@@ -40,9 +40,9 @@ struct Rk4Args {
 };
 
 /* Launch shape of the fused RK4 kernel.
-   ty: output rows per wavefront tile (8, 12, 16, 24);
-   ew: east/west exchange (0 = ds_bpermute shuffle, 1 / 2 = DPP wave shift, either lane direction);
-   pf: rows loaded ahead of first use (0 or 2). */
+   ty: output rows per wavefront tile (8, 10, 12, 16, 20, 24);
+   ew, pf: kept for the tuning interface; the kernel uses DPP wave shifts for
+   east/west and loads rows 2 ahead of first use. */
 struct Rk4Tune {
     int ty = 16, ew = 1, pf = 2;
 };
@@ -61,7 +61,7 @@ struct Rk4BatchEntry {
     const real* pump;
     const real* pot;
     int Nc, Nr, nstrips, tile0;
-    real kin_c, kin_x, kin_y;
+    real kin_c, kin_x, kin_y, kin_yx;
 };
 
 template <typename real>
@@ -75,6 +75,7 @@ struct Rk4BatchArgs {
        0 = shared by all items); ntiles = items * tiles_per_item */
     int tiles_per_item;
     size_t item_stride, param_stride;
+    int iso; /* every instance has dx == dy (kin_yx == 1): stencil without the y/x ratio */
 };
 
 /* All instances share the feature set (reservoir / pump / potential) and dt. */

@@ -41,16 +41,6 @@ namespace phx {
 constexpr int kHalo = 4;
 constexpr int kOutW = kWave - 2 * kHalo; /* 56 */
 
-/* sum of the east (lane+1) and west (lane-1) neighbours.
-   EW = 0: ds_bpermute shuffles; EW = 1: DPP wave shifts. */
-template <int EW, typename real>
-__device__ __forceinline__ real east_plus_west(real v, int lane) {
-    if constexpr (EW == 0)
-        return __shfl(v, lane + 1, kWave) + __shfl(v, lane - 1, kWave);
-    else
-        return dpp_move<kDppWaveShl1>(v) + dpp_move<kDppWaveShr1>(v);
-}
-
 template <typename real>
 __device__ __forceinline__ void ld_real(rsrc_t r, uint32_t off, real& x) {
     x = load_real<real>(r, off);
@@ -62,7 +52,25 @@ __device__ __forceinline__ void ld_cplx(rsrc_t r, uint32_t off, real& re, real& 
     im = v.y;
 }
 
-template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>
+/* Row index of a tile row: inside the grid, wrapped once (periodic; a grid shorter
+   than the tile's reach falls back to a full modulo), or outside (zero boundary). */
+__device__ __forceinline__ int tile_row(int g, int Nr, int periodic, bool& in) {
+    in = true;
+    if (g >= 0 && g < Nr) return g;
+    if (!periodic) {
+        in = false;
+        return 0;
+    }
+    g += g < 0 ? Nr : -Nr;
+    if (g < 0 || g >= Nr) g = ((g % Nr) + Nr) % Nr;
+    return g;
+}
+
+/* Neighbour part of the stencil: kin_x (W + (E + r (N+S))) with r = kin_y / kin_x,
+   both east/west moves folded into DPP adds.  ISO (dx == dy, r == 1 exactly) drops
+   the multiply by r; the two forms give identical bits for dx == dy, so a grid gets
+   the same result from either kernel (an ensemble mixing both uses the general one). */
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT>
 __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int band) {
     using v2 = typename Vec2<real>::type;
     const int lane = threadIdx.x & (kWave - 1);
@@ -90,39 +98,38 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
     real p0r[E], p0i[E], n0[E], pp[E], vv[E];
     real y1r[E], y1i[E], y2r[E], y2i[E], y3r[E], y3i[E];
     real yn1[E], yn2[E], yn3[E];
-    real acr[E], aci[E], acn[E];
+    real acr[E], aci[E], acn[E]; /* psi0 + dt/6 k1 + dt/3 k2 + dt/3 k3, built stage by stage */
 
     const real dth = a.dt * real(0.5);
     const real dt1 = a.dt;
     const real dt6 = a.dt / real(6);
+    const real dt3 = a.dt / real(3);
     PhxPhys<real> phv = ph;
     phv.mhalf_gamma_c = in_vgpr(ph.mhalf_gamma_c);
     phv.gamma_r = in_vgpr(ph.gamma_r);
-    const real kin_c = ph.kin_c, kin_x = ph.kin_x, kin_y = ph.kin_y;
+    const real kin_x = ph.kin_x, kin_yx = ph.kin_yx;
 
     auto load_row = [&](int r) {
-        int grow = y0 - kHalo + r; /* wave-uniform */
-        bool in = true;
-        if (grow < 0 || grow >= Nr) {
-            if (a.periodic_y)
-                grow = ((grow % Nr) + Nr) % Nr;
-            else
-                in = false;
-        }
-        const size_t row = (size_t)(in ? grow : 0) * (size_t)Nc;
+        bool in;
+        const int grow = tile_row(y0 - kHalo + r, Nr, a.periodic_y, in); /* wave-uniform */
+        const size_t row = (size_t)grow * (size_t)Nc;
         ld_cplx(row_rsrc(a.psi_in + row, in ? cbytes : 0u), off_c, p0r[r], p0i[r]);
         if (RES) ld_real(row_rsrc(a.n_in + row, in ? rbytes : 0u), off_r, n0[r]);
         if (PUMP) ld_real(row_rsrc(a.pump + row, in ? rbytes : 0u), off_r, pp[r]);
         if (POT) ld_real(row_rsrc(a.pot + row, in ? rbytes : 0u), off_r, vv[r]);
     };
 
+    /* neighbour part of the kinetic stencil: centre c, north+south ns */
+    auto nb = [&](real c, real ns) {
+        if constexpr (ISO)
+            return kin_x * (west(c) + (east(c) + ns));
+        else
+            return kin_x * (west(c) + (east(c) + kin_yx * ns));
+    };
     /* one RHS evaluation: centre (cr, ci), north+south sums, reservoir, pump*amp, V*amp/hbar */
     auto rhs = [&](real cr, real ci, real nsr, real nsi, real nv, real pmp, real poth, real& kr, real& ki, real& kn) {
-        const real ewr = east_plus_west<EW>(cr, lane);
-        const real ewi = east_plus_west<EW>(ci, lane);
-        const real tr = phx_fma(kin_x, ewr, phx_fma(kin_y, nsr, kin_c * cr));
-        const real ti = phx_fma(kin_x, ewi, phx_fma(kin_y, nsi, kin_c * ci));
-        phx_rhs_scalar<real>(phv, cr, ci, tr, ti, RES ? nv : real(0), PUMP ? pmp : real(0), POT ? poth : real(0), kr, ki, kn);
+        phx_rhs_scalar<real>(phv, cr, ci, nb(cr, nsr), nb(ci, nsi), RES ? nv : real(0), PUMP ? pmp : real(0), POT ? poth : real(0),
+                             kr, ki, kn);
     };
 
 #pragma unroll
@@ -138,11 +145,11 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
                     vv[e] * a.vamp_h[0], kr, ki, kn);
                 y1r[e] = phx_fma(dth, kr, p0r[e]);
                 y1i[e] = phx_fma(dth, ki, p0i[e]);
-                acr[e] = kr;
-                aci[e] = ki;
+                acr[e] = phx_fma(dt6, kr, p0r[e]);
+                aci[e] = phx_fma(dt6, ki, p0i[e]);
                 if (RES) {
                     yn1[e] = phx_fma(dth, kn, n0[e]);
-                    acn[e] = kn;
+                    acn[e] = phx_fma(dt6, kn, n0[e]);
                 }
             }
         }
@@ -155,11 +162,11 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
                     pp[e] * a.pamp[1], vv[e] * a.vamp_h[1], kr, ki, kn);
                 y2r[e] = phx_fma(dth, kr, p0r[e]);
                 y2i[e] = phx_fma(dth, ki, p0i[e]);
-                acr[e] = phx_fma(real(2), kr, acr[e]);
-                aci[e] = phx_fma(real(2), ki, aci[e]);
+                acr[e] = phx_fma(dt3, kr, acr[e]);
+                aci[e] = phx_fma(dt3, ki, aci[e]);
                 if (RES) {
                     yn2[e] = phx_fma(dth, kn, n0[e]);
-                    acn[e] = phx_fma(real(2), kn, acn[e]);
+                    acn[e] = phx_fma(dt3, kn, acn[e]);
                 }
             }
         }
@@ -172,15 +179,15 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
                     pp[e] * a.pamp[2], vv[e] * a.vamp_h[2], kr, ki, kn);
                 y3r[e] = phx_fma(dt1, kr, p0r[e]);
                 y3i[e] = phx_fma(dt1, ki, p0i[e]);
-                acr[e] = phx_fma(real(2), kr, acr[e]);
-                aci[e] = phx_fma(real(2), ki, aci[e]);
+                acr[e] = phx_fma(dt3, kr, acr[e]);
+                aci[e] = phx_fma(dt3, ki, aci[e]);
                 if (RES) {
                     yn3[e] = phx_fma(dt1, kn, n0[e]);
-                    acn[e] = phx_fma(real(2), kn, acn[e]);
+                    acn[e] = phx_fma(dt3, kn, acn[e]);
                 }
             }
         }
-        /* ---- stage 4 at row e = it-4, input y3; psi += dt/6 (k1 + 2k2 + 2k3 + k4) */
+        /* ---- stage 4 at row e = it-4, input y3; psi0 + dt/6 (k1 + 2k2 + 2k3 + k4) */
         {
             const int e = it - 4;
             if (e >= 4 && e <= E - 5) {
@@ -191,10 +198,10 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
                 if (out_lane && grow < Nr) {
                     const size_t idx = (size_t)grow * (size_t)Nc + (size_t)(x0 - kHalo + lane);
                     v2 o;
-                    o.x = phx_fma(dt6, acr[e] + kr, p0r[e]);
-                    o.y = phx_fma(dt6, aci[e] + ki, p0i[e]);
+                    o.x = phx_fma(dt6, kr, acr[e]);
+                    o.y = phx_fma(dt6, ki, aci[e]);
                     a.psi_out[idx] = o;
-                    if (RES) a.n_out[idx] = phx_fma(dt6, acn[e] + kn, n0[e]);
+                    if (RES) a.n_out[idx] = phx_fma(dt6, kn, acn[e]);
                 }
             }
         }
@@ -202,20 +209,20 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
 }
 
 /* one grid: tile (strip, band) from the block index, 4 vertically stacked bands per block */
-template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT>
 __global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
     const int strip = tile % a.nstrips;
     const int band = (tile / a.nstrips) * kWavesPerBlock + wave;
     if (band >= a.nbands) return;
-    rk4_tile<real, TY, EW, PF, RES, PUMP, POT>(a, strip, band);
+    rk4_tile<real, TY, ISO, PF, RES, PUMP, POT>(a, strip, band);
 }
 
 /* ensemble: one flattened list of wavefront tiles over all instances; each wave looks up
    its instance and steps one tile of it (instances of any size in the same launch).
    Without an instance table: `items` grids of the common size at a fixed element stride. */
-template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT>
 __global__ __launch_bounds__(256) void rk4_batched_fused(Rk4BatchArgs<real> b) {
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int gw = xcd_contiguous(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
@@ -237,6 +244,7 @@ __global__ __launch_bounds__(256) void rk4_batched_fused(Rk4BatchArgs<real> b) {
         a.ph.kin_c = en.kin_c;
         a.ph.kin_x = en.kin_x;
         a.ph.kin_y = en.kin_y;
+        a.ph.kin_yx = en.kin_yx;
         local = gw - en.tile0;
     } else {
         const int inst = gw / b.tiles_per_item;
@@ -252,51 +260,36 @@ __global__ __launch_bounds__(256) void rk4_batched_fused(Rk4BatchArgs<real> b) {
         a.nstrips = rk4_strips(a.Nc);
         local = gw - inst * b.tiles_per_item;
     }
-    rk4_tile<real, TY, EW, PF, RES, PUMP, POT>(a, local % a.nstrips, local / a.nstrips);
+    rk4_tile<real, TY, ISO, PF, RES, PUMP, POT>(a, local % a.nstrips, local / a.nstrips);
 }
 
-template <typename real, int TY, int EW, int PF, bool RES, bool PUMP, bool POT>
+template <typename real, int TY, bool ISO, bool RES, bool PUMP, bool POT>
 static hipError_t launch_ty(const Rk4Args<real>& a0, hipStream_t s) {
     Rk4Args<real> a = a0;
     a.nstrips = (a.Nc + kOutW - 1) / kOutW;
     a.nbands = (a.Nr + TY - 1) / TY;
     const int bandgroups = (a.nbands + kWavesPerBlock - 1) / kWavesPerBlock;
     const int nblocks = a.nstrips * bandgroups;
-    hipLaunchKernelGGL((rk4_scalar_fused<real, TY, EW, PF, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, a);
+    hipLaunchKernelGGL((rk4_scalar_fused<real, TY, ISO, 2, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, a);
     return hipGetLastError();
 }
 
-/* Tunables: TY in {8, 10, 12, 16, 20, 24, 32}, EW in {0, 1}, PF in {2, 4}.
-   All shapes are instantiated for the pumped-reservoir configurations (the
-   benchmark workloads); the others use DPP with PF = 2 and TY in {8, 12, 16, 24}.
-   The shuffle variant (EW = 0) is kept as a cross-check of the DPP lane layout. */
-template <typename real, bool RES, bool PUMP, bool POT, int EW, int PF>
+/* Tile height TY in {8, 10, 12, 16, 20, 24} (others: 16). */
+template <typename real, bool ISO, bool RES, bool PUMP, bool POT>
 static hipError_t launch_tys(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
-    if constexpr (RES && PUMP && EW == 1) {
-        switch (t.ty) {
-            case 10: return launch_ty<real, 10, EW, PF, RES, PUMP, POT>(a, s);
-            case 20: return launch_ty<real, 20, EW, PF, RES, PUMP, POT>(a, s);
-            case 32: return launch_ty<real, 32, EW, PF, RES, PUMP, POT>(a, s);
-            default: break;
-        }
-    }
     switch (t.ty) {
-        case 8: return launch_ty<real, 8, EW, PF, RES, PUMP, POT>(a, s);
-        case 12: return launch_ty<real, 12, EW, PF, RES, PUMP, POT>(a, s);
-        case 24: return launch_ty<real, 24, EW, PF, RES, PUMP, POT>(a, s);
-        default: return launch_ty<real, 16, EW, PF, RES, PUMP, POT>(a, s);
+        case 8: return launch_ty<real, 8, ISO, RES, PUMP, POT>(a, s);
+        case 10: return launch_ty<real, 10, ISO, RES, PUMP, POT>(a, s);
+        case 12: return launch_ty<real, 12, ISO, RES, PUMP, POT>(a, s);
+        case 20: return launch_ty<real, 20, ISO, RES, PUMP, POT>(a, s);
+        case 24: return launch_ty<real, 24, ISO, RES, PUMP, POT>(a, s);
+        default: return launch_ty<real, 16, ISO, RES, PUMP, POT>(a, s);
     }
 }
 
 template <typename real, bool RES, bool PUMP, bool POT>
 static hipError_t launch_flags(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
-    if constexpr (RES && PUMP) {
-        /* deeper prefetch (8, 16 rows) was measured slower at every size: it only adds registers */
-        if (t.ew == 0) return launch_tys<real, RES, PUMP, POT, 0, 2>(a, t, s);
-        return t.pf >= 4 ? launch_tys<real, RES, PUMP, POT, 1, 4>(a, t, s) : launch_tys<real, RES, PUMP, POT, 1, 2>(a, t, s);
-    } else {
-        return launch_tys<real, RES, PUMP, POT, 1, 2>(a, t, s);
-    }
+    return a.ph.kin_yx == real(1) ? launch_tys<real, true, RES, PUMP, POT>(a, t, s) : launch_tys<real, false, RES, PUMP, POT>(a, t, s);
 }
 
 template <typename real>
@@ -313,22 +306,27 @@ template hipError_t launch_rk4_scalar<float>(const Rk4Args<float>&, const Rk4Tun
 template hipError_t launch_rk4_scalar<double>(const Rk4Args<double>&, const Rk4Tune&, hipStream_t);
 
 /* ---------------------------------------------------------------- ensembles */
-template <typename real, int TY, bool RES, bool PUMP, bool POT>
+template <typename real, int TY, bool ISO, bool RES, bool PUMP, bool POT>
 static hipError_t launch_batch_ty(const Rk4BatchArgs<real>& b, hipStream_t s) {
     const int nblocks = (b.ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL((rk4_batched_fused<real, TY, 1, 2, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, b);
+    hipLaunchKernelGGL((rk4_batched_fused<real, TY, ISO, 2, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, b);
     return hipGetLastError();
+}
+
+template <typename real, bool ISO, bool RES, bool PUMP, bool POT>
+static hipError_t launch_batch_tys(const Rk4BatchArgs<real>& b, int ty, hipStream_t s) {
+    switch (ty) {
+        case 8: return launch_batch_ty<real, 8, ISO, RES, PUMP, POT>(b, s);
+        case 10: return launch_batch_ty<real, 10, ISO, RES, PUMP, POT>(b, s);
+        case 16: return launch_batch_ty<real, 16, ISO, RES, PUMP, POT>(b, s);
+        case 20: return launch_batch_ty<real, 20, ISO, RES, PUMP, POT>(b, s);
+        default: return launch_batch_ty<real, 12, ISO, RES, PUMP, POT>(b, s);
+    }
 }
 
 template <typename real, bool RES, bool PUMP, bool POT>
 static hipError_t launch_batch_flags(const Rk4BatchArgs<real>& b, int ty, hipStream_t s) {
-    switch (ty) {
-        case 8: return launch_batch_ty<real, 8, RES, PUMP, POT>(b, s);
-        case 10: return launch_batch_ty<real, 10, RES, PUMP, POT>(b, s);
-        case 16: return launch_batch_ty<real, 16, RES, PUMP, POT>(b, s);
-        case 20: return launch_batch_ty<real, 20, RES, PUMP, POT>(b, s);
-        default: return launch_batch_ty<real, 12, RES, PUMP, POT>(b, s);
-    }
+    return b.iso ? launch_batch_tys<real, true, RES, PUMP, POT>(b, ty, s) : launch_batch_tys<real, false, RES, PUMP, POT>(b, ty, s);
 }
 
 template <typename real>

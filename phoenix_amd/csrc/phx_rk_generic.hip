@@ -175,8 +175,6 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
         }
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            const real tr = phx_fma(ph.kin_x, ewr[k], phx_fma(ph.kin_y, nsr[k], ph.kin_c * cr_[k]));
-            const real ti = phx_fma(ph.kin_x, ewi[k], phx_fma(ph.kin_y, nsi[k], ph.kin_c * ci_[k]));
             const real nv = RES || TETM ? in_n(k, s, e) : real(0);
             const real pmp = PUMP ? pp[k][e] * pamp : real(0);
             const real poth = POT ? vv[k][e] * vamp_h : real(0);
@@ -193,8 +191,13 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
                 const real hx = ph.lt_h * ph.one_over_dx2, hy = ph.lt_h * ph.one_over_dy2;
                 const real ccr = phx_fma(hx, ewr[o], phx_fma(-hy, nsr[o], -sgn * ph.lt_hxy * Di));
                 const real cci = phx_fma(hx, ewi[o], phx_fma(-hy, nsi[o], sgn * ph.lt_hxy * Dr));
+                const real tr = phx_fma(ph.kin_x, ewr[k], phx_fma(ph.kin_y, nsr[k], ph.kin_c * cr_[k]));
+                const real ti = phx_fma(ph.kin_x, ewi[k], phx_fma(ph.kin_y, nsi[k], ph.kin_c * ci_[k]));
                 phx_rhs_tetm<real>(ph, cr_[k], ci_[k], tr, ti, ccr, cci, nrm[o], nv, pmp, poth, kr, ki, kn);
             } else {
+                /* neighbour part only: the hook adds the centre term */
+                const real tr = phx_fma(ph.kin_x, ewr[k], ph.kin_y * nsr[k]);
+                const real ti = phx_fma(ph.kin_x, ewi[k], ph.kin_y * nsi[k]);
                 phx_rhs_scalar<real>(ph, cr_[k], ci_[k], tr, ti, nv, pmp, poth, kr, ki, kn);
             }
             if (STOCH) {
