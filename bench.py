@@ -22,6 +22,11 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# Kernel arguments in device memory: the fused kernels read ~200 bytes of arguments at
+# wave start, and host-resident argument buffers add about 2 us to every 10 us launch.
+# Must be set before the HIP runtime initialises (before torch is imported).
+os.environ.setdefault("HIP_FORCE_DEV_KERNARG", "1")
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "rk4_scalar_fused_1024_pmc.md")
 
 METRIC = "µs/RK4 step at 1024² fp32/fp64 + % HBM roofline; batched & adjoint vs plain"
 PTH = 0.15 * 0.225 / 0.01
@@ -82,10 +87,60 @@ def time_rk4(prec: str, N: int, L: float, steps: int, warmup: int, kernel_sample
     k_ms = [a.elapsed_time(b) for a, b in evs] or [total_ms / steps]
     psi = s.get("psi_plus")
     finite = bool(np.isfinite(psi).all())
+    info = s.info
     s.close()
     return dict(total_ms=total_ms, wall_s=wall, us_per_step=1e3 * total_ms / steps,
                 kernel_us_mean=1e3 * float(np.mean(k_ms)), kernel_us_median=1e3 * float(np.median(k_ms)),
-                finite=finite)
+                finite=finite, tile_rows=info["tile_rows"], prefetch_rows=info["prefetch_rows"])
+
+
+def measured_traffic(prec: str, ty: int, pf: int):
+    """HBM bytes per launch of the fused kernel for this tile shape, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (lines 'traffic <prec> ty=<TY> pf=<PF>: ... bytes=<B>'), or None."""
+    import re
+    try:
+        txt = open(PMC_SUMMARY).read()
+    except OSError:
+        return None
+    m = re.search(rf"^traffic {prec} ty={ty} pf={pf}:.*bytes=(\d+)", txt, re.M)
+    return int(m.group(1)) if m else None
+
+
+def time_fft_filter(prec: str, N: int, L: float, calls: int = 50, every: int = 50):
+    """k-space low-pass filter of the 1024^2 workload: radial super-Gaussian mask (exponent 6) with
+    its 1/e radius at 0.8 k_max, applied every `every` RK4 steps.  One call = forward 2D FFT, mask/N,
+    inverse 2D FFT (three kernels: rows, fused columns+mask, rows)."""
+    import torch
+    from phoenix_amd._lib import Solver
+    args = ["--N", N, N, "--L", L, L, "--boundary", "periodic", "periodic", "--tstep", 0.04, "--tmax", 1e9,
+            "--fftEvery", 1e12, "--outEvery", 1e12,
+            "--fftMask", 1.0, "add", 0.565685, 0.565685, 0, 0, "both", 6, "none", "gauss+noDivide+local"]
+    s = Solver([str(a) for a in args], prec)
+    psi, _, _ = workload_inputs(N, L)
+    s.set("psi_plus", psi)
+    k = (np.arange(N) - N // 2) / (N // 2)  # k / k_max, centred layout (the filter applies fftshift)
+    KX, KY = np.meshgrid(k, k)
+    s.set("fft_mask_plus", np.exp(-((KX ** 2 + KY ** 2) / 0.8 ** 2) ** 6))
+    stream = torch.cuda.Stream()
+    s.set_stream(stream.cuda_stream)
+    for _ in range(5):
+        s.apply_fft_filter()
+    s.sync()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(calls):
+        s.apply_fft_filter()
+    e1.record(stream)
+    s.sync()
+    torch.cuda.synchronize()
+    us = 1e3 * e0.elapsed_time(e1) / calls
+    finite = bool(np.isfinite(s.get("psi_plus")).all())
+    s.close()
+    rs = 4 if prec == "fp32" else 8
+    # psi is 2*rs bytes: rows (read+write) 4 rs, columns with the mask (read, mask, write) 5 rs, rows 4 rs
+    moved = N * N * 13 * rs
+    return {"us_per_call": round(us, 3), "every_steps": every, "us_per_step_amortized": round(us / every, 4),
+            "bytes_per_call": moved, "hbm_frac": round(moved / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4), "finite": finite}
 
 
 def ensemble_inputs(i: int, N: int, amp: float, seed: int):
@@ -309,8 +364,12 @@ def main():
                    "cells": cells, "parallelism": "single GPU"},
         "gcell_steps_per_s": round(cells / (r["us_per_step"] * 1e-6) / 1e9, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "rk4_scalar_fused", "kernel_us": round(r["us_per_step"], 3),
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(main_prec, r["tile_rows"], r["prefetch_rows"]),
+                     "kernel": f"rk4_scalar_fused<{'float' if main_prec == 'fp32' else 'double'},TY={r['tile_rows']},"
+                               f"PF={r['prefetch_rows']}>",
+                     "kernel_us": round(r["us_per_step"], 3),
+                     "algorithmic_bytes": bytes_per_cell[main_prec] * cells,
                      "bytes_per_cell": bytes_per_cell[main_prec]},
         "precisions": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                        for k, v in res.items()},
@@ -318,6 +377,8 @@ def main():
     for k, v in res.items():
         out["precisions"][k]["roofline_frac"] = round(
             bytes_per_cell[k] * cells / (v["us_per_step"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4)
+    for k, v in res.items():
+        out["precisions"][k]["fft_filter"] = time_fft_filter(k, N, L)
     if not a.no_batched and a.only is None:
         out["batched"] = time_batched(a.batch_instances, a.batch_steps)
     if not a.no_adjoint and a.only is None:

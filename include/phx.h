@@ -71,6 +71,8 @@ typedef struct {
     double dx, dy, dt, t, t_max, fft_every, tolerance, dt_min, dt_max;
     int64_t iteration;          /* accepted steps so far */
     int64_t rejected;           /* rejected DP45 attempts so far */
+    int32_t tile_rows;          /* output rows per wavefront tile of the fused RK4 kernel (0 before the first step) */
+    int32_t prefetch_rows;      /* rows it loads ahead of use */
 } phx_info;
 
 /* Parse PHOENIX flags (argv[0] is the program name and is skipped), including

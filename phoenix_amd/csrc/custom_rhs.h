@@ -95,20 +95,23 @@ __host__ __device__ __forceinline__ void phx_rhs_scalar(const PhxPhys<real>& p, 
  * Transpose of the Jacobian of phx_rhs_scalar (the adjoint RK4 kernel calls it
  * once per cell and stage; an edit of phx_rhs_scalar needs the matching edit
  * here).  Given the cotangent (cr, ci, cn) of (kr, ki, kn) at the state
- * (pr, pi, n) and the kinetic stencil applied to the cotangent,
- * (Kr, Ki) = (m~/hbar) lap c (the 5-point stencil is symmetric, so it is its
- * own transpose), returns the cotangent (gr, gi, gn) of (pr, pi, n):
+ * (pr, pi, n) and the neighbour part of the kinetic stencil applied to the
+ * cotangent, (Kr, Ki) = kin_x (E+W) + kin_y (N+S) of c (the 5-point stencil is
+ * symmetric, so it is its own transpose; its centre term kin_c c is added here,
+ * folded into a as in phx_rhs_scalar), returns the cotangent (gr, gi, gn) of
+ * (pr, pi, n):
  *   gr = cr (b + 2 g uv) - ci (a + 2 g u^2) - cn 2 R n u - Ki
  *   gi = cr (a + 2 g v^2) + ci (b - 2 g uv) - cn 2 R n v + Kr
  *   gn = cr (R/2 u + g_r/hbar v) + ci (R/2 v - g_r/hbar u) - cn (gamma_r + R |psi|^2)
- * with u = pr, v = pi, g = g_c/hbar and a, b as in phx_rhs_scalar.
+ * with u = pr, v = pi, g = g_c/hbar, a = g |psi|^2 + g_r/hbar n + pot_h + kin_c
+ * and b as in phx_rhs_scalar.
  */
 template <typename real>
 __host__ __device__ __forceinline__ void phx_rhs_scalar_vjp(const PhxPhys<real>& p, real pr, real pi, real n, real pot_h,
                                                             real cr, real ci, real cn, real Kr, real Ki, real& gr,
                                                             real& gi, real& gn) {
     const real nrm = phx_fma(pr, pr, pi * pi);
-    const real a = phx_fma(p.gc_h, nrm, phx_fma(p.gr_h, n, pot_h));
+    const real a = phx_fma(p.gc_h, nrm, phx_fma(p.gr_h, n, pot_h + p.kin_c));
     const real b = phx_fma(p.half_R, n, p.mhalf_gamma_c);
     const real g2 = p.gc_h + p.gc_h;
     const real uv2 = g2 * (pr * pi);

@@ -59,7 +59,7 @@ constexpr int kPF = 1;                         /* rows loaded ahead */
 __host__ __device__ constexpr int slot(int j, int d) { return (j - kPF + d + 8 * kWin) & (kWin - 1); }
 } // namespace
 
-template <typename real, bool RES, bool PUMP, bool POT>
+template <typename real, bool ISO, bool RES, bool PUMP, bool POT>
 __global__ __launch_bounds__(256) void rk4_adjoint_fused(Rk4AdjArgs<real> a) {
     using v2 = typename Vec2<real>::type;
     const int lane = threadIdx.x & (kWave - 1);
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(256) void rk4_adjoint_fused(Rk4AdjArgs<real> a) {
     PhxPhys<real> ph = a.ph;
     ph.mhalf_gamma_c = in_vgpr(ph.mhalf_gamma_c);
     ph.gamma_r = in_vgpr(ph.gamma_r);
-    const real kin_c = ph.kin_c, kin_x = ph.kin_x, kin_y = ph.kin_y;
+    const real kin_x = ph.kin_x;
     const real dth = a.dt * real(0.5), dt1 = a.dt, dt6 = a.dt / real(6), dt3 = a.dt / real(3);
 
     /* state, parameters and incoming cotangent */
@@ -109,8 +109,10 @@ __global__ __launch_bounds__(256) void rk4_adjoint_fused(Rk4AdjArgs<real> a) {
     /* stage cotangents c3, c2, c1, running sum of Yb, parameter cotangents */
     real c3u[kWin], c3v[kWin], c3n[kWin], c2u[kWin], c2v[kWin], c2n[kWin], c1u[kWin], c1v[kWin], c1n[kWin];
     real au[kWin], av[kWin], an[kWin], gpa[kWin], gva[kWin];
+    real d3u[kWin], d3v[kWin], d3n[kWin]; /* dt/3 l' */
 #pragma unroll
     for (int k = 0; k < kWin; k++) {
+        d3u[k] = d3v[k] = d3n[k] = real(0);
         xu[k] = xv[k] = xn[k] = pp[k] = vv[k] = lu[k] = lv[k] = ln[k] = real(0);
         y2u[k] = y2v[k] = y2n[k] = y3u[k] = y3v[k] = y3n[k] = real(0);
         c3u[k] = c3v[k] = c3n[k] = c2u[k] = c2v[k] = c2n[k] = c1u[k] = c1v[k] = c1n[k] = real(0);
@@ -140,13 +142,15 @@ __global__ __launch_bounds__(256) void rk4_adjoint_fused(Rk4AdjArgs<real> a) {
         if (PUMP) pp[sl] = load_real<real>(row_rsrc(pump + row, in ? rbytes : 0u), off_r);
         if (POT) vv[sl] = load_real<real>(row_rsrc(pot + row, in ? rbytes : 0u), off_r);
     };
-    /* kinetic stencil (m~/hbar) lap of a field: centre c, north+south ns */
-    auto kin = [&](real c, real ns) {
-        const real ew = east(c) + west(c);
-        return phx_fma(kin_x, ew, phx_fma(kin_y, ns, kin_c * c));
+    /* neighbour part kin_x (W + (E + r (N+S))) of the stencil, r = kin_y / kin_x (the RHS
+       hooks add the centre); for dx == dy r is 1 and its multiply is skipped */
+    const real kin_yx = ph.kin_yx;
+    auto nbr = [&](real c, real ns) {
+        if constexpr (ISO)
+            return kin_x * (west(c) + (east(c) + ns));
+        else
+            return kin_x * (west(c) + (east(c) + kin_yx * ns));
     };
-    /* neighbour part kin_x (E+W) + kin_y (N+S) of the stencil (the RHS hook adds the centre) */
-    auto nbr = [&](real c, real ns) { return phx_fma(kin_x, east(c) + west(c), kin_y * ns); };
     auto rhs = [&](real u, real v, real nsu, real nsv, real n, real p, real vh, real& ku, real& kv, real& kn) {
         phx_rhs_scalar<real>(ph, u, v, nbr(u, nsu), nbr(v, nsv), RES ? n : real(0), PUMP ? p : real(0), POT ? vh : real(0), ku,
                              kv, kn);
@@ -191,20 +195,23 @@ __global__ __launch_bounds__(256) void rk4_adjoint_fused(Rk4AdjArgs<real> a) {
                     const real y4v = phx_fma(dt1, kv, xv[s]);
                     const real y4n = RES ? phx_fma(dt1, kn, xn[s]) : real(0);
                     const real c4u = dt6 * lu[s], c4v = dt6 * lv[s], c4n = RES ? dt6 * ln[s] : real(0);
-                    const real K4u = kin(c4u, dt6 * lu[sn] + dt6 * lu[ss]);
-                    const real K4v = kin(c4v, dt6 * lv[sn] + dt6 * lv[ss]);
+                    const real K4u = dt6 * nbr(lu[s], lu[sn] + lu[ss]);
+                    const real K4v = dt6 * nbr(lv[s], lv[sn] + lv[ss]);
                     real bu, bv, bn;
                     phx_rhs_scalar_vjp<real>(ph, y4u, y4v, y4n, POT ? vv[s] * a.vamp_h[3] : real(0), c4u, c4v, c4n, K4u, K4v, bu,
                                              bv, bn);
                     au[s] = bu;
                     av[s] = bv;
-                    c3u[s] = phx_fma(dt1, bu, dt3 * lu[s]);
-                    c3v[s] = phx_fma(dt1, bv, dt3 * lv[s]);
+                    d3u[s] = dt3 * lu[s];
+                    d3v[s] = dt3 * lv[s];
+                    c3u[s] = phx_fma(dt1, bu, d3u[s]);
+                    c3v[s] = phx_fma(dt1, bv, d3v[s]);
                     real gp4, gv4, gp3, gv3;
                     phx_rhs_scalar_param_vjp<real>(y4u, y4v, c4u, c4v, c4n, a.pamp[3], a.vamp_h[3], gp4, gv4);
                     if (RES) {
                         an[s] = bn;
-                        c3n[s] = phx_fma(dt1, bn, dt3 * ln[s]);
+                        d3n[s] = dt3 * ln[s];
+                        c3n[s] = phx_fma(dt1, bn, d3n[s]);
                     }
                     phx_rhs_scalar_param_vjp<real>(y3u[s], y3v[s], c3u[s], c3v[s], RES ? c3n[s] : real(0), a.pamp[2], a.vamp_h[2],
                                                    gp3, gv3);
@@ -216,18 +223,18 @@ __global__ __launch_bounds__(256) void rk4_adjoint_fused(Rk4AdjArgs<real> a) {
             {
                 const int e = it - 4, s = slot(j, -4), sn = slot(j, -5), ss = slot(j, -3);
                 {
-                    const real Ku = kin(c3u[s], c3u[sn] + c3u[ss]);
-                    const real Kv = kin(c3v[s], c3v[sn] + c3v[ss]);
+                    const real Ku = nbr(c3u[s], c3u[sn] + c3u[ss]);
+                    const real Kv = nbr(c3v[s], c3v[sn] + c3v[ss]);
                     real bu, bv, bn;
                     phx_rhs_scalar_vjp<real>(ph, y3u[s], y3v[s], RES ? y3n[s] : real(0), POT ? vv[s] * a.vamp_h[2] : real(0), c3u[s],
                                              c3v[s], RES ? c3n[s] : real(0), Ku, Kv, bu, bv, bn);
                     au[s] += bu;
                     av[s] += bv;
-                    c2u[s] = phx_fma(dth, bu, dt3 * lu[s]);
-                    c2v[s] = phx_fma(dth, bv, dt3 * lv[s]);
+                    c2u[s] = phx_fma(dth, bu, d3u[s]);
+                    c2v[s] = phx_fma(dth, bv, d3v[s]);
                     if (RES) {
                         an[s] += bn;
-                        c2n[s] = phx_fma(dth, bn, dt3 * ln[s]);
+                        c2n[s] = phx_fma(dth, bn, d3n[s]);
                     }
                     real gp2, gv2;
                     phx_rhs_scalar_param_vjp<real>(y2u[s], y2v[s], c2u[s], c2v[s], RES ? c2n[s] : real(0), a.pamp[1], a.vamp_h[1],
@@ -240,8 +247,8 @@ __global__ __launch_bounds__(256) void rk4_adjoint_fused(Rk4AdjArgs<real> a) {
             {
                 const int e = it - 5, s = slot(j, -5), sn = slot(j, -6), ss = slot(j, -4);
                 {
-                    const real Ku = kin(c2u[s], c2u[sn] + c2u[ss]);
-                    const real Kv = kin(c2v[s], c2v[sn] + c2v[ss]);
+                    const real Ku = nbr(c2u[s], c2u[sn] + c2u[ss]);
+                    const real Kv = nbr(c2v[s], c2v[sn] + c2v[ss]);
                     real bu, bv, bn;
                     phx_rhs_scalar_vjp<real>(ph, y2u[s], y2v[s], RES ? y2n[s] : real(0), POT ? vv[s] * a.vamp_h[1] : real(0), c2u[s],
                                              c2v[s], RES ? c2n[s] : real(0), Ku, Kv, bu, bv, bn);
@@ -264,8 +271,8 @@ __global__ __launch_bounds__(256) void rk4_adjoint_fused(Rk4AdjArgs<real> a) {
             {
                 const int e = it - 6, s = slot(j, -6), sn = slot(j, -7), ss = slot(j, -5);
                 if (e >= 6 && e <= E - 7) {
-                    const real Ku = kin(c1u[s], c1u[sn] + c1u[ss]);
-                    const real Kv = kin(c1v[s], c1v[sn] + c1v[ss]);
+                    const real Ku = nbr(c1u[s], c1u[sn] + c1u[ss]);
+                    const real Kv = nbr(c1v[s], c1v[sn] + c1v[ss]);
                     real bu, bv, bn;
                     phx_rhs_scalar_vjp<real>(ph, xu[s], xv[s], RES ? xn[s] : real(0), POT ? vv[s] * a.vamp_h[0] : real(0), c1u[s],
                                              c1v[s], RES ? c1n[s] : real(0), Ku, Kv, bu, bv, bn);
@@ -289,7 +296,10 @@ __global__ __launch_bounds__(256) void rk4_adjoint_fused(Rk4AdjArgs<real> a) {
 template <typename real, bool RES, bool PUMP, bool POT>
 static hipError_t launch_adj(const Rk4AdjArgs<real>& a, hipStream_t s) {
     const int nblocks = (a.ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL((rk4_adjoint_fused<real, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, a);
+    if (a.ph.kin_yx == real(1))
+        hipLaunchKernelGGL((rk4_adjoint_fused<real, true, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((rk4_adjoint_fused<real, false, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, a);
     return hipGetLastError();
 }
 
@@ -486,7 +496,7 @@ int autotune(phx_adjoint* a) {
         if (ms < best) best = ms, a->ty_fwd = ty;
     }
     best = 1e30f;
-    for (int ty : {16, 24, 32, 48, 64}) {
+    for (int ty : {16, 20, 24, 28, 32, 40, 48, 56, 64, 80}) {
         if (rc) break;
         float ms = 0;
         rc = timeit([&] { return adj_step<real>(a, a->ckpt_at(0), 0, ty); }, ms);

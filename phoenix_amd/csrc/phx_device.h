@@ -94,4 +94,46 @@ __device__ __forceinline__ double2 load_cplx<double>(rsrc_t r, uint32_t off) {
     return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
 }
 
+/* Loads and stores with a scalar byte offset (soffset) on top of the per-lane
+   offset: one descriptor per array, the row offset in an SGPR.  Lanes whose
+   per-lane offset is out of range read 0 and do not write. */
+template <typename real>
+__device__ __forceinline__ real load_real_s(rsrc_t r, uint32_t off, uint32_t soff);
+template <>
+__device__ __forceinline__ float load_real_s<float>(rsrc_t r, uint32_t off, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+}
+template <>
+__device__ __forceinline__ double load_real_s<double>(rsrc_t r, uint32_t off, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+}
+template <typename real>
+__device__ __forceinline__ typename Cplx<real>::type load_cplx_s(rsrc_t r, uint32_t off, uint32_t soff);
+template <>
+__device__ __forceinline__ float2 load_cplx_s<float>(rsrc_t r, uint32_t off, uint32_t soff) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+}
+template <>
+__device__ __forceinline__ double2 load_cplx_s<double>(rsrc_t r, uint32_t off, uint32_t soff) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
+}
+typedef int phx_v2i __attribute__((ext_vector_type(2)));
+typedef int phx_v4i __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_real_s(rsrc_t r, uint32_t off, uint32_t soff, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, off, soff, 0);
+}
+__device__ __forceinline__ void store_real_s(rsrc_t r, uint32_t off, uint32_t soff, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(phx_v2i, v), r, off, soff, 0);
+}
+__device__ __forceinline__ void store_cplx_s(rsrc_t r, uint32_t off, uint32_t soff, float2 v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(phx_v2i, v), r, off, soff, 0);
+}
+__device__ __forceinline__ void store_cplx_s(rsrc_t r, uint32_t off, uint32_t soff, double2 v) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(phx_v4i, v), r, off, soff, 0);
+}
+
+/* Largest array the whole-array descriptors address: offsets are 32-bit and the
+   out-of-range marker kOutOfRange must stay beyond every valid offset. */
+constexpr size_t kMaxArrayBytes = 0x7FFFFFFFu;
+
 } // namespace phx

@@ -19,6 +19,7 @@
 
 #include "../../include/phx.h"
 #include "phx_config.hpp"
+#include "phx_device.h"
 #include "phx_kernels.h"
 
 namespace phx {
@@ -279,6 +280,10 @@ int phx_ensemble_create(phx_ensemble** out, int precision, const phx_physics* ph
         if (N_c[i] <= 0 || N_r[i] <= 0) {
             delete e;
             return efail(PHX_ERR_INVALID, "instance " + std::to_string(i) + " has an empty grid");
+        }
+        if ((size_t)N_c[i] * (size_t)N_r[i] * 2 * e->rsize > phx::kMaxArrayBytes) {
+            delete e;
+            return efail(PHX_ERR_INVALID, "instance " + std::to_string(i) + " exceeds 2 GiB per field");
         }
         e->Nc.push_back(N_c[i]);
         e->Nr.push_back(N_r[i]);

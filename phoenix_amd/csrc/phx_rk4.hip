@@ -41,17 +41,6 @@ namespace phx {
 constexpr int kHalo = 4;
 constexpr int kOutW = kWave - 2 * kHalo; /* 56 */
 
-template <typename real>
-__device__ __forceinline__ void ld_real(rsrc_t r, uint32_t off, real& x) {
-    x = load_real<real>(r, off);
-}
-template <typename real>
-__device__ __forceinline__ void ld_cplx(rsrc_t r, uint32_t off, real& re, real& im) {
-    const auto v = load_cplx<real>(r, off);
-    re = v.x;
-    im = v.y;
-}
-
 /* Row index of a tile row: inside the grid, wrapped once (periodic; a grid shorter
    than the tile's reach falls back to a full modulo), or outside (zero boundary). */
 __device__ __forceinline__ int tile_row(int g, int Nr, int periodic, bool& in) {
@@ -90,9 +79,16 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
     }
     const uint32_t off_r = col_in ? (uint32_t)gcol * (uint32_t)sizeof(real) : kOutOfRange;
     const uint32_t off_c = col_in ? (uint32_t)gcol * (uint32_t)sizeof(v2) : kOutOfRange;
+    /* store offsets: halo lanes and lanes past the last column write nothing */
     const bool out_lane = lane >= kHalo && lane < kWave - kHalo && (x0 - kHalo + lane) < Nc;
-    const uint32_t rbytes = (uint32_t)Nc * (uint32_t)sizeof(real);
-    const uint32_t cbytes = (uint32_t)Nc * (uint32_t)sizeof(v2);
+    const uint32_t so_r = out_lane ? (uint32_t)(x0 - kHalo + lane) * (uint32_t)sizeof(real) : kOutOfRange;
+    const uint32_t so_c = out_lane ? (uint32_t)(x0 - kHalo + lane) * (uint32_t)sizeof(v2) : kOutOfRange;
+    /* one descriptor per array; a row is an SGPR offset.  Rows outside a zero-BC
+       domain read through a descriptor of size 0 (zeros). */
+    const uint32_t row_r = (uint32_t)Nc * (uint32_t)sizeof(real);
+    const uint32_t row_c = (uint32_t)Nc * (uint32_t)sizeof(v2);
+    const uint32_t all_r = row_r * (uint32_t)Nr, all_c = row_c * (uint32_t)Nr;
+    const rsrc_t w_psi = row_rsrc(a.psi_out, all_c), w_n = row_rsrc(a.n_out, all_r);
 
     constexpr int E = TY + 2 * kHalo;
     real p0r[E], p0i[E], n0[E], pp[E], vv[E];
@@ -111,12 +107,15 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
 
     auto load_row = [&](int r) {
         bool in;
-        const int grow = tile_row(y0 - kHalo + r, Nr, a.periodic_y, in); /* wave-uniform */
-        const size_t row = (size_t)grow * (size_t)Nc;
-        ld_cplx(row_rsrc(a.psi_in + row, in ? cbytes : 0u), off_c, p0r[r], p0i[r]);
-        if (RES) ld_real(row_rsrc(a.n_in + row, in ? rbytes : 0u), off_r, n0[r]);
-        if (PUMP) ld_real(row_rsrc(a.pump + row, in ? rbytes : 0u), off_r, pp[r]);
-        if (POT) ld_real(row_rsrc(a.pot + row, in ? rbytes : 0u), off_r, vv[r]);
+        const uint32_t grow = (uint32_t)tile_row(y0 - kHalo + r, Nr, a.periodic_y, in); /* wave-uniform */
+        const uint32_t sc = grow * row_c, sr = grow * row_r;
+        const uint32_t nc = in ? all_c : 0u, nr = in ? all_r : 0u;
+        const auto x = load_cplx_s<real>(row_rsrc(a.psi_in, nc), off_c, sc);
+        p0r[r] = x.x;
+        p0i[r] = x.y;
+        if (RES) n0[r] = load_real_s<real>(row_rsrc(a.n_in, nr), off_r, sr);
+        if (PUMP) pp[r] = load_real_s<real>(row_rsrc(a.pump, nr), off_r, sr);
+        if (POT) vv[r] = load_real_s<real>(row_rsrc(a.pot, nr), off_r, sr);
     };
 
     /* neighbour part of the kinetic stencil: centre c, north+south ns */
@@ -195,13 +194,12 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
                 rhs(y3r[e], y3i[e], y3r[e - 1] + y3r[e + 1], y3i[e - 1] + y3i[e + 1], RES ? yn3[e] : real(0),
                     pp[e] * a.pamp[3], vv[e] * a.vamp_h[3], kr, ki, kn);
                 const int grow = y0 + e - kHalo;
-                if (out_lane && grow < Nr) {
-                    const size_t idx = (size_t)grow * (size_t)Nc + (size_t)(x0 - kHalo + lane);
+                if (grow < Nr) {
                     v2 o;
                     o.x = phx_fma(dt6, kr, acr[e]);
                     o.y = phx_fma(dt6, ki, aci[e]);
-                    a.psi_out[idx] = o;
-                    if (RES) a.n_out[idx] = phx_fma(dt6, kn, acn[e]);
+                    store_cplx_s(w_psi, so_c, (uint32_t)grow * row_c, o);
+                    if (RES) store_real_s(w_n, so_r, (uint32_t)grow * row_r, phx_fma(dt6, kn, acn[e]));
                 }
             }
         }
@@ -263,37 +261,45 @@ __global__ __launch_bounds__(256) void rk4_batched_fused(Rk4BatchArgs<real> b) {
     rk4_tile<real, TY, ISO, PF, RES, PUMP, POT>(a, local % a.nstrips, local / a.nstrips);
 }
 
-template <typename real, int TY, bool ISO, bool RES, bool PUMP, bool POT>
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT>
 static hipError_t launch_ty(const Rk4Args<real>& a0, hipStream_t s) {
     Rk4Args<real> a = a0;
     a.nstrips = (a.Nc + kOutW - 1) / kOutW;
     a.nbands = (a.Nr + TY - 1) / TY;
     const int bandgroups = (a.nbands + kWavesPerBlock - 1) / kWavesPerBlock;
     const int nblocks = a.nstrips * bandgroups;
-    hipLaunchKernelGGL((rk4_scalar_fused<real, TY, ISO, 2, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, a);
+    hipLaunchKernelGGL((rk4_scalar_fused<real, TY, ISO, PF, RES, PUMP, POT>), dim3(nblocks), dim3(kWave * kWavesPerBlock), 0, s, a);
     return hipGetLastError();
 }
 
 /* Tile height TY in {8, 10, 12, 16, 20, 24} (others: 16). */
-template <typename real, bool ISO, bool RES, bool PUMP, bool POT>
+template <typename real, bool ISO, int PF, bool RES, bool PUMP, bool POT>
 static hipError_t launch_tys(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
     switch (t.ty) {
-        case 8: return launch_ty<real, 8, ISO, RES, PUMP, POT>(a, s);
-        case 10: return launch_ty<real, 10, ISO, RES, PUMP, POT>(a, s);
-        case 12: return launch_ty<real, 12, ISO, RES, PUMP, POT>(a, s);
-        case 20: return launch_ty<real, 20, ISO, RES, PUMP, POT>(a, s);
-        case 24: return launch_ty<real, 24, ISO, RES, PUMP, POT>(a, s);
-        default: return launch_ty<real, 16, ISO, RES, PUMP, POT>(a, s);
+        case 8: return launch_ty<real, 8, ISO, PF, RES, PUMP, POT>(a, s);
+        case 10: return launch_ty<real, 10, ISO, PF, RES, PUMP, POT>(a, s);
+        case 12: return launch_ty<real, 12, ISO, PF, RES, PUMP, POT>(a, s);
+        case 20: return launch_ty<real, 20, ISO, PF, RES, PUMP, POT>(a, s);
+        case 24: return launch_ty<real, 24, ISO, PF, RES, PUMP, POT>(a, s);
+        default: return launch_ty<real, 16, ISO, PF, RES, PUMP, POT>(a, s);
     }
 }
 
+/* Rows loaded ahead: 2, or 3 / 4 for the pumped reservoir model with dx == dy
+   (the benchmark configuration; chosen through Rk4Tune::pf). */
 template <typename real, bool RES, bool PUMP, bool POT>
 static hipError_t launch_flags(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
-    return a.ph.kin_yx == real(1) ? launch_tys<real, true, RES, PUMP, POT>(a, t, s) : launch_tys<real, false, RES, PUMP, POT>(a, t, s);
+    const bool iso = a.ph.kin_yx == real(1);
+    if constexpr (RES && PUMP && !POT) {
+        if (iso && t.pf == 3) return launch_tys<real, true, 3, RES, PUMP, POT>(a, t, s);
+        if (iso && t.pf >= 4) return launch_tys<real, true, 4, RES, PUMP, POT>(a, t, s);
+    }
+    return iso ? launch_tys<real, true, 2, RES, PUMP, POT>(a, t, s) : launch_tys<real, false, 2, RES, PUMP, POT>(a, t, s);
 }
 
 template <typename real>
 hipError_t launch_rk4_scalar(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
+    if ((size_t)a.Nc * (size_t)a.Nr * sizeof(typename Vec2<real>::type) > kMaxArrayBytes) return hipErrorInvalidValue;
     const bool res = a.n_in != nullptr;
     const bool pump = res && a.pump != nullptr;
     const bool pot = a.pot != nullptr;
@@ -331,6 +337,9 @@ static hipError_t launch_batch_flags(const Rk4BatchArgs<real>& b, int ty, hipStr
 
 template <typename real>
 hipError_t launch_rk4_batched(const Rk4BatchArgs<real>& b, int ty, bool res, bool pump, bool pot, hipStream_t s) {
+    /* (instance sizes are checked when the ensemble is created) */
+    if (!b.entries && (size_t)b.common.Nc * (size_t)b.common.Nr * sizeof(typename Vec2<real>::type) > kMaxArrayBytes)
+        return hipErrorInvalidValue;
     if (!res) return pot ? launch_batch_flags<real, false, false, true>(b, ty, s) : launch_batch_flags<real, false, false, false>(b, ty, s);
     if (pump) return pot ? launch_batch_flags<real, true, true, true>(b, ty, s) : launch_batch_flags<real, true, true, false>(b, ty, s);
     return pot ? launch_batch_flags<real, true, false, true>(b, ty, s) : launch_batch_flags<real, true, false, false>(b, ty, s);

@@ -352,11 +352,15 @@ int autotune_rk4(phx_solver* s) {
     PHX_HIP(hipEventCreate(&e0));
     PHX_HIP(hipEventCreate(&e1));
     const int cands[] = {8, 10, 12, 16, 20, 24};
+    /* deeper row prefetch exists for the pumped reservoir model */
+    const int npf = (c.use_reservoir && c.use_pumps && !c.use_potentials) ? 3 : 1;
     float best = 1e30f;
     phx::Rk4Tune best_t = s->tune;
-    for (int ty : cands) {
+    for (int cand = 0; cand < 6 * npf; cand++) {
+        const int ty = cands[cand % 6];
         phx::Rk4Tune t = s->tune;
         t.ty = ty;
+        t.pf = 2 + cand / 6;
         const phx::Rk4Tune keep = s->tune;
         s->tune = t;
         int rc = PHX_OK;
@@ -624,6 +628,8 @@ int phx_get_info(phx_solver* s, phx_info* info) {
     info->dt_max = c.dt_max;
     info->iteration = s->iteration;
     info->rejected = s->rejected;
+    info->tile_rows = s->tuned ? s->tune.ty : 0;
+    info->prefetch_rows = s->tuned ? s->tune.pf : 0;
     return PHX_OK;
 }
 
