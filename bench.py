@@ -315,6 +315,8 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-steps", type=int, default=1000)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--only-legs", action="store_true",
+                    help="run only the batched-ensemble and adjoint legs (profiling); prints their objects")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--copy-mb", type=int, default=0,
@@ -333,6 +335,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_per_launch": src.numel() * 4, "launches": 10}))
+        return
+    if a.only_legs:
+        print(json.dumps({"batched": time_batched(a.batch_instances, a.batch_steps),
+                          "adjoint": time_adjoint(steps=a.adjoint_steps)}), flush=True)
         return
     N = a.size
     L = 0.5 * (N - 1)
