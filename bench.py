@@ -172,7 +172,7 @@ def time_batched(instances: int, steps: int, seed: int = 99, graph_steps: int = 
     dev = torch.device("cuda:0")
     seq = []
     for i, N in enumerate(sides):
-        psi, n, P = ensemble_inputs(i, int(N), float(amps[i]), seed * 1000 + i)
+        psi, n, P = ensemble_inputs(i, int(N), float(amps[i]), 1000 + i)
         ens.set(i, "psi_plus", psi)
         ens.set(i, "n_plus", n)
         ens.set(i, "pump_plus", P)
@@ -209,34 +209,38 @@ def time_batched(instances: int, steps: int, seed: int = 99, graph_steps: int = 
             "hbm_frac": round(achieved / HBM_PEAK_GBS, 4), "bitwise_equal_first_%d" % min(check, instances): same}
 
 
-def smooth_random_field(rng, N: int, L: float, count: int, amp: float, width: tuple):
-    """Sum of `count` Gaussians with random centres, widths in `width` and amplitudes in [0, amp]."""
+def smooth_random_field(rng, N: int, L: float, count: int, amp: tuple, width: tuple):
+    """Sum of `count` Gaussians with centres uniform over the domain, widths uniform in `width`
+    and amplitudes uniform in `amp`."""
     x = -L / 2 + (L / (N - 1)) * np.arange(N)
     X, Y = np.meshgrid(x, x)
     f = np.zeros((N, N))
     for _ in range(count):
-        cx, cy = rng.uniform(-L / 3, L / 3, 2)
+        cx, cy = rng.uniform(-L / 2, L / 2, 2)
         w = rng.uniform(*width)
-        f += rng.uniform(0, amp) * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * w * w))
+        f += rng.uniform(*amp) * np.exp(-((X - cx) ** 2 + (Y - cy) ** 2) / (2 * w * w))
     return f
 
 
-def time_adjoint(N: int = 512, steps: int = 2000, K: int = 50, batch: int = 16, seed: int = 7):
-    """Gradient of J = sum_seeds int_target |psi(T)|^2 dV (target: 20 um disk 60 um off centre) with respect to
-    P(x,y) and V(x,y) (smooth random fields, 32 Gaussians each), FP32, `batch` noise seeds: forward with a
-    checkpoint every K steps, then the adjoint backward.  Compared with the plain forward of the same batch."""
+def time_adjoint(N: int = 512, steps: int = 2000, K: int = 50, batch: int = 16, seed: int = 77):
+    """Gradient of J = sum_seeds int_target |psi(T)|^2 dV (target: disk of radius 20 um centred 60 um from the
+    middle) with respect to P(x,y) and V(x,y), FP32, `batch` noise seeds (2000 + b): forward with a checkpoint
+    every K steps, then the adjoint backward.  P and V are sums of 32 Gaussians (centres uniform over the
+    domain, widths uniform in [5, 20] um, seed 77); P is scaled to a maximum of 1.5 P_th, V has amplitudes
+    uniform in [-0.5, 0.5] meV.  Compared with the plain forward of the same batch."""
     import torch
     from phoenix_amd import _lib as Lb
     from phoenix_amd.adjoint import release_engines, rk4_evolve
     L = 0.5 * (N - 1)
     dt = 0.04
     rng = np.random.default_rng(seed)
-    P = smooth_random_field(rng, N, L, 32, 0.5 * PTH, (8.0, 30.0))
-    V = smooth_random_field(rng, N, L, 32, 4e-4, (5.0, 20.0))
-    psi0 = np.stack([ensemble_inputs(0, N, 0.0, seed * 100 + b)[0] for b in range(batch)])
+    P = smooth_random_field(rng, N, L, 32, (0.0, 1.0), (5.0, 20.0))
+    P *= 1.5 * PTH / P.max()
+    V = smooth_random_field(rng, N, L, 32, (-5e-4, 5e-4), (5.0, 20.0))  # 1 code energy unit = 1 eV
+    psi0 = np.stack([ensemble_inputs(0, N, 0.0, 2000 + b)[0] for b in range(batch)])
     x = -L / 2 + 0.5 * np.arange(N)
     X, Y = np.meshgrid(x, x)
-    target = ((X - 60.0) ** 2 + Y ** 2 < 10.0 ** 2).astype(np.float32)
+    target = ((X - 60.0) ** 2 + Y ** 2 <= 20.0 ** 2).astype(np.float32)
     ph, _ = Lb.physics_defaults(N, N, L, L)
     dev = torch.device("cuda:0")
     tP = torch.tensor(P, dtype=torch.float32, device=dev, requires_grad=True)

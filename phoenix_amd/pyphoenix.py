@@ -12,6 +12,13 @@ Differences, all on the MI355X side:
   - there is no CPU executable and no download fallback: a missing binary is an error.
   - `run_inprocess(config)` runs the same configuration through the C ABI in this
     process and returns the solver handle (no subprocess, fields as numpy arrays).
+
+Additions that the reference wrapper does not have:
+  - `pyphoenix.rk4_evolve(P, V, psi0, n0, physics=..., dt=..., steps=..., checkpoint_every=50)`:
+    differentiable RK4 evolution (torch.autograd.Function `RK4Adjoint` over the hand-written
+    adjoint kernel) giving dJ/dP, dJ/dV, dJ/dpsi0, dJ/dn0 for any objective J of the final state;
+  - `pyphoenix.ensemble(sizes, ...)`: uneven grids stepped together, one launch per step;
+  - `pyphoenix.physics(N_c, N_r, L_x, L_y)`: the reference's default constants for a grid.
 """
 from __future__ import annotations
 
@@ -121,6 +128,23 @@ class pyphoenix:
             print(self.stderr)
         return process.returncode
 
+    @staticmethod
+    def physics(N_c, N_r, L_x, L_y):
+        """Reference default physical constants for a grid; returns (phx_physics, magic dt)."""
+        from ._lib import physics_defaults
+        return physics_defaults(N_c, N_r, L_x, L_y)
+
+    @staticmethod
+    def rk4_evolve(pump, pot, psi0, n0, **kw):
+        """Differentiable RK4 evolution (see phoenix_amd.adjoint.rk4_evolve)."""
+        from .adjoint import rk4_evolve
+        return rk4_evolve(pump, pot, psi0, n0, **kw)
+
+    def ensemble(self, sizes, physics, spacing=None, **kw):
+        """Batched ensemble of grids of the given (N_c, N_r) sizes (see phoenix_amd._lib.Ensemble)."""
+        from ._lib import Ensemble
+        return Ensemble(self.precision, physics, sizes, spacing, **kw)
+
     def run_inprocess(self, config, steps=None):
         """Run `config` through the C ABI in this process; returns the open Solver."""
         from ._lib import Solver
@@ -131,3 +155,11 @@ class pyphoenix:
         else:
             s.step(steps)
         return s
+
+
+def __getattr__(name):
+    # torch is imported only when the differentiable path is asked for
+    if name in ("RK4Adjoint", "rk4_evolve"):
+        from . import adjoint
+        return getattr(adjoint, name)
+    raise AttributeError(name)
