@@ -15,8 +15,9 @@
  * (cuFFT/FFTW convention).  Algorithm: Stockham autosort, radix 4 with a final
  * radix-2 pass when log2 N is odd (textbook form, e.g. Van Loan, "Computational
  * Frameworks for the FFT", 1992).  Twiddles come from a table computed on the
- * host in double precision.  Supported lengths: powers of two from 4 to 4096
- * (FP32) / 2048 (FP64), as two LDS buffers of one row must fit in 64 KiB.
+ * host in double precision.  The passes run in place in one LDS buffer (inputs of
+ * a pass staged in registers), so a row of up to 8192 (FP32) / 4096 (FP64) points
+ * fits in 64 KiB; column blocks hold 8 columns where they fit.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -38,118 +39,143 @@ __device__ __forceinline__ Cx<real> cmul(Cx<real> a, Cx<real> w) {
     return {phx_fma(a.r, w.r, -(a.i * w.i)), phx_fma(a.r, w.i, a.i * w.r)};
 }
 
-/* One Stockham pass group over `count` independent length-N sequences stored
-   back to back in src (sequence j at src + j*N).  Returns the buffer holding
-   the result.  sign = -1 forward, +1 inverse (conjugated twiddles). */
-template <typename real, int SIGN>
-__device__ Cx<real>* stockham(Cx<real>* src, Cx<real>* dst, const Cx<real>* __restrict__ tw, int N, int log2N, int count) {
+/* Butterflies a thread may own in one pass: the launchers pick KB, the smallest
+   power of two with count * N / 4 <= KB * threads, up to 8 in FP32 and 4 in FP64,
+   so the staged inputs stay in registers. */
+template <typename real>
+__host__ __device__ constexpr int max_bfly() { return sizeof(real) == 4 ? 8 : 4; }
+
+/* Stockham autosort FFT of `count` independent length-N sequences stored back to
+   back in one LDS buffer (sequence j at buf + j*N), in place: in every radix-4 pass
+   each thread first reads the inputs of its butterflies into registers, the block
+   synchronises, and then the outputs are written to their Stockham positions in the
+   same buffer.  One buffer instead of two doubles the sequences a workgroup holds.
+   sign = -1 forward, +1 inverse (conjugated twiddles). */
+template <typename real, int SIGN, int KB>
+__device__ void stockham(Cx<real>* buf, const Cx<real>* __restrict__ tw, int N, int log2N, int count) {
+    const int nb = N >> 2; /* radix-4 butterflies per sequence */
+    const int total = nb * count;
     int n = N, log2n = log2N, s = 1, log2s = 0;
     while (n >= 4) {
         const int m = n >> 2;
         const int stepshift = log2N - log2n; /* twiddle index p * N / n */
-        const int nb = N >> 2;              /* butterflies per sequence */
-        for (int b = threadIdx.x; b < nb * count; b += blockDim.x) {
-            const int seq = b / nb;
-            const int bb = b - seq * nb;
-            const int p = bb >> log2s;
-            const int q = bb & (s - 1);
-            const Cx<real>* x = src + seq * N;
-            Cx<real>* y = dst + seq * N;
-            const Cx<real> a = x[q + s * p], bv = x[q + s * (p + m)], c = x[q + s * (p + 2 * m)], d = x[q + s * (p + 3 * m)];
-            Cx<real> w1 = tw[p << stepshift], w2 = tw[(2 * p) << stepshift], w3 = tw[(3 * p) << stepshift];
-            if (SIGN > 0) {
-                w1.i = -w1.i;
-                w2.i = -w2.i;
-                w3.i = -w3.i;
+        Cx<real> in[KB][4];
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const int b = threadIdx.x + k * kFftThreads;
+            if (b < total) {
+                const int seq = b / nb, bb = b - seq * nb;
+                const int p = bb >> log2s, q = bb & (s - 1);
+                const Cx<real>* x = buf + seq * N;
+                in[k][0] = x[q + s * p];
+                in[k][1] = x[q + s * (p + m)];
+                in[k][2] = x[q + s * (p + 2 * m)];
+                in[k][3] = x[q + s * (p + 3 * m)];
             }
-            const Cx<real> apc = {a.r + c.r, a.i + c.i}, amc = {a.r - c.r, a.i - c.i};
-            const Cx<real> bpd = {bv.r + d.r, bv.i + d.i};
-            /* j(b - d) with j = -i (forward) or +i (inverse) */
-            const Cx<real> jbmd = SIGN < 0 ? Cx<real>{bv.i - d.i, -(bv.r - d.r)} : Cx<real>{-(bv.i - d.i), bv.r - d.r};
-            y[q + s * (4 * p)] = {apc.r + bpd.r, apc.i + bpd.i};
-            y[q + s * (4 * p + 1)] = cmul(Cx<real>{amc.r + jbmd.r, amc.i + jbmd.i}, w1);
-            y[q + s * (4 * p + 2)] = cmul(Cx<real>{apc.r - bpd.r, apc.i - bpd.i}, w2);
-            y[q + s * (4 * p + 3)] = cmul(Cx<real>{amc.r - jbmd.r, amc.i - jbmd.i}, w3);
         }
         __syncthreads();
-        Cx<real>* t = src;
-        src = dst;
-        dst = t;
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const int b = threadIdx.x + k * kFftThreads;
+            if (b < total) {
+                const int seq = b / nb, bb = b - seq * nb;
+                const int p = bb >> log2s, q = bb & (s - 1);
+                Cx<real>* y = buf + seq * N;
+                const Cx<real> a = in[k][0], bv = in[k][1], c = in[k][2], d = in[k][3];
+                Cx<real> w1 = tw[p << stepshift], w2 = tw[(2 * p) << stepshift], w3 = tw[(3 * p) << stepshift];
+                if (SIGN > 0) {
+                    w1.i = -w1.i;
+                    w2.i = -w2.i;
+                    w3.i = -w3.i;
+                }
+                const Cx<real> apc = {a.r + c.r, a.i + c.i}, amc = {a.r - c.r, a.i - c.i};
+                const Cx<real> bpd = {bv.r + d.r, bv.i + d.i};
+                /* j(b - d) with j = -i (forward) or +i (inverse) */
+                const Cx<real> jbmd = SIGN < 0 ? Cx<real>{bv.i - d.i, -(bv.r - d.r)} : Cx<real>{-(bv.i - d.i), bv.r - d.r};
+                y[q + s * (4 * p)] = {apc.r + bpd.r, apc.i + bpd.i};
+                y[q + s * (4 * p + 1)] = cmul(Cx<real>{amc.r + jbmd.r, amc.i + jbmd.i}, w1);
+                y[q + s * (4 * p + 2)] = cmul(Cx<real>{apc.r - bpd.r, apc.i - bpd.i}, w2);
+                y[q + s * (4 * p + 3)] = cmul(Cx<real>{amc.r - jbmd.r, amc.i - jbmd.i}, w3);
+            }
+        }
+        __syncthreads();
         n = m;
         log2n -= 2;
         s <<= 2;
         log2s += 2;
     }
     if (n == 2) {
+        /* final radix-2 pass: the butterfly (q, q + s) reads and writes the same two slots */
         for (int b = threadIdx.x; b < s * count; b += blockDim.x) {
             const int seq = b >> log2s;
             const int q = b & (s - 1);
-            const Cx<real>* x = src + seq * N;
-            Cx<real>* y = dst + seq * N;
+            Cx<real>* x = buf + seq * N;
             const Cx<real> a = x[q], bv = x[q + s];
-            y[q] = {a.r + bv.r, a.i + bv.i};
-            y[q + s] = {a.r - bv.r, a.i - bv.i};
+            x[q] = {a.r + bv.r, a.i + bv.i};
+            x[q + s] = {a.r - bv.r, a.i - bv.i};
         }
         __syncthreads();
-        Cx<real>* t = src;
-        src = dst;
-        dst = t;
     }
-    return src;
 }
 
-template <typename real, int SIGN>
+template <typename real, int SIGN, int KB>
 __global__ __launch_bounds__(kFftThreads) void fft_rows(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw, int N, int log2N) {
     extern __shared__ unsigned char lds_raw[];
     Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
-    Cx<real>* B = A + N;
     typename Vec2<real>::type* row = data + (size_t)blockIdx.x * (size_t)N;
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
         const auto v = row[i];
         A[i] = {v.x, v.y};
     }
     __syncthreads();
-    const Cx<real>* R = stockham<real, SIGN>(A, B, tw, N, log2N, 1);
+    stockham<real, SIGN, KB>(A, tw, N, log2N, 1);
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
         typename Vec2<real>::type v;
-        v.x = R[i].r;
-        v.y = R[i].i;
+        v.x = A[i].r;
+        v.y = A[i].i;
         row[i] = v;
     }
 }
 
+/* Column groups are handed out XCD by XCD (block b runs on XCD b % 8): consecutive
+   groups share cache lines, so they go to the same L2. */
+__device__ __forceinline__ int fft_xcd_group(int b, int nblocks) {
+    const int q = nblocks >> 3, r = nblocks & 7;
+    const int x = b & 7;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 /* CB adjacent columns: forward FFT along the column, F <- F / (N_c N_r) * mask[shifted], inverse FFT. */
-template <typename real, int CB>
+template <typename real, int CB, int KB>
 __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw,
                                                               const real* __restrict__ mask, int Nc, int Nr, int log2Nr, real Ntot) {
     extern __shared__ unsigned char lds_raw[];
     Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
-    Cx<real>* B = A + CB * Nr;
-    const int c0 = blockIdx.x * CB;
+    const int c0 = fft_xcd_group(blockIdx.x, gridDim.x) * CB;
     for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
         const int r = i / CB, j = i - r * CB; /* consecutive threads read consecutive columns */
         const auto v = data[(size_t)r * Nc + c0 + j];
         A[j * Nr + r] = {v.x, v.y};
     }
     __syncthreads();
-    Cx<real>* F = stockham<real, -1>(A, B, tw, Nr, log2Nr, CB);
-    Cx<real>* G = F == A ? B : A;
+    stockham<real, -1, KB>(A, tw, Nr, log2Nr, CB);
     /* mask of the quadrant-swapped grid (fft_shift_2D, kernel_fft.cuh:15-32), applied as data / N * mask */
     for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
-        const int j = i / Nr, r = i - j * Nr;
+        const int r = i / CB, j = i - r * CB; /* row-major walk: consecutive threads read consecutive mask cells */
         const int rs = (r + (Nr >> 1)) & (Nr - 1);
         const int cs = (c0 + j + (Nc >> 1)) & (Nc - 1);
         const real m = mask[(size_t)rs * Nc + cs];
-        F[i].r = F[i].r / Ntot * m;
-        F[i].i = F[i].i / Ntot * m;
+        Cx<real>& f = A[j * Nr + r];
+        f.r = f.r / Ntot * m;
+        f.i = f.i / Ntot * m;
     }
     __syncthreads();
-    const Cx<real>* R = stockham<real, +1>(F, G, tw, Nr, log2Nr, CB);
+    stockham<real, +1, KB>(A, tw, Nr, log2Nr, CB);
     for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
         const int r = i / CB, j = i - r * CB;
         typename Vec2<real>::type v;
-        v.x = R[j * Nr + r].r;
-        v.y = R[j * Nr + r].i;
+        v.x = A[j * Nr + r].r;
+        v.y = A[j * Nr + r].i;
         data[(size_t)r * Nc + c0 + j] = v;
     }
 }
@@ -162,35 +188,78 @@ static int ilog2(int n) {
 
 bool fft_supported(int N, size_t real_size) {
     if (N < 4 || (N & (N - 1))) return false;
-    return 2 * (size_t)N * 2 * real_size <= 65536;
+    /* one LDS row of N complex values, and at most max_bfly butterflies per thread */
+    const int mb = real_size == 4 ? max_bfly<float>() : max_bfly<double>();
+    return (size_t)N * 2 * real_size <= 65536 && N / 4 <= mb * kFftThreads;
+}
+
+/* smallest power of two KB with butterflies <= KB * threads */
+static int bfly_bound(int butterflies) {
+    int kb = 1;
+    while (kb * kFftThreads < butterflies) kb <<= 1;
+    return kb;
+}
+
+template <typename real, int SIGN>
+static hipError_t launch_rows(typename Vec2<real>::type* psi, const void* tw_c, int Nc, int Nr, hipStream_t s) {
+    const size_t lds = (size_t)Nc * sizeof(Cx<real>);
+    const int kb = bfly_bound(Nc / 4);
+    const auto* tw = (const Cx<real>*)tw_c;
+    const int l = ilog2(Nc);
+    if (kb == 1)
+        hipLaunchKernelGGL((fft_rows<real, SIGN, 1>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
+    else if (kb == 2)
+        hipLaunchKernelGGL((fft_rows<real, SIGN, 2>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
+    else if (kb == 4)
+        hipLaunchKernelGGL((fft_rows<real, SIGN, 4>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
+    else if constexpr (max_bfly<real>() >= 8)
+        hipLaunchKernelGGL((fft_rows<real, SIGN, 8>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
 }
 
 template <typename real, int CB>
 static hipError_t launch_cols(typename Vec2<real>::type* psi, const void* tw_r, const real* mask, int Nc, int Nr, hipStream_t s) {
-    const size_t lds = 2 * (size_t)CB * Nr * sizeof(Cx<real>);
-    hipLaunchKernelGGL((fft_cols_filter<real, CB>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, (const Cx<real>*)tw_r, mask, Nc, Nr,
-                       ilog2(Nr), (real)((double)Nc * (double)Nr));
+    const size_t lds = (size_t)CB * Nr * sizeof(Cx<real>);
+    const int kb = bfly_bound(CB * (Nr / 4));
+    const auto* tw = (const Cx<real>*)tw_r;
+    const int l = ilog2(Nr);
+    const real ntot = (real)((double)Nc * (double)Nr);
+    if (kb == 1)
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 1>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, Nc, Nr, l, ntot);
+    else if (kb == 2)
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 2>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, Nc, Nr, l, ntot);
+    else if (kb == 4)
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 4>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, Nc, Nr, l, ntot);
+    else if constexpr (max_bfly<real>() >= 8)
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 8>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, Nc, Nr, l, ntot);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
 template <typename real>
 hipError_t launch_fft_filter(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, int Nc, int Nr,
                              hipStream_t s) {
-    const size_t row_lds = 2 * (size_t)Nc * sizeof(Cx<real>);
-    hipLaunchKernelGGL((fft_rows<real, -1>), dim3(Nr), dim3(kFftThreads), row_lds, s, psi, (const Cx<real>*)tw_c, Nc, ilog2(Nc));
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_rows<real, -1>(psi, tw_c, Nc, Nr, s);
     if (e != hipSuccess) return e;
-    /* widest column block whose two LDS buffers fit in 64 KiB */
-    const size_t per_col = 2 * (size_t)Nr * sizeof(Cx<real>);
-    if (per_col * 4 <= 65536 && Nc % 4 == 0)
+    /* column block width: as wide as LDS and registers allow, but no wider than keeps
+       512 workgroups in flight (the machine has 256 CUs) */
+    const size_t per_col = (size_t)Nr * sizeof(Cx<real>);
+    auto fits = [&](int cb) {
+        return per_col * cb <= 65536 && Nc % cb == 0 && cb * (Nr / 4) <= max_bfly<real>() * kFftThreads && (cb == 1 || Nc / cb >= 512);
+    };
+    if (fits(8))
+        e = launch_cols<real, 8>(psi, tw_r, mask, Nc, Nr, s);
+    else if (fits(4))
         e = launch_cols<real, 4>(psi, tw_r, mask, Nc, Nr, s);
-    else if (per_col * 2 <= 65536 && Nc % 2 == 0)
+    else if (fits(2))
         e = launch_cols<real, 2>(psi, tw_r, mask, Nc, Nr, s);
     else
         e = launch_cols<real, 1>(psi, tw_r, mask, Nc, Nr, s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((fft_rows<real, +1>), dim3(Nr), dim3(kFftThreads), row_lds, s, psi, (const Cx<real>*)tw_c, Nc, ilog2(Nc));
-    return hipGetLastError();
+    return launch_rows<real, +1>(psi, tw_c, Nc, Nr, s);
 }
 
 template hipError_t launch_fft_filter<float>(float2*, const void*, const void*, const float*, int, int, hipStream_t);

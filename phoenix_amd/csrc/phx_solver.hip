@@ -144,7 +144,7 @@ int initialize_t(phx_solver* s) {
     if (const char* e = std::getenv("PHX_MAX_REJECTIONS")) s->max_rejections = std::atoi(e);
     if (c.use_fft_mask && (!phx::fft_supported(c.N_c, sizeof(real)) || !phx::fft_supported(c.N_r, sizeof(real))))
         return fail(PHX_ERR_UNSUPPORTED, "--fftMask needs power-of-two grid sizes between 4 and " +
-                                             std::to_string(sizeof(real) == 4 ? 4096 : 2048));
+                                             std::to_string(sizeof(real) == 4 ? 8192 : 4096));
     if (c.use_pumps && c.pump.groupSize() > 1 && !c.pump.allConstant())
         return fail(PHX_ERR_UNSUPPORTED, "more than one temporal pump group");
     if (c.use_potentials && c.potential.groupSize() > 1 && !c.potential.allConstant())

@@ -34,11 +34,14 @@ def make_solver(prec, Nc, Nr, steps=10, dt=0.04, fft_every=None, extra=()):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-@pytest.mark.parametrize("shape", [(64, 64), (128, 32), (16, 256), (1024, 1024), (2048, 512)])
+@pytest.mark.parametrize("shape", [(64, 64), (128, 32), (16, 256), (1024, 1024), (2048, 512), (4096, 32), (8, 8192),
+                                   (8, 4096)])
 def test_fft_filter_matches_oracle(prec, shape):
+    """Row lengths 8 .. 4096 and column lengths 32 .. 8192 cover every column-block width (8, 4, 2, 1)
+    and the final radix-2 pass (odd log2 N)."""
     Nc, Nr = shape
-    if prec == "fp64" and max(shape) > 2048:
-        pytest.skip("FP64 FFT limited to 2048")
+    if prec == "fp64" and max(shape) > 4096:
+        pytest.skip("FP64 FFT limited to 4096 points")
     say(f"[fft filter] prec={prec} shape={shape}")
     rng = np.random.default_rng(Nc * 7 + Nr)
     psi = rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc))
