@@ -136,3 +136,17 @@ def test_ensemble_rejects_invalid_arguments_before_touching_the_device():
     assert "instance 1" in lib.phx_last_error().decode()
     assert lib.phx_ensemble_step(None, 0.04, 1, 0) == L.PHX_ERR_INVALID
     assert lib.phx_ensemble_set_tile_rows(None, 12) == L.PHX_ERR_INVALID
+
+
+def test_executable_rejects_invalid_input_with_exit_code_1(tmp_path):
+    """Invalid input exits with status 1 before any device work (system_validate_input.cpp:53-56)."""
+    exe = os.path.join(ROOT, "phoenix_amd", "bin", "PHOENIX_gpu_fp32")
+    if not os.path.exists(exe):
+        pytest.skip("executables not built")
+    r = subprocess.run([exe, "--N", "64", "64", "--tstep", "-1", "--path", str(tmp_path)], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 1
+    r = subprocess.run([exe, "--N", "64", "64", "--iterator", "SSPRK3", "--path", str(tmp_path)], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode == 1
+    assert "SSPRK3" in (r.stdout + r.stderr)

@@ -1,0 +1,117 @@
+"""GPU tests of the command-line and pyphoenix surfaces (the drop-in for the reference's
+PHOENIX_gpu_{fp32,fp64} executables and pyphoenix.run).
+
+Bars: the executable's wavefunction_plus.txt / reservoir_plus.txt match the FP64 oracle
+(FP64: rel L2 <= 1e-12, FP32: <= 1e-5) for an initial state loaded from a PHOENIX txt
+matrix and an envelope pump given on the command line; the summary line is printed;
+pyphoenix.run returns 0 and leaves the same files.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "phoenix_amd", "bin")
+TOL = {"fp64": 1e-12, "fp32": 1e-5}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def write_txt(path, z, L):
+    """PHOENIX txt matrix (filehandler.cpp:282-307 layout): header, real rows, imaginary rows."""
+    N_r, N_c = z.shape
+    with open(path, "w") as f:
+        f.write(f"# SIZE {N_c} {N_r} LX {L} LY {L} DX {L / (N_c - 1)} DY {L / (N_r - 1)} TIME 0 :: PHOENIX_ MATRIX\n")
+        for part in (z.real, z.imag):
+            for row in part:
+                f.write(" ".join(f"{v:.17g}" for v in row) + "\n")
+
+
+def read_txt(path):
+    with open(path) as f:
+        head = f.readline().split()
+        N_c, N_r = int(head[2]), int(head[3])
+        vals = np.array([float(x) for line in f for x in line.split()])
+    return (vals[:N_c * N_r] + 1j * vals[N_c * N_r:2 * N_c * N_r]).reshape(N_r, N_c)
+
+
+CASE = dict(N=64, L=31.5, steps=40, dt=0.04)
+
+
+def cli_args(tmp, prec_path):
+    N, L, steps, dt = CASE["N"], CASE["L"], CASE["steps"], CASE["dt"]
+    return ["--N", N, N, "--L", L, L, "--boundary", "periodic", "zero", "--tstep", dt, "--tmax", (steps - 0.5) * dt,
+            "--outEvery", 1e9, "--fftEvery", 1e9, "--pump", 5.0, "add", 8, 8, 0, 0, "both", 1, "none", "gauss+noDivide",
+            "--initialState", "load", prec_path, 1, "add", "both", "--path", str(tmp)]
+
+
+def oracle_result(psi0):
+    N, L, steps, dt = CASE["N"], CASE["L"], CASE["steps"], CASE["dt"]
+    dx = L / (N - 1)
+    P = O.envelope(N, N, L, L, dx, dx, 5.0, 8, 8, 0, 0, 1.0, 1 | 8).real
+    p = O.params(N, N, L, L, periodic=(True, False), reservoir=True, pump=True)
+    return O.rk4_run(p, psi0, np.zeros((N, N)), pump=P, dt=dt, steps=steps)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_executable_matches_oracle(prec, tmp_path):
+    N, L = CASE["N"], CASE["L"]
+    say(f"[cli] PHOENIX_gpu_{prec} {N}x{N}")
+    psi0 = O.gaussian_field(N, N, 0.2, 31)
+    init = tmp_path / "psi0.txt"
+    write_txt(init, psi0, L)
+    exe = os.path.join(BIN, f"PHOENIX_gpu_{prec}")
+    r = subprocess.run([exe] + [str(a) for a in cli_args(tmp_path / "out", init)], capture_output=True, text=True,
+                       timeout=120)
+    say(r.stdout[-400:])
+    assert r.returncode == 0, r.stderr
+    assert "Total Runtime" in r.stdout
+    g_psi = read_txt(tmp_path / "out" / "wavefunction_plus.txt")
+    g_n = read_txt(tmp_path / "out" / "reservoir_plus.txt").real
+    o_psi, o_n = oracle_result(psi0)
+    e_psi, e_n = rel_l2(g_psi, o_psi), rel_l2(g_n, o_n)
+    say(f"  rel L2 psi {e_psi:.3e} n {e_n:.3e}")
+    assert e_psi <= TOL[prec] and e_n <= TOL[prec]
+
+
+def test_pyphoenix_run_matches_executable(tmp_path):
+    from phoenix_amd.pyphoenix import pyphoenix
+    N, L = CASE["N"], CASE["L"]
+    say("[pyphoenix] run fp32")
+    psi0 = O.gaussian_field(N, N, 0.2, 32)
+    init = tmp_path / "psi0.txt"
+    write_txt(init, psi0, L)
+    args = cli_args(tmp_path / "unused", init)
+    config = {}
+    i = 0
+    while i < len(args):  # flag -> values, as pyphoenix configs are written
+        key = args[i][2:]
+        j = i + 1
+        while j < len(args) and not str(args[j]).startswith("--"):
+            j += 1
+        if key != "path":
+            vals = [str(v) for v in args[i + 1:j]]
+            config[key] = vals if len(vals) > 1 else vals[0]
+        i = j
+    p = pyphoenix(precision="fp32", result_path=str(tmp_path / "py"), doprint=False)
+    assert p.have_bin
+    assert p.run(config) == 0, p.stderr
+    g_psi = read_txt(tmp_path / "py" / "wavefunction_plus.txt")
+    o_psi, _ = oracle_result(psi0)
+    e = rel_l2(g_psi, o_psi)
+    say(f"  rel L2 psi {e:.3e}")
+    assert e <= TOL["fp32"]
