@@ -355,9 +355,12 @@ struct phx_adjoint {
     int64_t nseg = 0;
     size_t N = 0;      /* cells per grid */
     size_t sbytes = 0; /* bytes of one batched state: B grids of psi, then B grids of n */
-    void* ckpt = nullptr; /* nseg states: x at steps 0, K, 2K, ... */
-    void* seg = nullptr;  /* K-1 states: x at steps jK+1 .. jK+K-1 of the segment in use */
-    int64_t seg_loaded = -1;
+    void* ckpt = nullptr;              /* nseg states: x at steps 0, K, 2K, ... */
+    void* seg[2] = {nullptr, nullptr}; /* K-1 states each: x at steps jK+1 .. jK+K-1 of segment j in seg[j % 2] */
+    int64_t seg_content[2] = {-1, -1}; /* segment held by each buffer (valid in stream order) */
+    std::vector<hipEvent_t> ev_rec, ev_adj; /* per segment: recompute done (aux), adjoint done (main) */
+    hipEvent_t ev_start = nullptr;
+    hipStream_t aux = nullptr; /* recomputes the next segment while the main stream runs the adjoint */
     void* lam[2] = {nullptr, nullptr};
     void* gp = nullptr; /* B grids each */
     void* gv = nullptr;
@@ -371,7 +374,7 @@ struct phx_adjoint {
     char* state_psi(void* st) const { return (char*)st; }
     char* state_n(void* st) const { return (char*)st + B * N * 2 * rsize; }
     void* ckpt_at(int64_t j) const { return (char*)ckpt + j * sbytes; }
-    void* seg_at(int64_t i) const { return (char*)seg + (i - 1) * sbytes; } /* i = 1 .. K-1 */
+    void* seg_at(int64_t j, int64_t i) const { return (char*)seg[j & 1] + (i - 1) * sbytes; } /* segment j, i = 1 .. K-1 */
 };
 
 namespace {
@@ -399,7 +402,7 @@ PhxPhys<real> adj_phys(const phx_adjoint* a) {
 
 /* one forward RK4 step of all items: state (psi_in, n_in) -> (psi_out, n_out) */
 template <typename real>
-int fwd_step(phx_adjoint* a, const void* psi_in, const void* n_in, void* psi_out, void* n_out, int ty) {
+int fwd_step(phx_adjoint* a, const void* psi_in, const void* n_in, void* psi_out, void* n_out, int ty, hipStream_t st = nullptr) {
     using v2 = typename phx::Vec2<real>::type;
     phx::Rk4BatchArgs<real> b{};
     b.entries = nullptr;
@@ -425,12 +428,26 @@ int fwd_step(phx_adjoint* a, const void* psi_in, const void* n_in, void* psi_out
     }
     c.ph = adj_phys<real>(a);
     b.iso = c.ph.kin_yx == real(1);
-    ADJ_HIP(phx::launch_rk4_batched<real>(b, ty, a->res, a->pump, a->pot, a->stream));
+    ADJ_HIP(phx::launch_rk4_batched<real>(b, ty, a->res, a->pump, a->pot, st ? st : a->stream));
     return PHX_OK;
 }
 
-/* state location of x_m during the forward pass (m < steps) */
-void* fwd_state(phx_adjoint* a, int64_t m) { return m % a->K == 0 ? a->ckpt_at(m / a->K) : a->seg_at(m % a->K); }
+/* state location of x_m (m < steps): a checkpoint, or a slot of its segment's buffer */
+void* fwd_state(phx_adjoint* a, int64_t m) { return m % a->K == 0 ? a->ckpt_at(m / a->K) : a->seg_at(m / a->K, m % a->K); }
+
+/* states jK+1 .. jK+len-1 of segment j from its checkpoint, on the aux stream */
+template <typename real>
+int recompute(phx_adjoint* a, int64_t j) {
+    const int len = (int)std::min<int64_t>(a->K, a->steps - j * a->K);
+    for (int i = 1; i < len; i++) {
+        void* in = i == 1 ? a->ckpt_at(j) : a->seg_at(j, i - 1);
+        void* out = a->seg_at(j, i);
+        if (int r = fwd_step<real>(a, a->state_psi(in), a->state_n(in), a->state_psi(out), a->state_n(out), a->ty_fwd, a->aux)) return r;
+    }
+    ADJ_HIP(hipEventRecord(a->ev_rec[j], a->aux));
+    a->seg_content[j & 1] = j;
+    return PHX_OK;
+}
 
 template <typename real>
 int adj_step(phx_adjoint* a, void* state, int lin, int ty) {
@@ -529,7 +546,9 @@ int forward_t(phx_adjoint* a, const void* psi0, const void* n0, const void* P, c
         ADJ_HIP(hipMemcpyAsync(psiT, psi0, cb, hipMemcpyDeviceToDevice, a->stream));
         if (a->res) ADJ_HIP(hipMemcpyAsync(nT, n0, rb, hipMemcpyDeviceToDevice, a->stream));
     }
-    a->seg_loaded = a->nseg - 1;
+    /* the last two segments' states are still in the two buffers */
+    a->seg_content[(a->nseg - 1) & 1] = a->nseg - 1;
+    a->seg_content[a->nseg & 1] = a->nseg >= 2 ? a->nseg - 2 : -1;
     a->have_forward = true;
     return PHX_OK;
 }
@@ -548,23 +567,32 @@ int backward_t(phx_adjoint* a, const void* gpsiT, const void* gnT, void* gP, voi
         if (int r = autotune<real>(a)) return r;
     ADJ_HIP(hipMemsetAsync(a->gp, 0, rb, a->stream));
     ADJ_HIP(hipMemsetAsync(a->gv, 0, rb, a->stream));
+    /* Segment j's states live in seg[j % 2].  While the main stream runs the adjoint
+       of segment j, the aux stream recomputes segment j-1 into the other buffer, after
+       the adjoint of segment j+1 (the buffer's previous reader) has finished. */
+    ADJ_HIP(hipEventRecord(a->ev_start, a->stream));
+    ADJ_HIP(hipStreamWaitEvent(a->aux, a->ev_start, 0));
+    std::vector<char> pending(a->nseg, 0); /* recomputed on aux in this call: main must wait */
+    const int64_t last = a->nseg - 1;
+    if (a->K >= 2 && a->seg_content[last & 1] != last) {
+        if (int r = recompute<real>(a, last)) return r;
+        pending[last] = 1;
+    }
     int lin = 0;
-    for (int64_t j = a->nseg - 1; j >= 0; j--) {
-        const int64_t k0 = j * a->K;
-        const int len = (int)std::min<int64_t>(a->K, a->steps - k0);
-        if (a->seg_loaded != j) {
-            for (int i = 1; i < len; i++) {
-                void* in = i == 1 ? a->ckpt_at(j) : a->seg_at(i - 1);
-                void* out = a->seg_at(i);
-                if (int r = fwd_step<real>(a, a->state_psi(in), a->state_n(in), a->state_psi(out), a->state_n(out), a->ty_fwd)) return r;
-            }
-            a->seg_loaded = j;
+    for (int64_t j = last; j >= 0; j--) {
+        if (a->K >= 2 && j >= 1 && a->seg_content[(j - 1) & 1] != j - 1) {
+            if (j + 1 <= last) ADJ_HIP(hipStreamWaitEvent(a->aux, a->ev_adj[j + 1], 0));
+            if (int r = recompute<real>(a, j - 1)) return r;
+            pending[j - 1] = 1;
         }
+        if (pending[j]) ADJ_HIP(hipStreamWaitEvent(a->stream, a->ev_rec[j], 0));
+        const int len = (int)std::min<int64_t>(a->K, a->steps - j * a->K);
         for (int i = len - 1; i >= 0; i--) {
-            void* x = i == 0 ? a->ckpt_at(j) : a->seg_at(i);
+            void* x = i == 0 ? a->ckpt_at(j) : a->seg_at(j, i);
             if (int r = adj_step<real>(a, x, lin, a->ty_adj)) return r;
             lin ^= 1;
         }
+        ADJ_HIP(hipEventRecord(a->ev_adj[j], a->stream));
     }
     if (gpsi0) ADJ_HIP(hipMemcpyAsync(gpsi0, a->state_psi(a->lam[lin]), cb, hipMemcpyDeviceToDevice, a->stream));
     if (gn0 && a->res) ADJ_HIP(hipMemcpyAsync(gn0, a->state_n(a->lam[lin]), rb, hipMemcpyDeviceToDevice, a->stream));
@@ -580,8 +608,13 @@ int backward_t(phx_adjoint* a, const void* gpsiT, const void* gnT, void* gP, voi
 }
 
 void release(phx_adjoint* a) {
-    for (void* p : {a->ckpt, a->seg, a->lam[0], a->lam[1], a->gp, a->gv})
+    for (void* p : {a->ckpt, a->seg[0], a->seg[1], a->lam[0], a->lam[1], a->gp, a->gv})
         if (p) (void)hipFree(p);
+    for (auto* v : {&a->ev_rec, &a->ev_adj})
+        for (hipEvent_t e : *v)
+            if (e) (void)hipEventDestroy(e);
+    if (a->ev_start) (void)hipEventDestroy(a->ev_start);
+    if (a->aux) (void)hipStreamDestroy(a->aux);
     if (a->own) (void)hipStreamDestroy(a->own);
     delete a;
 }
@@ -617,12 +650,19 @@ int phx_adjoint_create(phx_adjoint** out, int precision, const phx_physics* ph, 
     a->sbytes = (size_t)batch * a->N * 3 * a->rsize;
     const size_t rb = (size_t)batch * a->N * a->rsize;
     bool ok = hipMalloc(&a->ckpt, a->nseg * a->sbytes) == hipSuccess;
-    ok = ok && (a->K < 2 || hipMalloc(&a->seg, (a->K - 1) * a->sbytes) == hipSuccess);
+    ok = ok && (a->K < 2 || (hipMalloc(&a->seg[0], (a->K - 1) * a->sbytes) == hipSuccess &&
+                             hipMalloc(&a->seg[1], (a->K - 1) * a->sbytes) == hipSuccess));
     ok = ok && hipMalloc(&a->lam[0], a->sbytes) == hipSuccess && hipMalloc(&a->lam[1], a->sbytes) == hipSuccess;
     ok = ok && hipMalloc(&a->gp, rb) == hipSuccess && hipMalloc(&a->gv, rb) == hipSuccess;
-    ok = ok && hipStreamCreate(&a->own) == hipSuccess;
+    ok = ok && hipStreamCreate(&a->own) == hipSuccess && hipStreamCreate(&a->aux) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&a->ev_start, hipEventDisableTiming) == hipSuccess;
+    a->ev_rec.assign(a->nseg, nullptr);
+    a->ev_adj.assign(a->nseg, nullptr);
+    for (int64_t j = 0; ok && j < a->nseg; j++)
+        ok = hipEventCreateWithFlags(&a->ev_rec[j], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&a->ev_adj[j], hipEventDisableTiming) == hipSuccess;
     if (!ok) {
-        const size_t total = (a->nseg + a->K + 1) * a->sbytes + 2 * rb;
+        const size_t total = (a->nseg + 2 * a->K) * a->sbytes + 2 * rb;
         release(a);
         return afail(PHX_ERR_HIP, "device allocation of about " + std::to_string(total >> 20) + " MiB failed");
     }
@@ -663,6 +703,7 @@ int phx_adjoint_sync(phx_adjoint* a) {
 void phx_adjoint_destroy(phx_adjoint* a) {
     if (!a) return;
     (void)hipStreamSynchronize(a->stream);
+    (void)hipStreamSynchronize(a->aux);
     release(a);
 }
 
