@@ -79,6 +79,12 @@ struct phx_solver {
     int64_t iteration = 0, rejected = 0;
     phx::Rk4Tune tune; /* launch shape of the fused RK4 kernel */
     bool tuned = false;
+    /* captured run of kGraphSteps fused RK4 launches (replayed while nothing changes between steps) */
+    hipGraphExec_t graph = nullptr;
+    int graph_parity = -1;
+    double graph_dt = 0;
+    hipStream_t graph_stream = nullptr;
+    bool graph_off = false;
     std::vector<void*> allocs;
 
     size_t cells() const { return (size_t)cfg.N_c * cfg.N_r; }
@@ -438,6 +444,63 @@ int iterate(phx_solver* s, bool force_fixed) {
     return s->precision == PHX_FP32 ? iterate_t<float>(s, force_fixed) : iterate_t<double>(s, force_fixed);
 }
 
+/* Fixed RK4 steps of the scalar model whose kernel arguments do not change from step to
+   step (constant temporal amplitudes, no noise) are replayed from a hipGraph of kGraphSteps
+   launches.  The host advances t and the FFT guard exactly as iterate() would; a block is
+   only replayed if no step in it reaches t_max or triggers the k-space filter.  Returns
+   the number of steps taken (0: not eligible, the caller iterates). */
+constexpr int kGraphSteps = 50; /* even: the buffer parity returns */
+
+template <typename real>
+int graph_block_t(phx_solver* s, int64_t* taken) {
+    auto& c = s->cfg;
+    *taken = 0;
+    const auto constant = [](const phx::Envelope& e) { return e.groupSize() != 1 || e.allConstant(); };
+    if (s->graph_off || !s->tuned || c.iterator != "RK4" || c.tetm || c.use_stochastic || !constant(c.pump) ||
+        !constant(c.potential))
+        return PHX_OK;
+    if (const char* e = std::getenv("PHX_GRAPH"))
+        if (std::atoi(e) == 0) return PHX_OK; /* PHX_GRAPH=0: launch by launch */
+    double t = s->t, fc = s->fft_cached_t;
+    for (int k = 0; k < kGraphSteps; k++) {
+        if (t >= c.t_max) return PHX_OK;
+        t = s->rnd(t + s->dt);
+        if (!(s->rnd(t - fc) < c.fft_every)) {
+            if (c.use_fft_mask) return PHX_OK;
+            fc = t;
+        }
+    }
+    if (!s->graph || s->graph_parity != s->cur || s->graph_dt != s->dt || s->graph_stream != s->stream) {
+        if (s->graph) PHX_HIP(hipGraphExecDestroy(s->graph));
+        s->graph = nullptr;
+        const int cur = s->cur;
+        if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            (void)hipGetLastError();
+            s->graph_off = true; /* e.g. the legacy default stream: step launch by launch */
+            return PHX_OK;
+        }
+        int rc = PHX_OK;
+        for (int k = 0; k < kGraphSteps && !rc; k++) rc = rk4_launch<real>(s, s->t, s->dt);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(s->stream, &g);
+        s->cur = cur;
+        if (rc) return rc;
+        PHX_HIP(ce);
+        const hipError_t ie = hipGraphInstantiate(&s->graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        PHX_HIP(ie);
+        s->graph_parity = cur;
+        s->graph_dt = s->dt;
+        s->graph_stream = s->stream;
+    }
+    PHX_HIP(hipGraphLaunch(s->graph, s->stream));
+    s->t = t;
+    s->fft_cached_t = fc;
+    s->iteration += kGraphSteps;
+    *taken = kGraphSteps;
+    return PHX_OK;
+}
+
 int field_info(phx_solver* s, int field, void*** slot, size_t* elems) {
     const size_t N = s->cells();
     const int k = (field & 1);
@@ -500,7 +563,19 @@ int phx_initialize(phx_solver* s) {
 int phx_step(phx_solver* s, int64_t n_steps, int64_t* n_done) {
     if (!s || !s->initialized) return fail(PHX_ERR_STATE, "phx_step before phx_initialize");
     int64_t done = 0;
-    for (int64_t i = 0; i < n_steps; i++) {
+    while (done < n_steps) {
+        if (n_steps - done >= kGraphSteps) {
+            int64_t taken = 0;
+            const int r = s->precision == PHX_FP32 ? graph_block_t<float>(s, &taken) : graph_block_t<double>(s, &taken);
+            if (r < 0) {
+                if (n_done) *n_done = done;
+                return r;
+            }
+            if (taken) {
+                done += taken;
+                continue;
+            }
+        }
         int r = iterate(s, false);
         if (r < 0) {
             if (n_done) *n_done = done;
@@ -648,6 +723,7 @@ int phx_sync(phx_solver* s) {
 void phx_destroy(phx_solver* s) {
     if (!s) return;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
+    if (s->graph) (void)hipGraphExecDestroy(s->graph);
     for (void* p : s->allocs) (void)hipFree(p);
     if (s->sums_host) (void)hipHostFree(s->sums_host);
     if (s->own_stream) (void)hipStreamDestroy(s->own_stream);

@@ -155,3 +155,36 @@ def test_rk4_device_entry_matches_handle():
     torch.cuda.synchronize()
     assert np.array_equal(tpsi.cpu().numpy(), h_psi)
     assert np.array_equal(tn.cpu().numpy(), h_n)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "fp64"])
+@pytest.mark.parametrize("with_filter", [False, True])
+def test_graph_replayed_steps_equal_single_steps(prec, with_filter):
+    """phx_step(n >= 50) replays captured blocks of 50 launches; the result equals stepping one
+    at a time bit for bit, also when the k-space filter falls due inside the run (every 50 steps)."""
+    from phoenix_amd._lib import Solver
+    N, Lg, steps, dt = 128, 63.5, 130, 0.04
+    say(f"[rk4 graph blocks] prec={prec} filter={with_filter}")
+    psi, n, P, _ = make_inputs(N, N, Lg, seed=21)
+    args = solver_args(N, N, Lg, 10 * steps, dt, True)
+    if with_filter:
+        args = [a for a in args]
+        i = args.index("--fftEvery")
+        args[i + 1] = str(49.5 * dt)
+        args += ["--fftMask", "1.0", "add", "0.565685", "0.565685", "0", "0", "both", "6", "none", "gauss+noDivide+local"]
+    out = []
+    for chunked in (False, True):
+        s = Solver(args, prec)
+        s.set("psi_plus", psi)
+        s.set("n_plus", n)
+        s.set("pump_plus", P)
+        if chunked:
+            assert s.step(steps) == steps
+        else:
+            for _ in range(steps):
+                assert s.step(1) == 1
+        out.append((s.get("psi_plus"), s.get("n_plus"), s.time()[0], s.info["iteration"]))
+        s.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2] and out[0][3] == out[1][3] == steps
