@@ -143,6 +143,49 @@ def time_fft_filter(prec: str, N: int, L: float, calls: int = 50, every: int = 5
             "bytes_per_call": moved, "hbm_frac": round(moved / (us * 1e-6) / 1e9 / HBM_PEAK_GBS, 4), "finite": finite}
 
 
+def time_spinor(N: int = 2048, steps: int = 40, prec: str = "fp32"):
+    """TE-TM spinor model with adaptive Dormand-Prince 5(4), tolerance 1e-6, additive noise (--dw 1) and an
+    elliptically polarised Gaussian pump (A+- = 1.5 P_th (1 +- sin 2chi) / 2, chi = pi/8), 2048^2, FP32: accepted
+    steps per second.  One accepted step = one or more fused 7-stage launches, each followed by the FP64 error
+    reduction and the host step-size decision."""
+    import math
+    import torch
+    from phoenix_amd._lib import Solver
+    L = 0.5 * (N - 1)
+    ap = 1.5 * PTH * (1 + math.sin(math.pi / 4)) / 2
+    am = 1.5 * PTH * (1 - math.sin(math.pi / 4)) / 2
+    args = ["-tetm", "--N", N, N, "--L", L, L, "--boundary", "periodic", "periodic", "--iterator", "DP45", "-adaptive",
+            "--tol", 1e-6, "--rkvdt", 1e-4, 0.1, "--tstep", 0.04, "--dw", 1, "--tmax", 20, "--outEvery", 1e9,
+            "--fftEvery", 1e9, "--initRandom", 0.01, 7,
+            "--pump", ap, "add", 10, 10, 0, 0, "plus", 1, "none", "gauss+noDivide",
+            "--pump", am, "add", 10, 10, 0, 0, "minus", 1, "none", "gauss+noDivide"]
+    s = Solver([str(a) for a in args], prec)
+    stream = torch.cuda.Stream()
+    s.set_stream(stream.cuda_stream)
+    s.step(5)
+    s.sync()
+    i0 = s.info
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    done = s.step(steps)
+    e1.record(stream)
+    s.sync()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1)
+    i1 = s.info
+    finite = bool(np.isfinite(s.get("psi_plus")).all() and np.isfinite(s.get("psi_minus")).all())
+    s.close()
+    attempts = done + (i1["rejected"] - i0["rejected"])
+    us_att = 1e3 * ms / max(attempts, 1)
+    bytes_att = (56 if prec == "fp32" else 112) * N * N  # psi+-, n+-, P+- read; psi+-, n+- written
+    return {"grid": f"{N}x{N}", "precision": prec, "accepted_steps": done, "attempts": attempts,
+            "us_per_accepted_step": round(1e3 * ms / max(done, 1), 2), "us_per_attempt": round(us_att, 2),
+            "dt_end": i1["dt"], "t_end": i1["t"], "hbm_frac_per_attempt": round(bytes_att / (us_att * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+            "wall_ms": round(1e3 * wall, 2), "finite": finite}
+
+
 def ensemble_inputs(i: int, N: int, amp: float, seed: int):
     """Instance i of the ensemble workload: dx = 0.5 um, Gaussian pump of width 10 um at amp * P_th,
     psi0 = 0.01 complex Gaussian noise from the instance's own seed, n0 = 0."""
@@ -322,6 +365,7 @@ def main():
     ap.add_argument("--only-legs", action="store_true",
                     help="run only the batched-ensemble and adjoint legs (profiling); prints their objects")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-spinor", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB 10 times (calibrates rocprof byte counters)")
@@ -393,6 +437,8 @@ def main():
         out["batched"] = time_batched(a.batch_instances, a.batch_steps)
     if not a.no_adjoint and a.only is None:
         out["adjoint"] = time_adjoint(steps=a.adjoint_steps)
+    if not a.no_spinor and a.only is None:
+        out["spinor_dp45"] = time_spinor()
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(N, L, a.cpu_steps)
     print(json.dumps(out), flush=True)

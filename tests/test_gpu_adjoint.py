@@ -206,3 +206,61 @@ def test_adjoint_interleaved_forward_passes():
     assert np.array_equal(g1, ref)
     assert np.array_equal(g1b, ref)
     del x2, m2
+
+
+def test_adjoint_per_item_parameters():
+    """P and V with a batch dimension: each item gets its own gradient (FP64 vs the oracle, 1e-10)."""
+    import torch
+    from phoenix_amd import _lib
+    from phoenix_amd.adjoint import rk4_evolve
+    N_c, N_r, L, B, steps, dt = 40, 48, 23.5, 2, 25, 0.02
+    say("[adjoint per-item P, V]")
+    P, V, psi0, n0, mask = problem(N_c, N_r, L, B, seed=9)
+    Pb = np.stack([P, 0.7 * P[::-1, :]])
+    Vb = np.stack([V, -V[:, ::-1]])
+    ph, _ = _lib.physics_defaults(N_c, N_r, L, L)
+    dev = torch.device("cuda:0")
+    tP = torch.tensor(Pb, dtype=torch.float64, device=dev, requires_grad=True)
+    tV = torch.tensor(Vb, dtype=torch.float64, device=dev, requires_grad=True)
+    tpsi = torch.tensor(psi0, dtype=torch.complex128, device=dev)
+    tn = torch.tensor(n0, dtype=torch.float64, device=dev)
+    tm = torch.tensor(mask, dtype=torch.float64, device=dev)
+    psiT, nT = rk4_evolve(tP, tV, tpsi, tn, physics=ph, dt=dt, steps=steps, checkpoint_every=8)
+    ((psiT.abs() ** 2 * tm).sum() + 0.3 * (nT * tm).sum()).backward()
+    p = O.params(N_c, N_r, L, L, periodic=(True, True), reservoir=True, pump=True, potential=True)
+    for b in range(B):
+        oT, onT = O.rk4_run(p, psi0[b], n0[b], pump=Pb[b], pot=Vb[b], dt=dt, steps=steps)
+        _, _, dP, dV, *_ = O.rk4_adjoint(p, psi0[b], n0[b], Pb[b], Vb[b], dt, steps, 2 * mask * oT, 0.3 * mask)
+        e = max(rel_l2(tP.grad[b].cpu().numpy(), dP), rel_l2(tV.grad[b].cpu().numpy(), dV))
+        say(f"  item {b}: rel L2 {e:.2e}")
+        assert e <= 1e-10
+
+
+def test_adjoint_without_reservoir_gradient_of_potential():
+    """Reservoir off (psi-only model with a potential) through the C ABI: dJ/dV vs the oracle."""
+    import torch
+    from phoenix_amd import _lib
+    N, L, steps, dt = 48, 23.5, 30, 0.02
+    say("[adjoint no reservoir]")
+    _, V, psi0, _, mask = problem(N, N, L, 1, seed=4)
+    ph, _ = _lib.physics_defaults(N, N, L, L)
+    dev = torch.device("cuda:0")
+    tV = torch.tensor(V, dtype=torch.float64, device=dev)
+    tpsi = torch.tensor(psi0, dtype=torch.complex128, device=dev)
+    psiT = torch.empty_like(tpsi)
+    eng = _lib.Adjoint("fp64", ph, N, N, 1, dt, steps, 7, (True, True), reservoir=False, pump=False, potential=True)
+    s = torch.cuda.current_stream().cuda_stream
+    eng.set_stream(s)
+    eng.forward(tpsi.data_ptr(), None, None, tV.data_ptr(), psiT.data_ptr(), None)
+    g = torch.tensor(2 * mask, dtype=torch.float64, device=dev) * psiT
+    gV = torch.empty_like(tV)
+    eng.backward(g.contiguous().data_ptr(), None, None, gV.data_ptr(), None, None)
+    torch.cuda.synchronize()
+    p = O.params(N, N, L, L, periodic=(True, True), reservoir=False, pump=False, potential=True)
+    oT, _ = O.rk4_run(p, psi0[0], None, pot=V, dt=dt, steps=steps)
+    assert rel_l2(psiT[0].cpu().numpy(), oT) <= 1e-12
+    _, _, _, dV, *_ = O.rk4_adjoint(p, psi0[0], None, None, V, dt, steps, 2 * mask * oT)
+    e = rel_l2(gV.cpu().numpy(), dV)
+    say(f"  rel L2 dV {e:.2e}")
+    assert e <= 1e-10
+    eng.close()

@@ -140,3 +140,43 @@ def test_ensemble_fp64_matches_oracle():
         say(f"  instance {i} ({N_c}x{N_r}): rel L2 psi {ep:.3e}  n {en:.3e}")
         assert ep <= 1e-12 and en <= 1e-12
     e.close()
+
+
+@pytest.mark.parametrize("flags", [dict(reservoir=False, pump=False, potential=True),
+                                   dict(reservoir=True, pump=True, potential=True)])
+def test_ensemble_feature_sets_match_oracle(flags):
+    """Ensembles without a reservoir (psi only, potential) and with every field, FP64 vs the oracle."""
+    from phoenix_amd import _lib as L
+    steps, dt = 30, 0.02
+    say(f"[ensemble features] {flags}")
+    ph, _ = L.physics_defaults(64, 64, 31.5, 31.5)
+    sizes = [(48, 40, 23.5), (72, 64, 35.5)]
+    e = L.Ensemble("fp64", ph, [(a, b) for a, b, _ in sizes], [(Lg / (a - 1), Lg / (b - 1)) for a, b, Lg in sizes], **flags)
+    data = []
+    for i, (N_c, N_r, Lg) in enumerate(sizes):
+        psi, n, P, dx, dy = inputs(i, N_c, N_r, Lg)
+        x = -Lg / 2 + dx * np.arange(N_c)
+        y = -Lg / 2 + dy * np.arange(N_r)
+        X, Y = np.meshgrid(x, y)
+        V = 3e-4 * np.exp(-((X - 2) ** 2 + (Y + 1) ** 2) / 20.0)
+        e.set(i, "psi_plus", psi)
+        if flags["reservoir"]:
+            e.set(i, "n_plus", n)
+            e.set(i, "pump_plus", P)
+        e.set(i, "potential_plus", V)
+        data.append((psi, n, P, V))
+    e.step(dt, steps)
+    e.sync()
+    for i, (N_c, N_r, Lg) in enumerate(sizes):
+        psi, n, P, V = data[i]
+        p = O.params(N_c, N_r, Lg, Lg, periodic=(True, True), reservoir=flags["reservoir"], pump=flags["pump"],
+                     potential=True)
+        if flags["reservoir"]:
+            o_psi, o_n = O.rk4_run(p, psi, n, pump=P, pot=V, dt=dt, steps=steps)
+            assert rel_l2(e.get(i, "n_plus"), o_n) <= 1e-12
+        else:
+            o_psi, _ = O.rk4_run(p, psi, None, pot=V, dt=dt, steps=steps)
+        err = rel_l2(e.get(i, "psi_plus"), o_psi)
+        say(f"  instance {i}: rel L2 psi {err:.2e}")
+        assert err <= 1e-12
+    e.close()
