@@ -36,6 +36,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 #include <string>
 #include <vector>
@@ -565,6 +566,7 @@ int backward_t(phx_adjoint* a, const void* gpsiT, const void* gnT, void* gP, voi
     }
     if (!a->tuned)
         if (int r = autotune<real>(a)) return r;
+    if (const char* e = std::getenv("PHX_ADJ_TY")) a->ty_adj = std::max(1, std::atoi(e)); /* tuning experiments */
     ADJ_HIP(hipMemsetAsync(a->gp, 0, rb, a->stream));
     ADJ_HIP(hipMemsetAsync(a->gv, 0, rb, a->stream));
     /* Segment j's states live in seg[j % 2].  While the main stream runs the adjoint
