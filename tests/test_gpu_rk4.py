@@ -1,7 +1,7 @@
 """GPU parity of the fused RK4 step (phx_rk4.hip) against the FP64 oracle.
 
 Bar (stated per test): relative L2 of psi and n after the given number of steps
-  FP64: <= 1e-12, FP32: <= 1e-5  (tolerances of the north-star contract).
+  FP64: <= 1e-12, FP32: <= 1e-5  (the parity bar of this path).
 At the full 1024^2 size the oracle would take minutes, so size-independent
 properties are checked there instead (translation equivariance, FP32 vs FP64).
 """
@@ -85,7 +85,7 @@ CASES = [
     (96, 80, 40.0, 120, True, True, True, 5.0),     # ragged: 96 = 56 + 40 columns, 80 rows
     (112, 48, 50.0, 100, False, False, True, 0.0),  # no reservoir (psi-only with potential)
     (8, 8, 3.5, 50, True, True, False, 2.0),        # smaller than one tile and than the halo reach
-    (256, 256, 127.5, 200, True, True, False, 0.0),  # workload 1 geometry (dx = 0.5)
+    (256, 256, 127.5, 200, True, True, False, 0.0),  # 256^2 CPU-path geometry (dx = 0.5)
 ]
 
 
@@ -123,7 +123,7 @@ def test_rk4_translation_equivariance_1024(prec):
 
 
 def test_rk4_fp32_vs_fp64_1024():
-    """Full workload-2 size, 200 steps: FP32 stays within 1e-5 (rel L2) of FP64."""
+    """Full 1024^2 benchmark size, 200 steps: FP32 stays within 1e-5 (rel L2) of FP64."""
     N, L, steps = 1024, 511.5, 200
     say(f"[rk4 fp32 vs fp64] N={N} steps={steps}")
     psi, n, P, _ = make_inputs(N, N, L, seed=4242)
