@@ -66,11 +66,11 @@ typedef struct {
     int32_t use_reservoir, use_pump, use_potential, use_stochastic, use_fft_mask;
     int32_t adaptive;           /* -adaptive with an adaptive iterator */
     int32_t periodic_x, periodic_y;
-    int32_t halo;               /* k_max of the iterator: 4 (RK4), 7 (DP45) */
-    char iterator[16];          /* "RK4" or "DP45" */
+    int32_t halo;               /* stages of the step (= halo in cells): 4 (RK4), 6 (DP45), 7 (DP45 -adaptive) */
+    char iterator[16];          /* registered --iterator key: "RK4", "DP45", "Heun2", "BoSha", ... */
     double dx, dy, dt, t, t_max, fft_every, tolerance, dt_min, dt_max;
     int64_t iteration;          /* accepted steps so far */
-    int64_t rejected;           /* rejected DP45 attempts so far */
+    int64_t rejected;           /* rejected adaptive attempts so far */
     int32_t tile_rows;          /* output rows per wavefront tile of the fused RK4 kernel (0 before the first step) */
     int32_t prefetch_rows;      /* rows it loads ahead of use */
 } phx_info;

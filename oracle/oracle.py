@@ -70,6 +70,14 @@ def lib():
         L.orc_philox4x32_10.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         L.orc_gaussian_field.argtypes = [dp, C.c_int, C.c_double, C.c_uint64]
         L.orc_rk4_adjoint.argtypes = [P, dp, dp, dp, dp, C.c_double, C.c_int, dp, dp, dp, dp, dp, dp]
+        cs = C.c_char_p
+        L.orc_iter_info.argtypes = [cs, C.POINTER(C.c_int), dp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.orc_iter_run.argtypes = [P, cs, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_int, C.c_int64]
+        L.orc_iter_attempt.argtypes = [P, cs, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int64, dp, dp]
+        L.orc_adapt_dt_pow.argtypes = [C.c_double] * 7 + [dp, dp]
+        L.orc_iter_adaptive_run.argtypes = [P, cs, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double,
+                                            C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int64)]
+        L.orc_iter_adaptive_run.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -191,6 +199,84 @@ def dp45_run(p, psi, n, pump=None, pot=None, t=0.0, dt=0.04, t_max=1.0, tol=1e-6
                              max_rejections, C.byref(nrej))
     if acc < 0:
         raise RuntimeError(f"orc_dp45_run failed: {acc}")
+    res = dict(psi=psi, n=n, t=tt.value, dt=dd.value, accepted=int(acc), rejected=int(nrej.value))
+    if pm is not None:
+        res.update(psi_minus=pm, n_minus=nm)
+    return res
+
+
+ITERATORS = ("Newton", "MP", "Heun2", "Ralston", "Heun3", "Ralston3", "RK3", "SSPRK3", "HW", "RK4", "Ralston4", "Rule38",
+             "Nystroem", "Fehlberg2", "BoSha", "CashKarp", "Fehlberg5", "DP45")
+
+
+def iter_info(key):
+    """(halo, adaptTimeStep power or 0, stages fixed, stages adaptive or 0) of an iterator key."""
+    h, st_f, st_a = C.c_int(), C.c_int(), C.c_int()
+    pw = C.c_double()
+    if lib().orc_iter_info(key.encode(), C.byref(h), C.byref(pw), C.byref(st_f), C.byref(st_a)):
+        raise KeyError(key)
+    return h.value, pw.value, st_f.value, st_a.value
+
+
+def iter_run(p, key, psi, n=None, pump=None, pot=None, dt=0.04, steps=1, psi_minus=None, n_minus=None, pump_minus=None,
+             pot_minus=None, step0=0):
+    """`steps` fixed steps of the iterator `key` (orc_iter_run); returns as rk4_run."""
+    psi = _c128(psi)
+    pm = _c128(psi_minus) if psi_minus is not None else None
+    n = _f64(n) if n is not None else np.zeros(psi.shape)
+    nm = _f64(n_minus) if n_minus is not None else (np.zeros(psi.shape) if psi_minus is not None else None)
+    rc = lib().orc_iter_run(C.byref(p), key.encode(), _dp(psi.view(np.float64)),
+                            _dp(pm.view(np.float64)) if pm is not None else None, _dp(n), _dp(nm), _dp(_f64(pump)),
+                            _dp(_f64(pump_minus)), _dp(_f64(pot)), _dp(_f64(pot_minus)), dt, steps, step0)
+    if rc:
+        raise RuntimeError(f"orc_iter_run({key}) failed: {rc}")
+    if psi_minus is not None:
+        return psi, n, pm, nm
+    return psi, n
+
+
+def iter_attempt(p, key, psi, n=None, pump=None, pot=None, dt=0.04, noise_dt=None, step=0, psi_minus=None, n_minus=None,
+                 pump_minus=None, pot_minus=None):
+    """One adaptive attempt of `key` (orc_iter_attempt); returns as dp45_attempt."""
+    psi = _c128(psi)
+    n = _f64(n) if n is not None else np.zeros(psi.shape)
+    pm = _c128(psi_minus) if psi_minus is not None else None
+    nm = _f64(n_minus) if n_minus is not None else (np.zeros(psi.shape) if psi_minus is not None else None)
+    es, nr = C.c_double(), C.c_double()
+    rc = lib().orc_iter_attempt(C.byref(p), key.encode(), _dp(psi.view(np.float64)),
+                                _dp(pm.view(np.float64)) if pm is not None else None, _dp(n), _dp(nm), _dp(_f64(pump)),
+                                _dp(_f64(pump_minus)), _dp(_f64(pot)), _dp(_f64(pot_minus)), dt,
+                                dt if noise_dt is None else noise_dt, step, C.byref(es), C.byref(nr))
+    if rc:
+        raise RuntimeError(f"orc_iter_attempt({key}) failed: {rc}")
+    out = (psi, n, es.value, nr.value)
+    if psi_minus is not None:
+        out = out + (pm, nm)
+    return out
+
+
+def adapt_dt_pow(err_sum, norm, tol, dt, dt_min, dt_max, power):
+    nd, fe = C.c_double(), C.c_double()
+    acc = lib().orc_adapt_dt_pow(err_sum, norm, tol, dt, dt_min, dt_max, power, C.byref(nd), C.byref(fe))
+    return bool(acc), nd.value, fe.value
+
+
+def iter_adaptive_run(p, key, psi, n, pump=None, pot=None, t=0.0, dt=0.04, t_max=1.0, tol=1e-6, dt_min=1e-4, dt_max=0.1,
+                      max_rejections=1000, psi_minus=None, n_minus=None, pump_minus=None, pot_minus=None):
+    """Adaptive run of `key` until t >= t_max (orc_iter_adaptive_run); returns as dp45_run."""
+    psi = _c128(psi)
+    n = _f64(n)
+    pm = _c128(psi_minus) if psi_minus is not None else None
+    nm = _f64(n_minus)
+    tt, dd = C.c_double(t), C.c_double(dt)
+    nrej = C.c_int64()
+    acc = lib().orc_iter_adaptive_run(C.byref(p), key.encode(), _dp(psi.view(np.float64)),
+                                      _dp(pm.view(np.float64)) if pm is not None else None, _dp(n), _dp(nm),
+                                      _dp(_f64(pump)), _dp(_f64(pump_minus)), _dp(_f64(pot)), _dp(_f64(pot_minus)),
+                                      C.byref(tt), C.byref(dd), t_max, tol, dt_min, dt_max, max_rejections,
+                                      C.byref(nrej))
+    if acc < 0:
+        raise RuntimeError(f"orc_iter_adaptive_run({key}) failed: {acc}")
     res = dict(psi=psi, n=n, t=tt.value, dt=dd.value, accepted=int(acc), rejected=int(nrej.value))
     if pm is not None:
         res.update(psi_minus=pm, n_minus=nm)

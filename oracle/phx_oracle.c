@@ -398,11 +398,12 @@ static void sum_to_buffer(ws_t* w, int index, int nw, const double* wts) {
             }
 }
 
-/* FINAL_SUM_K (cuda_macro.cuh:200-229): X += dt*sum b_j k_j on the whole padded region. */
-static void final_sum(ws_t* w, int nw, const double* wts) {
+/* FINAL_SUM_K(index, weights) (cuda_macro.cuh:200-229): X += dt*sum b_j k_j on the whole
+   padded region; single weight -> slot index-1, else slots 0..n-1. */
+static void final_sum(ws_t* w, int index, int nw, const double* wts) {
     const int ch = w->h;
     int slot[8];
-    for (int j = 0; j < nw; j++) slot[j] = (nw == 1) ? nw - 1 : j;
+    for (int j = 0; j < nw; j++) slot[j] = (nw == 1) ? index - 1 : j;
     double wsum = 0.0;
     for (int j = 0; j < nw; j++) wsum += wts[j];
     const double dt = w->dt;
@@ -493,7 +494,7 @@ static void rk4_step(ws_t* w) {
     calc_k(w, 3, 0.5, 1);
     sum_to_buffer(w, 3, 1, &one);
     calc_k(w, 4, 1.0, 1);
-    final_sum(w, 4, b);
+    final_sum(w, 4, 4, b);
 }
 
 int orc_rk4_run(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
@@ -631,6 +632,328 @@ int64_t orc_dp45_run(const orc_params* p, double* psi_plus, double* psi_minus, d
     for (int c = 0; c < nc; c++) { free(tp[c]); free(tn[c]); }
     *n_rejected = rejected;
     return accepted;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Generic iterators: the step() sequences of solver/iterator/<name>.cu as data   */
+/* ------------------------------------------------------------------------ */
+/* One macro call of a step(): 'K' CALCULATE_K(index, w, input = buffer?),
+   'I' INTERMEDIATE_SUM_K(index, weights), 'F' FINAL_SUM_K(index, weights),
+   'E' ERROR_K(weights) (the psi+ error and the buffer norm on interior cells). */
+typedef struct {
+    char op;
+    int index;
+    double w;
+    int nw;
+    double wts[8];
+} orc_op;
+
+typedef struct {
+    const char* key;
+    int halo;        /* halo_size_ of the reference iterator */
+    double power;    /* adaptTimeStep exponent; 0: no variable time step */
+    int nfixed, nadapt;
+    orc_op fixed[16];
+    orc_op adapt[16];
+} orc_iter;
+
+#define K1_ {'K', 1, 0.0, 0, {0}}
+#define KB(i, w) {'K', i, w, 0, {0}}
+
+static const orc_iter ITERS[] = {
+    /* Newton::step (newton.cu:24-31) */
+    {"Newton", 1, 0, 2, 0, {K1_, {'F', 1, 0, 1, {1.0}}}, {{0}}},
+    /* MidPoint::step (explicit_mp.cu:28-39); halo_size_ = 3 (explicit_mp.cu:15) */
+    {"MP", 3, 0, 4, 0, {K1_, {'I', 1, 0, 1, {0.5}}, KB(2, 0.5), {'F', 2, 0, 2, {0.0, 1.0}}}, {{0}}},
+    /* Heun2::step (heun.cu:31-42) */
+    {"Heun2", 2, 0, 4, 0, {K1_, {'I', 1, 0, 1, {1.0}}, KB(2, 1.0), {'F', 3, 0, 2, {0.5, 0.5}}}, {{0}}},
+    /* Ralston::step (ralston.cu:31-42) */
+    {"Ralston", 2, 0, 4, 0, {K1_, {'I', 1, 0, 1, {2.0 / 3.0}}, KB(2, 2.0 / 3.0), {'F', 2, 0, 2, {1.0 / 4.0, 3.0 / 4.0}}}, {{0}}},
+    /* Heun3::step (heun.cu:66-82) */
+    {"Heun3", 3, 0, 6, 0,
+     {K1_, {'I', 1, 0, 1, {1.0 / 3.0}}, KB(2, 1.0 / 3.0), {'I', 2, 0, 2, {0.0, 2.0 / 3.0}}, KB(3, 2.0 / 3.0),
+      {'F', 3, 0, 3, {1.0 / 4.0, 0.0, 3.0 / 4.0}}},
+     {{0}}},
+    /* Ralston3::step (ralston.cu:66-81) */
+    {"Ralston3", 3, 0, 6, 0,
+     {K1_, {'I', 1, 0, 1, {1.0 / 2.0}}, KB(2, 1.0 / 2.0), {'I', 2, 0, 2, {0.0, 3.0 / 4.0}}, KB(3, 3.0 / 4.0),
+      {'F', 3, 0, 3, {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0}}},
+     {{0}}},
+    /* RungeKutta3::step (runge_kutta.cu:33-48); its final sum names a fourth weight for a K
+       slot the solver never writes (k_max_ = 3): the three-weight sum of its tableau string
+       (runge_kutta.cu:23-30) is used */
+    {"RK3", 3, 0, 6, 0,
+     {K1_, {'I', 1, 0, 1, {0.5}}, KB(2, 0.5), {'I', 2, 0, 2, {-1.0, 2.0}}, KB(3, 1.0),
+      {'F', 3, 0, 3, {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0}}},
+     {{0}}},
+    /* SSPRK3::step (SSPRK3.cu:27-42), third stage at w = 1 as passed */
+    {"SSPRK3", 3, 0, 6, 0,
+     {K1_, {'I', 1, 0, 1, {1.0}}, KB(2, 1.0), {'I', 2, 0, 2, {1.0 / 4.0, 1.0 / 4.0}}, KB(3, 1.0),
+      {'F', 3, 0, 3, {1.0 / 6.0, 1.0 / 6.0, 2.0 / 3.0}}},
+     {{0}}},
+    /* HouwenWray::step (houwen_wray.cu:27-42) */
+    {"HW", 3, 0, 6, 0,
+     {K1_, {'I', 1, 0, 1, {8.0 / 15.0}}, KB(2, 8.0 / 15.0), {'I', 2, 0, 2, {1.0 / 4.0, 5.0 / 12.0}}, KB(3, 2.0 / 3.0),
+      {'F', 3, 0, 3, {1.0 / 4.0, 0.0, 3.0 / 4.0}}},
+     {{0}}},
+    /* RungeKutta4::step (runge_kutta.cu:74-94) */
+    {"RK4", 4, 0, 8, 0,
+     {K1_, {'I', 1, 0, 1, {0.5}}, KB(2, 0.5), {'I', 2, 0, 1, {0.5}}, KB(3, 0.5), {'I', 3, 0, 1, {1.0}}, KB(4, 1.0),
+      {'F', 4, 0, 4, {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0}}},
+     {{0}}},
+    /* Ralston4::step (ralston.cu:106-125); the reference runs its four stages with
+       k_max_ = halo_size_ = 3 (ralston.cu:92-93), which leaves the fourth K outside the K
+       buffer: here with the halo of four that the four stages need */
+    {"Ralston4", 4, 0, 8, 0,
+     {K1_, {'I', 1, 0, 1, {0.4}}, KB(2, 0.4), {'I', 2, 0, 2, {0.29697760924, 0.15875964497}}, KB(3, 0.51823725421),
+      {'I', 3, 0, 3, {0.21810038822, -3.05096514869, 3.83286476047}}, KB(4, 0.51823725421),
+      {'F', 4, 0, 4, {0.17476028226, -0.55148066287, 1.2055355994, 0.17118478122}}},
+     {{0}}},
+    /* Rule38::step (rule38.cu:28-48) */
+    {"Rule38", 4, 0, 8, 0,
+     {K1_, {'I', 1, 0, 1, {1.0 / 3.0}}, KB(2, 1.0 / 3.0), {'I', 2, 0, 2, {-1.0 / 3.0, 1.0}}, KB(3, 2.0 / 3.0),
+      {'I', 3, 0, 3, {1.0, -1.0, 1.0}}, KB(4, 1.0), {'F', 4, 0, 4, {1.0 / 8.0, 3.0 / 8.0, 3.0 / 8.0, 1.0 / 8.0}}},
+     {{0}}},
+    /* Nystroem::step (nystroem.cu:30-57) */
+    {"Nystroem", 6, 0, 12, 0,
+     {K1_, {'I', 1, 0, 1, {1.0 / 3.0}}, KB(2, 1.0 / 3.0), {'I', 2, 0, 2, {4.0 / 25.0, 6.0 / 25.0}}, KB(3, 2.0 / 5.0),
+      {'I', 3, 0, 3, {1.0 / 4.0, -3.0, 15.0 / 4.0}}, KB(4, 1.0),
+      {'I', 4, 0, 4, {2.0 / 27.0, 10.0 / 9.0, -50.0 / 81.0, 8.0 / 81.0}}, KB(5, 2.0 / 3.0),
+      {'I', 5, 0, 4, {2.0 / 25.0, 12.0 / 25.0, 2.0 / 15.0, 8.0 / 75.0}}, KB(6, 4.0 / 5.0),
+      {'F', 6, 0, 6, {23.0 / 192.0, 0.0, 125.0 / 192.0, 0.0, -27.0 / 64.0, 125.0 / 192.0}}},
+     {{0}}},
+    /* Fehlberg2::step (fehlberg.cu:33-70) */
+    {"Fehlberg2", 3, 0.25, 6, 7,
+     {K1_, {'I', 1, 0, 1, {1.0 / 2.0}}, KB(2, 1.0 / 2.0), {'I', 2, 0, 2, {1.0 / 256.0, 255.0 / 256.0}}, KB(3, 1.0),
+      {'F', 3, 0, 3, {1.0 / 512.0, 255.0 / 256.0, 1.0 / 512.0}}},
+     {K1_, {'I', 1, 0, 1, {1.0 / 2.0}}, KB(2, 1.0 / 2.0), {'I', 2, 0, 2, {1.0 / 256.0, 255.0 / 256.0}}, KB(3, 1.0),
+      {'I', 3, 0, 3, {1.0 / 512.0, 255.0 / 256.0, 1.0 / 512.0}}, {'E', 3, 0, 3, {1.0 / 512.0 - 1.0 / 256.0, 0.0, 1.0 / 512.0}}}},
+    /* BogackiShampine::step (bogacki.cu:28-75); a32 = 3/9 as written there (bogacki.cu:41) */
+    {"BoSha", 4, 1.0 / 3.0, 6, 9,
+     {K1_, {'I', 1, 0, 1, {1.0 / 2.0}}, KB(2, 1.0 / 2.0), {'I', 2, 0, 2, {0.0, 3.0 / 9.0}}, KB(3, 3.0 / 4.0),
+      {'F', 3, 0, 3, {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0}}},
+     {K1_, {'I', 1, 0, 1, {1.0 / 2.0}}, KB(2, 1.0 / 2.0), {'I', 2, 0, 2, {0.0, 3.0 / 9.0}}, KB(3, 3.0 / 4.0),
+      {'I', 3, 0, 3, {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0}}, KB(4, 1.0),
+      {'E', 4, 0, 4, {2.0 / 9.0 - 7.0 / 24.0, 1.0 / 3.0 - 1.0 / 4.0, 4.0 / 9.0 - 1.0 / 3.0, 0.0 - 1.0 / 8.0}},
+      {'I', 3, 0, 3, {2.0 / 9.0, 1.0 / 3.0, 4.0 / 9.0}}}},
+    /* CashKarp::step (cash_karp.cu:31-83); a41 = 3/1 (cash_karp.cu:47) and the K5 error
+       weight -2187/6784 (cash_karp.cu:70) as written there */
+    {"CashKarp", 6, 0.25, 12, 13,
+     {K1_, {'I', 1, 0, 1, {1.0 / 5.0}}, KB(2, 1.0 / 5.0), {'I', 2, 0, 2, {3.0 / 40.0, 9.0 / 40.0}}, KB(3, 3.0 / 10.0),
+      {'I', 3, 0, 3, {3.0 / 1.0, -9.0 / 10.0, 6.0 / 5.0}}, KB(4, 3.0 / 5.0),
+      {'I', 4, 0, 4, {-11.0 / 54.0, 5.0 / 2.0, -70.0 / 27.0, 35.0 / 27.0}}, KB(5, 1.0),
+      {'I', 5, 0, 5, {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}}, KB(6, 7.0 / 8.0),
+      {'F', 6, 0, 6, {37.0 / 378.0, 0.0, 250.0 / 621.0, 125.0 / 594.0, 0.0, 512.0 / 1771.0}}},
+     {K1_, {'I', 1, 0, 1, {1.0 / 5.0}}, KB(2, 1.0 / 5.0), {'I', 2, 0, 2, {3.0 / 40.0, 9.0 / 40.0}}, KB(3, 3.0 / 10.0),
+      {'I', 3, 0, 3, {3.0 / 1.0, -9.0 / 10.0, 6.0 / 5.0}}, KB(4, 3.0 / 5.0),
+      {'I', 4, 0, 4, {-11.0 / 54.0, 5.0 / 2.0, -70.0 / 27.0, 35.0 / 27.0}}, KB(5, 1.0),
+      {'I', 5, 0, 5, {1631.0 / 55296.0, 175.0 / 512.0, 575.0 / 13824.0, 44275.0 / 110592.0, 253.0 / 4096.0}}, KB(6, 7.0 / 8.0),
+      {'I', 6, 0, 6, {37.0 / 378.0, 0.0, 250.0 / 621.0, 125.0 / 594.0, 0.0, 512.0 / 1771.0}},
+      {'E', 6, 0, 6,
+       {37.0 / 378.0 - 2825.0 / 27648.0, 0.0, 250.0 / 621.0 - 18575.0 / 48384.0, 125.0 / 594.0 - 13525.0 / 55296.0,
+        0.0 - 2187.0 / 6784.0, 512.0 / 1771.0 - 1.0 / 4.0}}}},
+    /* Fehlberg5::step (fehlberg.cu:100-152); the K5 error weight -9/50 - 1/5 as written
+       there (fehlberg.cu:137) */
+    {"Fehlberg5", 6, 0.25, 12, 13,
+     {K1_, {'I', 1, 0, 1, {1.0 / 4.0}}, KB(2, 1.0 / 4.0), {'I', 2, 0, 2, {3.0 / 32.0, 9.0 / 32.0}}, KB(3, 3.0 / 8.0),
+      {'I', 3, 0, 3, {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0}}, KB(4, 12.0 / 13.0),
+      {'I', 4, 0, 4, {439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0}}, KB(5, 1.0),
+      {'I', 5, 0, 5, {-8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0}}, KB(6, 1.0 / 2.0),
+      {'F', 6, 0, 6, {16.0 / 135.0, 0.0, 6656.0 / 12825.0, 28561.0 / 56430.0, -9.0 / 50.0, 2.0 / 55.0}}},
+     {K1_, {'I', 1, 0, 1, {1.0 / 4.0}}, KB(2, 1.0 / 4.0), {'I', 2, 0, 2, {3.0 / 32.0, 9.0 / 32.0}}, KB(3, 3.0 / 8.0),
+      {'I', 3, 0, 3, {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0}}, KB(4, 12.0 / 13.0),
+      {'I', 4, 0, 4, {439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0}}, KB(5, 1.0),
+      {'I', 5, 0, 5, {-8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0}}, KB(6, 1.0 / 2.0),
+      {'I', 6, 0, 6, {16.0 / 135.0, 0.0, 6656.0 / 12825.0, 28561.0 / 56430.0, -9.0 / 50.0, 2.0 / 55.0}},
+      {'E', 6, 0, 6,
+       {16.0 / 135.0 - 25.0 / 216.0, 0.0, 6656.0 / 12825.0 - 1408.0 / 2565.0, 28561.0 / 56430.0 - 2197.0 / 4104.0,
+        -9.0 / 50.0 - 1.0 / 5.0, 2.0 / 55.0}}}},
+    /* DormandPrince45::step (dormand_prince.cu:37-95) */
+    {"DP45", 7, 0.25, 12, 14,
+     {K1_, {'I', 1, 0, 1, {1.0 / 5.0}}, KB(2, 1.0 / 5.0), {'I', 2, 0, 2, {3.0 / 40.0, 9.0 / 40.0}}, KB(3, 3.0 / 10.0),
+      {'I', 3, 0, 3, {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0}}, KB(4, 4.0 / 5.0),
+      {'I', 4, 0, 4, {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0}}, KB(5, 8.0 / 9.0),
+      {'I', 5, 0, 5, {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0}}, KB(6, 1.0),
+      {'F', 6, 0, 6, {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}}},
+     {K1_, {'I', 1, 0, 1, {1.0 / 5.0}}, KB(2, 1.0 / 5.0), {'I', 2, 0, 2, {3.0 / 40.0, 9.0 / 40.0}}, KB(3, 3.0 / 10.0),
+      {'I', 3, 0, 3, {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0}}, KB(4, 4.0 / 5.0),
+      {'I', 4, 0, 4, {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0}}, KB(5, 8.0 / 9.0),
+      {'I', 5, 0, 5, {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0}}, KB(6, 1.0),
+      {'I', 6, 0, 6, {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}}, KB(7, 1.0),
+      {'E', 7, 0, 7,
+       {35.0 / 384.0 - 5179.0 / 57600.0, 0.0, 500.0 / 1113.0 - 7571.0 / 16695.0, 125.0 / 192.0 - 393.0 / 640.0,
+        -2187.0 / 6784.0 + 92097.0 / 339200.0, 11.0 / 84.0 - 187.0 / 2100.0, -1.0 / 40.0}}}},
+};
+#undef K1_
+#undef KB
+
+static const orc_iter* find_iter(const char* key) {
+    for (size_t k = 0; k < sizeof(ITERS) / sizeof(ITERS[0]); k++)
+        if (strcmp(ITERS[k].key, key) == 0) return &ITERS[k];
+    return NULL;
+}
+
+static int kmax_of(const orc_op* ops, int n) {
+    int m = 1;
+    for (int k = 0; k < n; k++)
+        if (ops[k].op == 'K' && ops[k].index > m) m = ops[k].index;
+    return m;
+}
+
+/* ERROR_K (runge_sum_to_error, kernel_summation.cuh:270-289) on interior cells, psi+
+   counted twice with -tetm (cuda_macro.cuh:244); norm = sum |buffer psi+|^2 on
+   interior cells (Solver::adaptTimeStep, solver.cu:73-74). */
+static void error_sum(const ws_t* w, int nw, const double* wts, double* err_sum, double* norm) {
+    const orc_params* p = w->p;
+    int slot[8];
+    for (int j = 0; j < nw; j++) slot[j] = j;
+    double es = 0.0, ns = 0.0;
+    for (int g = 0; g < w->S; g++)
+        for (int r = w->h; r < w->h + w->sgr; r++)
+            for (int c = w->h; c < w->h + w->sgc; c++) {
+                size_t i = (size_t)g * w->N2h + (size_t)r * w->ro + c;
+                double s[2];
+                ksum(w, w->kpsi[0], 2, i, nw, slot, wts, s);
+                double er = w->dt * s[0], ei = w->dt * s[1];
+                double e = er * er + ei * ei;
+                es += p->tetm ? 2.0 * e : e;
+                ns += w->bpsi[0][2 * i] * w->bpsi[0][2 * i] + w->bpsi[0][2 * i + 1] * w->bpsi[0][2 * i + 1];
+            }
+    *err_sum = es;
+    *norm = ns;
+}
+
+/* Run one step() sequence on the workspace; returns 1 if the result is in the buffer. */
+static int run_ops(ws_t* w, const orc_op* ops, int n, double* err_sum, double* norm) {
+    int in_buffer = 0;
+    sync_state(w);
+    for (int k = 0; k < n; k++) {
+        const orc_op* o = &ops[k];
+        switch (o->op) {
+            case 'K': calc_k(w, o->index, o->w, o->index > 1); break;
+            case 'I': sum_to_buffer(w, o->index, o->nw, o->wts); in_buffer = 1; break;
+            case 'F': final_sum(w, o->index, o->nw, o->wts); in_buffer = 0; break;
+            case 'E': error_sum(w, o->nw, o->wts, err_sum, norm); break;
+        }
+    }
+    return in_buffer;
+}
+
+int orc_iter_info(const char* key, int* halo, double* power, int* stages_fixed, int* stages_adaptive) {
+    const orc_iter* it = find_iter(key);
+    if (!it) return -1;
+    if (halo) *halo = it->halo;
+    if (power) *power = it->power;
+    if (stages_fixed) *stages_fixed = kmax_of(it->fixed, it->nfixed);
+    if (stages_adaptive) *stages_adaptive = it->nadapt ? kmax_of(it->adapt, it->nadapt) : 0;
+    return 0;
+}
+
+int orc_iter_run(const orc_params* p, const char* key, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                 const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                 double dt, int steps, int64_t step0) {
+    const orc_iter* it = find_iter(key);
+    if (!it) return -3;
+    ws_t w;
+    if (ws_init(&w, p, it->halo)) return -1;
+    w.dt = dt;
+    load_state(&w, psi_plus, psi_minus, n_plus, n_minus, pump_plus, pump_minus, pot_plus, pot_minus);
+    for (int s = 0; s < steps; s++) {
+        gen_noise(&w, step0 + s, dt);
+        double es, ns;
+        run_ops(&w, it->fixed, it->nfixed, &es, &ns);
+    }
+    store_state(&w, psi_plus, psi_minus, n_plus, n_minus, 0);
+    ws_free(&w);
+    return 0;
+}
+
+int orc_iter_attempt(const orc_params* p, const char* key, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                     const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                     double dt, double noise_dt, int64_t step_index, double* err_sum, double* norm) {
+    const orc_iter* it = find_iter(key);
+    if (!it) return -3;
+    if (!it->nadapt) return -2;
+    ws_t w;
+    if (ws_init(&w, p, it->halo)) return -1;
+    w.dt = dt;
+    load_state(&w, psi_plus, psi_minus, n_plus, n_minus, pump_plus, pump_minus, pot_plus, pot_minus);
+    gen_noise(&w, step_index, noise_dt);
+    *err_sum = 0.0;
+    *norm = 0.0;
+    run_ops(&w, it->adapt, it->nadapt, err_sum, norm);
+    store_state(&w, psi_plus, psi_minus, n_plus, n_minus, 1); /* swapBuffers (solver.cu:107-114) */
+    ws_free(&w);
+    return 0;
+}
+
+int orc_adapt_dt_pow(double err_sum, double norm, double tolerance, double dt, double dt_min, double dt_max, double power,
+                     double* new_dt, double* final_error) {
+    double fe = fabs(err_sum / norm);
+    double dh_arg = tolerance / 2.0 / (fe > DBL_MIN ? fe : DBL_MIN);
+    double dh = pow(dh_arg, power);
+    if (isnan(dh)) { dh = 0.9; fe = DBL_MAX; }
+    if (isnan(fe)) { dh = 0.9; fe = DBL_MAX; }
+    double nd = dt * dh < dt_max ? dt * dh : dt_max;
+    if (nd < dt_min) nd = dt_min;
+    *new_dt = nd;
+    *final_error = fe;
+    return fe < tolerance;
+}
+
+int64_t orc_iter_adaptive_run(const orc_params* p, const char* key, double* psi_plus, double* psi_minus, double* n_plus,
+                              double* n_minus, const double* pump_plus, const double* pump_minus, const double* pot_plus,
+                              const double* pot_minus, double* t, double* dt, double t_max, double tolerance, double dt_min,
+                              double dt_max, int max_rejections, int64_t* n_rejected) {
+    const orc_iter* it = find_iter(key);
+    if (!it) return -3;
+    if (!it->nadapt) return -2;
+    const size_t N = (size_t)p->N_c * p->N_r;
+    const int nc = p->tetm ? 2 : 1;
+    double *ps[2] = {psi_plus, psi_minus}, *ns[2] = {n_plus, n_minus};
+    double *tp[2] = {0, 0}, *tn[2] = {0, 0};
+    for (int c = 0; c < nc; c++) {
+        tp[c] = malloc(2 * N * sizeof(double));
+        tn[c] = malloc(N * sizeof(double));
+    }
+    int64_t accepted = 0, rejected = 0, step = 0;
+    int64_t rc = 0;
+    while (*t < t_max) {
+        const double noise_dt = *dt; /* noise drawn once per iterate() with sqrt(dt) (solver.cu:36-45) */
+        *t = *t + *dt;               /* solver.cu:50, before the attempts */
+        int rej = 0;
+        for (;;) {
+            for (int c = 0; c < nc; c++) {
+                memcpy(tp[c], ps[c], 2 * N * sizeof(double));
+                if (p->use_reservoir) memcpy(tn[c], ns[c], N * sizeof(double));
+            }
+            double es, nm, nd, fe;
+            if (orc_iter_attempt(p, key, tp[0], tp[1], tn[0], tn[1], pump_plus, pump_minus, pot_plus, pot_minus, *dt, noise_dt, step,
+                                 &es, &nm)) {
+                rc = -1;
+                goto done;
+            }
+            int acc = orc_adapt_dt_pow(es, nm, tolerance, *dt, dt_min, dt_max, it->power, &nd, &fe);
+            *dt = nd;
+            if (acc) break;
+            rejected++;
+            if (++rej > max_rejections) {
+                rc = -2;
+                goto done;
+            }
+        }
+        for (int c = 0; c < nc; c++) {
+            memcpy(ps[c], tp[c], 2 * N * sizeof(double));
+            if (p->use_reservoir) memcpy(ns[c], tn[c], N * sizeof(double));
+        }
+        accepted++;
+        step++;
+    }
+    rc = accepted;
+done:
+    for (int c = 0; c < nc; c++) { free(tp[c]); free(tn[c]); }
+    *n_rejected = rejected;
+    return rc;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -82,6 +82,29 @@ int64_t orc_dp45_run(const orc_params* p, double* psi_plus, double* psi_minus, d
                      double* t, double* dt, double t_max, double tolerance, double dt_min, double dt_max,
                      int max_rejections, int64_t* n_rejected);
 
+/* Generic iterators: the step() sequences of the reference iterators
+   (solver/iterator/<name>.cu) restated as data and run on the subgridded workspace with
+   the reference's halo_size_.  key: a REGISTER_SOLVER key ("Newton", "MP", "Heun2",
+   "Ralston", "Heun3", "Ralston3", "RK3", "SSPRK3", "HW", "RK4", "Ralston4", "Rule38",
+   "Nystroem", "Fehlberg2", "BoSha", "CashKarp", "Fehlberg5", "DP45"); -3 if unknown.
+   orc_iter_info: halo, adaptTimeStep power (0: fixed-step only) and stage counts.
+   orc_iter_run: `steps` fixed steps (step(false)).
+   orc_iter_attempt: one step(true) attempt, as orc_dp45_attempt (-2 if fixed-step only).
+   orc_iter_adaptive_run: as orc_dp45_run with the iterator's power. */
+int orc_iter_info(const char* key, int* halo, double* power, int* stages_fixed, int* stages_adaptive);
+int orc_iter_run(const orc_params* p, const char* key, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                 const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                 double dt, int steps, int64_t step0);
+int orc_iter_attempt(const orc_params* p, const char* key, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                     const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                     double dt, double noise_dt, int64_t step_index, double* err_sum, double* norm);
+int orc_adapt_dt_pow(double err_sum, double norm, double tolerance, double dt, double dt_min, double dt_max, double power,
+                     double* new_dt, double* final_error);
+int64_t orc_iter_adaptive_run(const orc_params* p, const char* key, double* psi_plus, double* psi_minus, double* n_plus,
+                              double* n_minus, const double* pump_plus, const double* pump_minus, const double* pot_plus,
+                              const double* pot_minus, double* t, double* dt, double t_max, double tolerance, double dt_min,
+                              double dt_max, int max_rejections, int64_t* n_rejected);
+
 /* k-space filter (solver_fft.cu:67-100, kernel_fft.cuh:15-39):
    psi <- IDFT( DFT(psi) / (N_c*N_r) * fftshift(mask) ), unnormalised transforms,
    forward sign e^{-2 pi i ...}.  mask is the unshifted N_r x N_c real mask. */

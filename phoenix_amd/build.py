@@ -34,10 +34,14 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unuse
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include")]
 # The streaming kernels keep their row windows in registers only when the row loop is
 # unrolled completely; the 7-stage spinor kernels exceed LLVM's default pragma threshold.
-HIP_FLAGS_FILE = {"phx_rk_generic.hip": ["-mllvm", "-pragma-unroll-threshold=2000000"]}
+_UNROLL = ["-mllvm", "-pragma-unroll-threshold=2000000"]
+HIP_FLAGS_FILE = {f: _UNROLL for f in ("phx_rk_generic.hip", "phx_rk_tab3.hip", "phx_rk_tab4.hip", "phx_rk_tab6.hip",
+                                     "phx_rk_tab6f.hip")}
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include")]
 
-HIP_SOURCES = ["phx_rk4.hip", "phx_rk_generic.hip", "phx_fft.hip", "phx_solver.hip", "phx_ensemble.hip", "phx_adjoint.hip"]
+# longest compiles first: the thread pool then finishes them side by side
+HIP_SOURCES = ["phx_rk_generic.hip", "phx_rk_tab6.hip", "phx_rk_tab6f.hip", "phx_rk_tab4.hip", "phx_rk_tab3.hip", "phx_rk4.hip", "phx_fft.hip",
+               "phx_solver.hip", "phx_ensemble.hip", "phx_adjoint.hip"]
 CXX_SOURCES = ["phx_config.cpp"]
 
 

@@ -4,6 +4,10 @@
  */
 #include "phx_config.hpp"
 
+#include <strings.h>
+
+#include "phx_tableaux.h"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -555,19 +559,22 @@ std::string parse_config(SystemConfig& c, const std::vector<std::string>& argv_i
 }
 
 std::string finalize_config(SystemConfig& c) {
-    /* iterator registry of this library (solver_factory.hpp:53-60 throws on unknown keys) */
-    std::string it = c.iterator;
-    for (auto& ch : it) ch = (char)std::toupper((unsigned char)ch);
-    if (it == "RK4") {
-        c.iterator = "RK4";
-        c.halo = 4;
-        if (c.adaptive) c.adaptive = false; /* "Cannot use variable time step with this solver." (solver.cu:23-26) */
-    } else if (it == "DP45") {
-        c.iterator = "DP45";
-        c.halo = 7;
-    } else {
-        return "iterator '" + c.iterator + "' is not provided by this library (available: RK4, DP45)";
+    /* iterator registry (solver_factory.hpp:53-60 throws on unknown keys); keys match
+       without regard to case.  An adaptive request on an iterator registered without
+       variable time stepping runs it fixed ("Cannot use variable time step with this
+       solver.", solver.cu:23-26). */
+    int n_it = 0;
+    const phx::IteratorInfo* table = phx::iterator_table(&n_it);
+    const phx::IteratorInfo* found = nullptr;
+    std::string available;
+    for (int k = 0; k < n_it; k++) {
+        if (strcasecmp(table[k].key, c.iterator.c_str()) == 0) found = &table[k];
+        available += (k ? ", " : "") + std::string(table[k].key);
     }
+    if (!found) return "iterator '" + c.iterator + "' is not provided by this library (available: " + available + ")";
+    c.iterator = found->key;
+    if (c.adaptive && !found->can_adapt) c.adaptive = false;
+    c.halo = phx::rk_tableau_stages(c.adaptive ? found->adaptive : found->fixed);
     if (c.imag_time_amplitude != 0.0) return "--imagTime is not provided by this library";
     if (c.use_pulses) return "--pulse is not provided by this library";
     /* calculateAuto (system.cpp:116-160) */

@@ -32,6 +32,20 @@ __device__ __forceinline__ int xcd_contiguous(int b, int nblocks) {
     return start + (b >> 3);
 }
 
+/* Row index of a tile row: inside the grid, wrapped once (periodic; a grid shorter
+   than the tile's reach falls back to a full modulo), or outside (zero boundary). */
+__device__ __forceinline__ int tile_row(int g, int Nr, int periodic, bool& in) {
+    in = true;
+    if (g >= 0 && g < Nr) return g;
+    if (!periodic) {
+        in = false;
+        return 0;
+    }
+    g += g < 0 ? Nr : -Nr;
+    if (g < 0 || g >= Nr) g = ((g % Nr) + Nr) % Nr;
+    return g;
+}
+
 template <int CTRL>
 __device__ __forceinline__ float dpp_move(float v) {
     return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xF, 0xF, true));

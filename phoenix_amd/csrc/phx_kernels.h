@@ -8,6 +8,7 @@
 #include <stdint.h>
 
 #include "custom_rhs.h"
+#include "phx_tableaux.h"
 
 namespace phx {
 
@@ -141,22 +142,35 @@ struct RkGenArgs {
     real dt;
     /* temporal amplitudes at t and t+dt; a(w) = a(t) + w (a(t+dt) - a(t)) */
     real amp_pump, amp_next_pump, amp_pot_h, amp_next_pot_h;
-    /* noise: xi = noise_amp * N2(seed, step, cell), noise_amp = stochastic amplitude * sqrt(dt) */
-    uint64_t noise_seed, noise_step;
-    real noise_amp;
+    /* noise field of this step (launch_noise_field), read with the state; nullptr without noise */
+    const typename Vec2<real>::type* noise;
     double* partials; /* 2 doubles per wavefront (error, norm), adaptive tableaux only */
     double* sums;     /* 2 doubles: reduced error and norm */
     PhxPhys<real> ph;
 };
 
-enum class RkTableau { RK4, DP45_FIXED, DP45_ADAPTIVE };
-
 template <typename real>
 hipError_t launch_rk_generic(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
+/* groups of tableaux instantiated in their own translation units (1-3, 3-4 and 6 stages) */
+template <typename real>
+hipError_t launch_rk_tab3(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
+template <typename real>
+hipError_t launch_rk_tab4(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
+template <typename real>
+hipError_t launch_rk_tab6(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
+template <typename real>
+hipError_t launch_rk_tab6f(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
+
+/* noise field of one step: xi[cell] = amp * N2(seed, step, cell) over the whole grid (row-major
+   cell index), N2 the Philox4x32-10 + Box-Muller pair of phx_noise.h; amp = stochastic amplitude
+   * sqrt(dt).  The fused kernels read it like a field (0 outside a zero-BC domain). */
+template <typename real>
+hipError_t launch_noise_field(typename Vec2<real>::type* xi, int Nc, int Nr, uint64_t seed, uint64_t step, real amp,
+                              hipStream_t s);
 
 /* wavefront tiles of the generic kernel for a grid (partials buffer size = 2 * this) */
 inline int rk_generic_waves(RkTableau tab, int Nc, int Nr) {
-    const int S = tab == RkTableau::RK4 ? 4 : tab == RkTableau::DP45_FIXED ? 6 : 7;
+    const int S = rk_tableau_stages(tab);
     const int outw = 64 - 2 * S, ty = 16;
     return ((Nc + outw - 1) / outw) * ((Nr + ty - 1) / ty);
 }

@@ -41,20 +41,6 @@ namespace phx {
 constexpr int kHalo = 4;
 constexpr int kOutW = kWave - 2 * kHalo; /* 56 */
 
-/* Row index of a tile row: inside the grid, wrapped once (periodic; a grid shorter
-   than the tile's reach falls back to a full modulo), or outside (zero boundary). */
-__device__ __forceinline__ int tile_row(int g, int Nr, int periodic, bool& in) {
-    in = true;
-    if (g >= 0 && g < Nr) return g;
-    if (!periodic) {
-        in = false;
-        return 0;
-    }
-    g += g < 0 ? Nr : -Nr;
-    if (g < 0 || g >= Nr) g = ((g % Nr) + Nr) % Nr;
-    return g;
-}
-
 /* Neighbour part of the stencil: kin_x (W + (E + r (N+S))) with r = kin_y / kin_x,
    both east/west moves folded into DPP adds.  ISO (dx == dy, r == 1 exactly) drops
    the multiply by r; the two forms give identical bits for dx == dy, so a grid gets

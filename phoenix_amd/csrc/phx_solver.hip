@@ -69,9 +69,11 @@ struct phx_solver {
     void* mask[2] = {nullptr, nullptr};
     void* tw_c = nullptr; /* forward FFT twiddles of the row length N_c */
     void* tw_r = nullptr; /* ... and of the column length N_r */
+    void* noise = nullptr;      /* noise field of the current step (complex, real precision) */
     double* partials = nullptr; /* per-wavefront (error, norm) of the adaptive kernel */
     double* sums = nullptr;     /* reduced (error, norm), device */
     double* sums_host = nullptr; /* pinned host copy */
+    const phx::IteratorInfo* iter = nullptr; /* registry entry of cfg.iterator */
     int max_rejections = 10000;
     double last_error = 0; /* E of the last adaptive attempt */
     /* time bookkeeping in the build's precision (system_parameters.hpp:33) */
@@ -139,8 +141,13 @@ int initialize_t(phx_solver* s) {
     auto& c = s->cfg;
     const size_t N = s->cells();
     const int ncomp = c.tetm ? 2 : 1;
-    if (c.iterator == "DP45" && c.adaptive) {
-        const int nw = phx::rk_generic_waves(phx::RkTableau::DP45_ADAPTIVE, c.N_c, c.N_r);
+    int n_it = 0;
+    const phx::IteratorInfo* table = phx::iterator_table(&n_it);
+    for (int k = 0; k < n_it && !s->iter; k++)
+        if (c.iterator == table[k].key) s->iter = &table[k];
+    if (!s->iter) return fail(PHX_ERR_UNSUPPORTED, "iterator '" + c.iterator + "' is not provided by this library");
+    if (c.adaptive && s->iter->can_adapt) {
+        const int nw = phx::rk_generic_waves(s->iter->adaptive, c.N_c, c.N_r);
         PHX_HIP(hipMalloc((void**)&s->partials, 2 * sizeof(double) * (size_t)nw));
         s->allocs.push_back(s->partials);
         PHX_HIP(hipMalloc((void**)&s->sums, 2 * sizeof(double)));
@@ -169,6 +176,8 @@ int initialize_t(phx_solver* s) {
         if (c.use_fft_mask)
             if (int r = dev_alloc(s, &s->mask[k], N * sizeof(real))) return r;
     }
+    if (c.use_stochastic)
+        if (int r = dev_alloc(s, &s->noise, 2 * N * sizeof(real))) return r;
     if (c.use_fft_mask) {
         for (int d = 0; d < 2; d++) {
             const int L = d == 0 ? c.N_c : c.N_r;
@@ -268,12 +277,14 @@ int rk4_launch(phx_solver* s, double t_pre, double dt) {
     return PHX_OK;
 }
 
-/* Solver::adaptTimeStep (solver.cu:72-105), power 1/4, on FP64 sums over the grid:
-   E = |err / norm|, dh = (tol / 2 / max(E, DBL_MIN))^(1/4), NaN -> dh = 0.9 and reject,
-   dt' = max(min(dt dh, dt_max), dt_min), accept iff E < tol. */
-bool adapt_dt(double err, double norm, double tol, double dt, double dt_min, double dt_max, double* new_dt, double* final_error) {
+/* Solver::adaptTimeStep (solver.cu:72-105) on FP64 sums over the grid:
+   E = |err / norm|, dh = (tol / 2 / max(E, DBL_MIN))^power, NaN -> dh = 0.9 and reject,
+   dt' = max(min(dt dh, dt_max), dt_min), accept iff E < tol.  power: 1/4 for DP45,
+   CashKarp, Fehlberg2 and Fehlberg5, 1/3 for BoSha (the iterators' adaptTimeStep calls). */
+bool adapt_dt(double err, double norm, double tol, double dt, double dt_min, double dt_max, double power, double* new_dt,
+              double* final_error) {
     double fe = std::fabs(err / norm);
-    double dh = std::pow(tol / 2.0 / std::max(fe, DBL_MIN), 0.25);
+    double dh = std::pow(tol / 2.0 / std::max(fe, DBL_MIN), power);
     if (std::isnan(dh) || std::isnan(fe)) {
         dh = 0.9;
         fe = DBL_MAX;
@@ -287,7 +298,7 @@ bool adapt_dt(double err, double norm, double tol, double dt, double dt_min, dou
 
 /* One fused generic-tableau step from the current buffers into the other ones. */
 template <typename real>
-int generic_launch(phx_solver* s, phx::RkTableau tab, double t_k, double dt, double noise_dt) {
+int generic_launch(phx_solver* s, phx::RkTableau tab, double t_k, double dt) {
     auto& c = s->cfg;
     using v2 = typename phx::Vec2<real>::type;
     phx::RkGenArgs<real> a{};
@@ -315,9 +326,7 @@ int generic_launch(phx_solver* s, phx::RkTableau tab, double t_k, double dt, dou
     a.amp_next_pump = pa[1];
     a.amp_pot_h = (real)((double)va[0] * c.one_over_h_bar_s);
     a.amp_next_pot_h = (real)((double)va[1] * c.one_over_h_bar_s);
-    a.noise_seed = c.random_seed;
-    a.noise_step = (uint64_t)s->iteration;
-    a.noise_amp = (real)(c.stochastic_amplitude * std::sqrt(noise_dt));
+    a.noise = (const v2*)s->noise;
     a.partials = s->partials;
     a.sums = s->sums;
     a.ph = phx::make_phys<real>(c);
@@ -397,9 +406,14 @@ int iterate_t(phx_solver* s, bool force_fixed) {
     auto& c = s->cfg;
     if (s->t >= c.t_max) return 0;
     const double t_pre = s->t;
-    const double noise_dt = s->dt; /* the noise of this step is drawn once with sqrt(dt) (solver.cu:36-45) */
+    if (c.use_stochastic) {
+        /* the noise of this step is drawn once, with sqrt(dt), before any attempt (solver.cu:36-45) */
+        using v2 = typename phx::Vec2<real>::type;
+        PHX_HIP(phx::launch_noise_field<real>((v2*)s->noise, c.N_c, c.N_r, c.random_seed, (uint64_t)s->iteration,
+                                              (real)(c.stochastic_amplitude * std::sqrt(s->dt)), s->stream));
+    }
     s->t = s->rnd(s->t + s->dt);   /* t += dt before the step (solver.cu:50) */
-    const bool adaptive = c.iterator == "DP45" && c.adaptive && !force_fixed;
+    const bool adaptive = c.adaptive && s->iter->can_adapt && !force_fixed;
     if (c.iterator == "RK4" && !c.tetm && !c.use_stochastic) {
         if (!s->tuned) {
             if (int r = autotune_rk4<real>(s)) return r;
@@ -407,20 +421,21 @@ int iterate_t(phx_solver* s, bool force_fixed) {
         }
         if (int r = rk4_launch<real>(s, t_pre, s->dt)) return r;
     } else if (!adaptive) {
-        const auto tab = c.iterator == "RK4" ? phx::RkTableau::RK4 : phx::RkTableau::DP45_FIXED;
-        if (int r = generic_launch<real>(s, tab, t_pre, s->dt, noise_dt)) return r;
+        if (int r = generic_launch<real>(s, s->iter->fixed, t_pre, s->dt)) return r;
         s->cur ^= 1;
     } else {
-        /* DormandPrince45::step(true) + Solver::adaptTimeStep (dormand_prince.cu:37-95, solver.cu:72-105) */
+        /* step(true) of the adaptive iterators (dormand_prince.cu:37-95, bogacki.cu:28-75,
+           cash_karp.cu:31-83, fehlberg.cu:33-70,100-152) + Solver::adaptTimeStep (solver.cu:72-105) */
         int rejections = 0;
         for (;;) {
             /* retries see the kernel time already advanced (updateKernelTime in adaptTimeStep, solver.cu:97) */
             const double t_k = rejections == 0 ? t_pre : s->t;
-            if (int r = generic_launch<real>(s, phx::RkTableau::DP45_ADAPTIVE, t_k, s->dt, noise_dt)) return r;
+            if (int r = generic_launch<real>(s, s->iter->adaptive, t_k, s->dt)) return r;
             PHX_HIP(hipMemcpyAsync(s->sums_host, s->sums, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
             PHX_HIP(hipStreamSynchronize(s->stream));
             double new_dt = 0, final_error = 0;
-            const bool accept = adapt_dt(s->sums_host[0], s->sums_host[1], c.tolerance, s->dt, c.dt_min, c.dt_max, &new_dt, &final_error);
+            const bool accept = adapt_dt(s->sums_host[0], s->sums_host[1], c.tolerance, s->dt, c.dt_min, c.dt_max,
+                                         s->iter->power, &new_dt, &final_error);
             s->last_error = final_error;
             s->dt = s->rnd(new_dt);
             if (accept) break;

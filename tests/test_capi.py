@@ -86,7 +86,8 @@ def test_envelope_flags_enable_reservoir_and_pump():
     (["--N", 31, 32], -1),               # odd dimension
     (["--N", 32, 32, "--tstep", 0], -1),  # dt <= 0
     (["--N", 32, 32, "--tstep", 2, "--tmax", 1], -1),  # dt > t_max
-    (["--iterator", "RK3"], -3),          # not provided by this library
+    (["--iterator", "DP85"], -3),         # not provided by this library
+    (["--iterator", "NoSuchKey"], -3),    # unknown key (solver_factory.hpp:59)
     (["--imagTime", 1.0], -3),
     (["--N", "abc", 32], -1),
     (["--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "notatype"], -1),
@@ -102,6 +103,23 @@ def test_adaptive_flag_only_with_adaptive_iterator():
     assert make(["--N", 32, 32, "-adaptive"]).info["adaptive"] == 0  # RK4 cannot adapt (solver.cu:23-26)
     i = make(["--N", 32, 32, "--iterator", "DP45", "-adaptive", "--tol", 1e-6, "--rkvdt", 1e-4, 0.1]).info
     assert i["adaptive"] == 1 and i["halo"] == 7 and i["tolerance"] == 1e-6 and i["dt_max"] == 0.1
+
+
+# REGISTER_SOLVER keys served by the fused kernel: (stages fixed, stages adaptive or None)
+ITERATORS = {"Newton": (1, None), "MP": (2, None), "Heun2": (2, None), "Ralston": (2, None), "Heun3": (3, None),
+             "Ralston3": (3, None), "RK3": (3, None), "SSPRK3": (3, None), "HW": (3, None), "RK4": (4, None),
+             "Ralston4": (4, None), "Rule38": (4, None), "Nystroem": (6, None), "Fehlberg2": (3, 3), "BoSha": (3, 4),
+             "CashKarp": (6, 6), "Fehlberg5": (6, 6), "DP45": (6, 7)}
+
+
+@pytest.mark.parametrize("key", list(ITERATORS))
+def test_every_registered_iterator_key_is_accepted(key):
+    fixed, adaptive = ITERATORS[key]
+    i = make(["--N", 32, 32, "--iterator", key.lower()]).info  # keys match without regard to case
+    assert i["iterator"] == key and i["halo"] == fixed and i["adaptive"] == 0
+    i = make(["--N", 32, 32, "--iterator", key, "-adaptive"]).info
+    # iterators registered without variable time stepping run fixed (solver.cu:23-26)
+    assert i["adaptive"] == (adaptive is not None) and i["halo"] == (adaptive or fixed)
 
 
 def test_physics_defaults_match_reference_constants():
@@ -146,7 +164,7 @@ def test_executable_rejects_invalid_input_with_exit_code_1(tmp_path):
     r = subprocess.run([exe, "--N", "64", "64", "--tstep", "-1", "--path", str(tmp_path)], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1
-    r = subprocess.run([exe, "--N", "64", "64", "--iterator", "SSPRK3", "--path", str(tmp_path)], capture_output=True,
+    r = subprocess.run([exe, "--N", "64", "64", "--iterator", "DP85", "--path", str(tmp_path)], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 1
-    assert "SSPRK3" in (r.stdout + r.stderr)
+    assert "DP85" in (r.stdout + r.stderr)
