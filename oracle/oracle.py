@@ -34,6 +34,7 @@ class OrcParams(C.Structure):
         ("stochastic_amplitude", C.c_double),
         ("pump_amp", C.c_double), ("pump_amp_next", C.c_double), ("pot_amp", C.c_double), ("pot_amp_next", C.c_double),
         ("noise_seed", C.c_uint64),
+        ("imag_time_amplitude", C.c_double),
     ]
 
 
@@ -75,6 +76,7 @@ def lib():
         L.orc_iter_run.argtypes = [P, cs, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_int, C.c_int64]
         L.orc_iter_attempt.argtypes = [P, cs, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int64, dp, dp]
         L.orc_adapt_dt_pow.argtypes = [C.c_double] * 7 + [dp, dp]
+        L.orc_imag_normalize.argtypes = [P, dp, dp]
         L.orc_iter_adaptive_run.argtypes = [P, cs, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double,
                                             C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int64)]
         L.orc_iter_adaptive_run.restype = C.c_int64

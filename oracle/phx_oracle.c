@@ -31,6 +31,7 @@ void orc_defaults(orc_params* p) {
     p->delta_LT = 0.025e-3;
     p->N_c = p->N_r = 400;
     p->subgrids_columns = p->subgrids_rows = 1;
+    p->imag_time_amplitude = 0.0;
     p->pump_amp = p->pump_amp_next = 1.0;
     p->pot_amp = p->pot_amp_next = 1.0;
 }
@@ -373,6 +374,7 @@ static void add_dw(const ws_t* w, size_t i, double wsum, double* out_r, double* 
    on extent halo-index; single weight -> slot index-1, else slots 0..n-1. */
 static void sum_to_buffer(ws_t* w, int index, int nw, const double* wts) {
     const int ch = w->h - index;
+    const int imag = w->p->imag_time_amplitude != 0.0;
     int slot[8];
     for (int j = 0; j < nw; j++) slot[j] = (nw == 1) ? index - 1 : j;
     double wsum = 0.0;
@@ -385,8 +387,9 @@ static void sum_to_buffer(ws_t* w, int index, int nw, const double* wts) {
                 for (int comp = 0; comp < w->ncomp; comp++) {
                     double s[2];
                     ksum(w, w->kpsi[comp], 2, i, nw, slot, wts, s);
-                    double orr = w->psi[comp][2 * i] + dt * s[0];
-                    double oi = w->psi[comp][2 * i + 1] + dt * s[1];
+                    /* complex_dt: input + (0, -dt) * sum (kernel_summation.cuh:195-201) */
+                    double orr = imag ? w->psi[comp][2 * i] + dt * s[1] : w->psi[comp][2 * i] + dt * s[0];
+                    double oi = imag ? w->psi[comp][2 * i + 1] - dt * s[0] : w->psi[comp][2 * i + 1] + dt * s[1];
                     add_dw(w, i, wsum, &orr, &oi);
                     w->bpsi[comp][2 * i] = orr;
                     w->bpsi[comp][2 * i + 1] = oi;
@@ -402,6 +405,7 @@ static void sum_to_buffer(ws_t* w, int index, int nw, const double* wts) {
    padded region; single weight -> slot index-1, else slots 0..n-1. */
 static void final_sum(ws_t* w, int index, int nw, const double* wts) {
     const int ch = w->h;
+    const int imag = w->p->imag_time_amplitude != 0.0;
     int slot[8];
     for (int j = 0; j < nw; j++) slot[j] = (nw == 1) ? index - 1 : j;
     double wsum = 0.0;
@@ -417,8 +421,13 @@ static void final_sum(ws_t* w, int index, int nw, const double* wts) {
                 for (int comp = 0; comp < w->ncomp; comp++) {
                     double s[2];
                     ksum(w, w->kpsi[comp], 2, i, nw, slot, wts, s);
-                    w->psi[comp][2 * i] += dt * s[0];
-                    w->psi[comp][2 * i + 1] += dt * s[1];
+                    if (imag) { /* (0, -dt) * sum (kernel_summation.cuh:227-233) */
+                        w->psi[comp][2 * i] += dt * s[1];
+                        w->psi[comp][2 * i + 1] -= dt * s[0];
+                    } else {
+                        w->psi[comp][2 * i] += dt * s[0];
+                        w->psi[comp][2 * i + 1] += dt * s[1];
+                    }
                     w->psi[comp][2 * i] += dwr;
                     w->psi[comp][2 * i + 1] += dwi;
                 }
@@ -823,6 +832,45 @@ static void error_sum(const ws_t* w, int nw, const double* wts, double* err_sum,
     *norm = ns;
 }
 
+/* normalizeImaginaryTimePropagation (solver_normalize_imag_time.cu:13-45) on the interior cells:
+   psi_k *= sqrt(amp / (S_k dV)), S_k = sum |psi_k|^2 (< 1e-10 -> 1).  The reference sums its padded
+   subgrids including halo cells; this sums each grid cell once.  Without a reservoir the reservoir
+   normalisation of the reference scales zeros. */
+static void ws_imag_normalize(ws_t* w) {
+    const orc_params* p = w->p;
+    for (int comp = 0; comp < w->ncomp; comp++) {
+        double sum = 0.0;
+        for (int g = 0; g < w->S; g++)
+            for (int r = w->h; r < w->h + w->sgr; r++)
+                for (int c = w->h; c < w->h + w->sgc; c++) {
+                    size_t i = (size_t)g * w->N2h + (size_t)r * w->ro + c;
+                    sum += w->psi[comp][2 * i] * w->psi[comp][2 * i] + w->psi[comp][2 * i + 1] * w->psi[comp][2 * i + 1];
+                }
+        if (sum < 1e-10) sum = 1.0;
+        const double f = sqrt(p->imag_time_amplitude / (sum * p->dV));
+        for (int g = 0; g < w->S; g++)
+            for (int r = w->h; r < w->h + w->sgr; r++)
+                for (int c = w->h; c < w->h + w->sgc; c++) {
+                    size_t i = (size_t)g * w->N2h + (size_t)r * w->ro + c;
+                    w->psi[comp][2 * i] *= f;
+                    w->psi[comp][2 * i + 1] *= f;
+                }
+    }
+}
+
+void orc_imag_normalize(const orc_params* p, double* psi_plus, double* psi_minus) {
+    const size_t N = (size_t)p->N_c * p->N_r;
+    double* f[2] = {psi_plus, psi_minus};
+    for (int comp = 0; comp < 2; comp++) {
+        if (!f[comp]) continue;
+        double sum = 0.0;
+        for (size_t i = 0; i < N; i++) sum += f[comp][2 * i] * f[comp][2 * i] + f[comp][2 * i + 1] * f[comp][2 * i + 1];
+        if (sum < 1e-10) sum = 1.0;
+        const double s = sqrt(p->imag_time_amplitude / (sum * p->dV));
+        for (size_t i = 0; i < 2 * N; i++) f[comp][i] *= s;
+    }
+}
+
 /* Run one step() sequence on the workspace; returns 1 if the result is in the buffer. */
 static int run_ops(ws_t* w, const orc_op* ops, int n, double* err_sum, double* norm) {
     int in_buffer = 0;
@@ -862,6 +910,7 @@ int orc_iter_run(const orc_params* p, const char* key, double* psi_plus, double*
         gen_noise(&w, step0 + s, dt);
         double es, ns;
         run_ops(&w, it->fixed, it->nfixed, &es, &ns);
+        if (p->imag_time_amplitude != 0.0) ws_imag_normalize(&w);
     }
     store_state(&w, psi_plus, psi_minus, n_plus, n_minus, 0);
     ws_free(&w);
@@ -946,6 +995,7 @@ int64_t orc_iter_adaptive_run(const orc_params* p, const char* key, double* psi_
             memcpy(ps[c], tp[c], 2 * N * sizeof(double));
             if (p->use_reservoir) memcpy(ns[c], tn[c], N * sizeof(double));
         }
+        if (p->imag_time_amplitude != 0.0) orc_imag_normalize(p, ps[0], nc > 1 ? ps[1] : NULL);
         accepted++;
         step++;
     }

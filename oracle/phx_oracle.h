@@ -43,6 +43,9 @@ typedef struct {
     double pump_amp, pump_amp_next, pot_amp, pot_amp_next;
     /* noise stream (own Philox4x32-10 design, see orc_noise_normal2) */
     uint64_t noise_seed;
+    /* --imagTime amplitude; != 0: psi advances by -i dt and is normalised after every step
+       (cuda_macro.cuh:178,225; solver_normalize_imag_time.cu:13-45) */
+    double imag_time_amplitude;
 } orc_params;
 
 /* Fill derived constants exactly as SystemParameters::calculateAuto does.
@@ -98,6 +101,10 @@ int orc_iter_run(const orc_params* p, const char* key, double* psi_plus, double*
 int orc_iter_attempt(const orc_params* p, const char* key, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
                      const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
                      double dt, double noise_dt, int64_t step_index, double* err_sum, double* norm);
+/* Imaginary-time normalisation of full-grid fields: psi_k *= sqrt(amp / (S_k dV)), S_k = sum of
+   |psi_k|^2 over the grid (S_k < 1e-10 -> 1); psi_minus may be NULL.
+   (solver_normalize_imag_time.cu:13-45) */
+void orc_imag_normalize(const orc_params* p, double* psi_plus, double* psi_minus);
 int orc_adapt_dt_pow(double err_sum, double norm, double tolerance, double dt, double dt_min, double dt_max, double power,
                      double* new_dt, double* final_error);
 int64_t orc_iter_adaptive_run(const orc_params* p, const char* key, double* psi_plus, double* psi_minus, double* n_plus,

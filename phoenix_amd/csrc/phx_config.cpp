@@ -575,7 +575,9 @@ std::string finalize_config(SystemConfig& c) {
     c.iterator = found->key;
     if (c.adaptive && !found->can_adapt) c.adaptive = false;
     c.halo = phx::rk_tableau_stages(c.adaptive ? found->adaptive : found->fixed);
-    if (c.imag_time_amplitude != 0.0) return "--imagTime is not provided by this library";
+    /* imaginary time turns the reference's (complex-stored) reservoir complex; the reservoir here is real */
+    if (c.imag_time_amplitude != 0.0 && c.use_reservoir)
+        return "--imagTime with a reservoir (pump or initial reservoir) is not provided by this library";
     if (c.use_pulses) return "--pulse is not provided by this library";
     /* calculateAuto (system.cpp:116-160) */
     if (c.h_bar_s < 0) c.h_bar_s = c.h_bar / c.e_e * 1E12;

@@ -139,6 +139,7 @@ struct RkGenArgs {
     int periodic_x, periodic_y;
     int nstrips, nbands; /* set by the launcher */
     int tetm, stochastic;
+    int imag; /* imaginary time: psi advances by -i dt */
     real dt;
     /* temporal amplitudes at t and t+dt; a(w) = a(t) + w (a(t+dt) - a(t)) */
     real amp_pump, amp_next_pump, amp_pot_h, amp_next_pot_h;
@@ -160,6 +161,15 @@ template <typename real>
 hipError_t launch_rk_tab6(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
 template <typename real>
 hipError_t launch_rk_tab6f(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
+
+/* Imaginary-time normalisation (Solver::normalizeImaginaryTimePropagation): psi_k *= sqrt(amp /
+   (S_k dV)) with S_k = sum over the grid of |psi_k|^2 (FP64, fixed order; S_k < 1e-10 -> 1), for
+   psi_plus and, if not null, psi_minus.  partials: 2 doubles per 256-cell block, sums: 2 doubles;
+   no host synchronisation. */
+template <typename real>
+hipError_t launch_imag_normalize(typename Vec2<real>::type* psi_plus, typename Vec2<real>::type* psi_minus, size_t cells,
+                                 double amp, double dV, double* partials, double* sums, hipStream_t s);
+inline size_t imag_norm_blocks(size_t cells) { return (cells + 255) / 256; }
 
 /* noise field of one step: xi[cell] = amp * N2(seed, step, cell) over the whole grid (row-major
    cell index), N2 the Philox4x32-10 + Box-Muller pair of phx_noise.h; amp = stochastic amplitude

@@ -99,6 +99,10 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
     ph.mhalf_gamma_c = in_vgpr(ph.mhalf_gamma_c);
     ph.gamma_r = in_vgpr(ph.gamma_r);
     const real dt = a.dt;
+    /* psi advances by dt (real time) or by -i dt (imaginary time, --imagTime):
+       psi0 + (dtr + i dti) S with (dtr, dti) = (dt, 0) or (0, -dt); the reservoir always by dt */
+    const real dtr = a.imag ? real(0) : dt;
+    const real dti = a.imag ? -dt : real(0);
 
     /* per component, per extended row */
     real p0r[NC][E], p0i[NC][E], n0[NC][E], pp[NC][E], vv[NC][E];
@@ -216,8 +220,8 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
                         sn = phx_fma(wj, Kn[k][j][e], sn);
                     }
                 }
-                real yr = phx_fma(dt, sr, p0r[k][e]);
-                real yi = phx_fma(dt, si, p0i[k][e]);
+                real yr = phx_fma(dtr, sr, phx_fma(-dti, si, p0r[k][e]));
+                real yi = phx_fma(dtr, si, phx_fma(dti, sr, p0i[k][e]));
                 if (STOCH) {
                     yr = phx_fma(wsum * xr[e], sg[e], yr);
                     yi = phx_fma(wsum * xi[e], sg[e], yi);
@@ -254,8 +258,8 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
                     sn = phx_fma(wj, Kn[k][j][e], sn);
                 }
             }
-            real o_r = phx_fma(dt, sr, p0r[k][e]);
-            real o_i = phx_fma(dt, si, p0i[k][e]);
+            real o_r = phx_fma(dtr, sr, phx_fma(-dti, si, p0r[k][e]));
+            real o_i = phx_fma(dtr, si, phx_fma(dti, sr, p0i[k][e]));
             if (STOCH) {
                 o_r = phx_fma(bsum * xr[e], sg[e], o_r);
                 o_i = phx_fma(bsum * xi[e], sg[e], o_i);

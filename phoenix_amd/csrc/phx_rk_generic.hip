@@ -2,8 +2,8 @@
  * phx_rk_generic.hip -- instantiations of the fused Runge-Kutta kernel
  * (phx_rk_generic.h) for RK4 (with noise or TE-TM; the plain scalar RK4 has its own
  * hand-tuned kernel in phx_rk4.hip) and Dormand-Prince 5(4) fixed and adaptive, the
- * deterministic reduction of the adaptive error partials, the per-step noise field, and
- * the dispatch over all tableaux (the other groups are instantiated in phx_rk_tab3.hip,
+ * deterministic reduction of the adaptive error partials, the per-step noise field, the
+ * imaginary-time normalisation, and the dispatch over all tableaux (the other groups are instantiated in phx_rk_tab3.hip,
  * phx_rk_tab4.hip, phx_rk_tab6.hip and phx_rk_tab6f.hip).
  */
 #include "phx_rk_generic.h"
@@ -38,6 +38,75 @@ hipError_t launch_reduce_partials(const double* partials, int nwaves, double* su
     hipLaunchKernelGGL(reduce_partials, dim3(1), dim3(256), 0, s, partials, nwaves, sums);
     return hipGetLastError();
 }
+
+/* per-256-cell FP64 sums of |psi+|^2 and |psi-|^2, fixed order inside the block */
+template <typename real>
+__global__ __launch_bounds__(256) void sumsq_blocks(const typename Vec2<real>::type* __restrict__ a,
+                                                    const typename Vec2<real>::type* __restrict__ b, size_t cells,
+                                                    double* __restrict__ partials) {
+    __shared__ double sa[256], sb[256];
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    double x = 0.0, y = 0.0;
+    if (i < cells) {
+        const auto u = a[i];
+        x = (double)u.x * (double)u.x + (double)u.y * (double)u.y;
+        if (b) {
+            const auto v = b[i];
+            y = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+        }
+    }
+    sa[threadIdx.x] = x;
+    sb[threadIdx.x] = y;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+            sb[threadIdx.x] += sb[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partials[2 * blockIdx.x] = sa[0];
+        partials[2 * blockIdx.x + 1] = sb[0];
+    }
+}
+
+/* psi_k *= sqrt(amp / (S_k dV)), S_k < 1e-10 -> 1 (solver_normalize_imag_time.cu:13-45) */
+template <typename real>
+__global__ __launch_bounds__(256) void imag_scale(typename Vec2<real>::type* __restrict__ a, typename Vec2<real>::type* __restrict__ b,
+                                                  size_t cells, const double* __restrict__ sums, double amp, double dV) {
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= cells) return;
+    const double s0 = sums[0] < 1e-10 ? 1.0 : sums[0];
+    const real f0 = (real)sqrt(amp / (s0 * dV));
+    auto u = a[i];
+    u.x *= f0;
+    u.y *= f0;
+    a[i] = u;
+    if (b) {
+        const double s1 = sums[1] < 1e-10 ? 1.0 : sums[1];
+        const real f1 = (real)sqrt(amp / (s1 * dV));
+        auto v = b[i];
+        v.x *= f1;
+        v.y *= f1;
+        b[i] = v;
+    }
+}
+
+template <typename real>
+hipError_t launch_imag_normalize(typename Vec2<real>::type* psi_plus, typename Vec2<real>::type* psi_minus, size_t cells,
+                                 double amp, double dV, double* partials, double* sums, hipStream_t s) {
+    const size_t nb = imag_norm_blocks(cells);
+    hipLaunchKernelGGL(sumsq_blocks<real>, dim3((unsigned)nb), dim3(256), 0, s, psi_plus, psi_minus, cells, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = launch_reduce_partials(partials, (int)nb, sums, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(imag_scale<real>, dim3((unsigned)nb), dim3(256), 0, s, psi_plus, psi_minus, cells, sums, amp, dV);
+    return hipGetLastError();
+}
+
+template hipError_t launch_imag_normalize<float>(float2*, float2*, size_t, double, double, double*, double*, hipStream_t);
+template hipError_t launch_imag_normalize<double>(double2*, double2*, size_t, double, double, double*, double*, hipStream_t);
 
 /* one complex normal pair per cell, scaled by amp (own Philox stream, see phx_noise.h) */
 template <typename real>

@@ -70,6 +70,8 @@ struct phx_solver {
     void* tw_c = nullptr; /* forward FFT twiddles of the row length N_c */
     void* tw_r = nullptr; /* ... and of the column length N_r */
     void* noise = nullptr;      /* noise field of the current step (complex, real precision) */
+    double* norm_partials = nullptr; /* imaginary-time normalisation: per-block sums, reduced sums */
+    double* norm_sums = nullptr;
     double* partials = nullptr; /* per-wavefront (error, norm) of the adaptive kernel */
     double* sums = nullptr;     /* reduced (error, norm), device */
     double* sums_host = nullptr; /* pinned host copy */
@@ -178,6 +180,10 @@ int initialize_t(phx_solver* s) {
     }
     if (c.use_stochastic)
         if (int r = dev_alloc(s, &s->noise, 2 * N * sizeof(real))) return r;
+    if (c.imag_time_amplitude != 0.0) {
+        if (int r = dev_alloc(s, (void**)&s->norm_partials, 2 * sizeof(double) * phx::imag_norm_blocks(N))) return r;
+        if (int r = dev_alloc(s, (void**)&s->norm_sums, 2 * sizeof(double))) return r;
+    }
     if (c.use_fft_mask) {
         for (int d = 0; d < 2; d++) {
             const int L = d == 0 ? c.N_c : c.N_r;
@@ -317,6 +323,7 @@ int generic_launch(phx_solver* s, phx::RkTableau tab, double t_k, double dt) {
     a.periodic_y = c.periodic_y;
     a.tetm = c.tetm;
     a.stochastic = c.use_stochastic;
+    a.imag = c.imag_time_amplitude != 0.0;
     a.dt = (real)dt;
     static const double w01[2] = {0.0, 1.0};
     real pa[2], va[2];
@@ -414,7 +421,8 @@ int iterate_t(phx_solver* s, bool force_fixed) {
     }
     s->t = s->rnd(s->t + s->dt);   /* t += dt before the step (solver.cu:50) */
     const bool adaptive = c.adaptive && s->iter->can_adapt && !force_fixed;
-    if (c.iterator == "RK4" && !c.tetm && !c.use_stochastic) {
+    const bool imag = c.imag_time_amplitude != 0.0;
+    if (c.iterator == "RK4" && !c.tetm && !c.use_stochastic && !imag) {
         if (!s->tuned) {
             if (int r = autotune_rk4<real>(s)) return r;
             s->tuned = true;
@@ -446,6 +454,13 @@ int iterate_t(phx_solver* s, bool force_fixed) {
         }
         s->cur ^= 1; /* swapBuffers (solver.cu:107-114) */
     }
+    if (imag) {
+        /* normalizeImaginaryTimePropagation after the step (solver.cu:55-56) */
+        using v2 = typename phx::Vec2<real>::type;
+        PHX_HIP(phx::launch_imag_normalize<real>((v2*)s->psi[0][s->cur], c.tetm ? (v2*)s->psi[1][s->cur] : nullptr,
+                                                 s->cells(), c.imag_time_amplitude, c.dV, s->norm_partials, s->norm_sums,
+                                                 s->stream));
+    }
     s->iteration++;
     if (s->rnd(s->t - s->fft_cached_t) < c.fft_every) return 1; /* FFT guard in the build's real (solver.cu:62-63) */
     s->fft_cached_t = s->t;
@@ -471,8 +486,8 @@ int graph_block_t(phx_solver* s, int64_t* taken) {
     auto& c = s->cfg;
     *taken = 0;
     const auto constant = [](const phx::Envelope& e) { return e.groupSize() != 1 || e.allConstant(); };
-    if (s->graph_off || !s->tuned || c.iterator != "RK4" || c.tetm || c.use_stochastic || !constant(c.pump) ||
-        !constant(c.potential))
+    if (s->graph_off || !s->tuned || c.iterator != "RK4" || c.tetm || c.use_stochastic || c.imag_time_amplitude != 0.0 ||
+        !constant(c.pump) || !constant(c.potential))
         return PHX_OK;
     if (const char* e = std::getenv("PHX_GRAPH"))
         if (std::atoi(e) == 0) return PHX_OK; /* PHX_GRAPH=0: launch by launch */

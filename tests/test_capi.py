@@ -88,7 +88,7 @@ def test_envelope_flags_enable_reservoir_and_pump():
     (["--N", 32, 32, "--tstep", 2, "--tmax", 1], -1),  # dt > t_max
     (["--iterator", "DP85"], -3),         # not provided by this library
     (["--iterator", "NoSuchKey"], -3),    # unknown key (solver_factory.hpp:59)
-    (["--imagTime", 1.0], -3),
+    (["--imagTime", 1.0, "--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss"], -3),  # real reservoir
     (["--N", "abc", 32], -1),
     (["--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "notatype"], -1),
 ])
