@@ -35,6 +35,7 @@ class OrcParams(C.Structure):
         ("pump_amp", C.c_double), ("pump_amp_next", C.c_double), ("pot_amp", C.c_double), ("pot_amp_next", C.c_double),
         ("noise_seed", C.c_uint64),
         ("imag_time_amplitude", C.c_double),
+        ("L_x", C.c_double), ("L_y", C.c_double),
     ]
 
 
@@ -77,6 +78,7 @@ def lib():
         L.orc_iter_attempt.argtypes = [P, cs, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int64, dp, dp]
         L.orc_adapt_dt_pow.argtypes = [C.c_double] * 7 + [dp, dp]
         L.orc_imag_normalize.argtypes = [P, dp, dp]
+        L.orc_ssfm_run.argtypes = [P, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_int, C.c_int64]
         L.orc_iter_adaptive_run.argtypes = [P, cs, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double,
                                             C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int64)]
         L.orc_iter_adaptive_run.restype = C.c_int64
@@ -283,6 +285,23 @@ def iter_adaptive_run(p, key, psi, n, pump=None, pot=None, t=0.0, dt=0.04, t_max
     if pm is not None:
         res.update(psi_minus=pm, n_minus=nm)
     return res
+
+
+def ssfm_run(p, psi, n=None, pump=None, pot=None, dt=0.04, steps=1, psi_minus=None, n_minus=None, pump_minus=None,
+             pot_minus=None, step0=0):
+    """`steps` split-step Fourier steps (orc_ssfm_run); returns as rk4_run."""
+    psi = _c128(psi)
+    pm = _c128(psi_minus) if psi_minus is not None else None
+    n = _f64(n) if n is not None else np.zeros(psi.shape)
+    nm = _f64(n_minus) if n_minus is not None else (np.zeros(psi.shape) if psi_minus is not None else None)
+    rc = lib().orc_ssfm_run(C.byref(p), _dp(psi.view(np.float64)), _dp(pm.view(np.float64)) if pm is not None else None,
+                            _dp(n), _dp(nm), _dp(_f64(pump)), _dp(_f64(pump_minus)), _dp(_f64(pot)), _dp(_f64(pot_minus)),
+                            dt, steps, step0)
+    if rc:
+        raise RuntimeError(f"orc_ssfm_run failed: {rc}")
+    if psi_minus is not None:
+        return psi, n, pm, nm
+    return psi, n
 
 
 def fft_filter(psi, mask):

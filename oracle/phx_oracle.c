@@ -49,6 +49,8 @@ double orc_derive(orc_params* p, double L_x, double L_y, double h_bar, double e_
     p->m2_over_dx2_p_dy2 = -2.0 * (p->one_over_dx2 + p->one_over_dy2);
     p->h_bar_s = h_bar_s;
     p->m_eff = m_eff;
+    p->L_x = L_x;
+    p->L_y = L_y;
     p->m_eff_scaled = -0.5 * h_bar_s * h_bar_s / m_eff;
     p->one_over_h_bar_s = 1.0 / h_bar_s;
     return 0.5 * p->dx * p->dy / dt_scaling_factor;
@@ -1077,6 +1079,103 @@ int orc_fft_filter(double* psi, const double* mask, int N_c, int N_r) {
             f[1] = f[1] / (double)N * m;
         }
     return orc_fft2d(psi, N_c, N_r, +1);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Split-step Fourier (split_step_fourier_method.cu:25-71)                  */
+/* ------------------------------------------------------------------------ */
+/* gp_scalar_linear_fourier (kernel_gp_compute.cuh:244-262) between a forward and an inverse
+   transform: F <- F / N2 * exp(-i hbar/(2 m) (kx^2 + ky^2) dt / 2) */
+static int ssfm_linear(const orc_params* p, double* f, double dt) {
+    const int Nc = p->N_c, Nr = p->N_r;
+    const double N2 = (double)Nc * Nr;
+    if (orc_fft2d(f, Nc, Nr, -1)) return -1;
+    for (int r = 0; r < Nr; r++)
+        for (int c = 0; c < Nc; c++) {
+            const double kx = 2.0 * 3.1415926535 * (double)(c <= Nc / 2 ? c : c - Nc) / p->L_x;
+            const double ky = 2.0 * 3.1415926535 * (double)(r <= Nr / 2 ? r : r - Nr) / p->L_y;
+            const double lin = p->h_bar_s / 2.0 / p->m_eff * (kx * kx + ky * ky);
+            const double ph = -lin * dt / 2.0;
+            double* v = f + 2 * ((size_t)r * Nc + c);
+            const double ar = v[0] / N2, ai = v[1] / N2;
+            v[0] = ar * cos(ph) - ai * sin(ph);
+            v[1] = ar * sin(ph) + ai * cos(ph);
+        }
+    return orc_fft2d(f, Nc, Nr, +1);
+}
+
+int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                 const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                 double dt, int steps, int64_t step0) {
+    const size_t N = (size_t)p->N_c * p->N_r;
+    const int nc = p->tetm ? 2 : 1;
+    double* psi[2] = {psi_plus, psi_minus};
+    double* n[2] = {n_plus, n_minus};
+    const double* pump[2] = {pump_plus, pump_minus};
+    const double* pot[2] = {pot_plus, pot_minus};
+    const int stoch = p->use_stochastic && p->stochastic_amplitude > 0.0;
+    double* nn[2] = {0, 0};
+    double* xi = calloc(2 * N, sizeof(double));
+    for (int k = 0; k < nc; k++) nn[k] = calloc(N, sizeof(double));
+    for (int st = 0; st < steps; st++) {
+        /* per-step noise: amplitude * sqrt(dt) * N2(seed, step, cell) (solver.cu:36-45) */
+        if (stoch)
+            for (size_t i = 0; i < N; i++) {
+                double re, im;
+                orc_noise_normal2(p->noise_seed, (uint64_t)(step0 + st), (uint32_t)i, 0, &re, &im);
+                xi[2 * i] = p->stochastic_amplitude * sqrt(dt) * re;
+                xi[2 * i + 1] = p->stochastic_amplitude * sqrt(dt) * im;
+            }
+        for (int k = 0; k < nc; k++)
+            if (ssfm_linear(p, psi[k], dt)) return -1;
+        /* gp_scalar_nonlinear (kernel_gp_compute.cuh:264-382) */
+        for (size_t i = 0; i < N; i++) {
+            double nrm[2] = {0, 0}, nv[2] = {0, 0};
+            for (int k = 0; k < nc; k++) {
+                nrm[k] = psi[k][2 * i] * psi[k][2 * i] + psi[k][2 * i + 1] * psi[k][2 * i + 1];
+                nv[k] = p->use_reservoir ? n[k][i] : 0.0;
+            }
+            for (int k = 0; k < nc; k++) {
+                /* result = (g_c |psi|^2, -hbar/2 gamma_c) + V a + g_r n + i hbar/2 R n (+ g_pm |psi_o|^2) (- g_c/dV) */
+                double A = p->g_c * nrm[k];
+                if (p->use_potential) A += pot[k][i] * p->pot_amp;
+                A += p->g_r * nv[k];
+                if (p->tetm) A += p->g_pm * nrm[1 - k];
+                if (stoch) A -= p->g_c / p->dV;
+                const double B = -p->h_bar_s * 0.5 * p->gamma_c + p->h_bar_s * 0.5 * p->R * nv[k];
+                /* psi * exp((B - i A) dt / hbar) */
+                const double er = B * p->one_over_h_bar_s * dt, ei = -A * p->one_over_h_bar_s * dt;
+                const double m = exp(er), fr = m * cos(ei), fi = m * sin(ei);
+                const double pr = psi[k][2 * i], pi = psi[k][2 * i + 1];
+                psi[k][2 * i] = pr * fr - pi * fi;
+                psi[k][2 * i + 1] = pr * fi + pi * fr;
+                if (p->use_reservoir) {
+                    double res = -p->gamma_r * nv[k];
+                    res -= p->R * nrm[k] * nv[k];
+                    if (p->use_pump) res += pump[k][i] * p->pump_amp;
+                    if (stoch) res += p->R * nv[k] / p->dV;
+                    nn[k][i] = nv[k] + res * dt;
+                }
+            }
+        }
+        for (int k = 0; k < nc; k++)
+            if (ssfm_linear(p, psi[k], dt)) return -1;
+        /* gp_scalar_independent (kernel_gp_compute.cuh:387-440) */
+        for (size_t i = 0; i < N; i++) {
+            for (int k = 0; k < nc; k++) {
+                if (!stoch) continue;
+                /* scalar: the reservoir of the step start; TE-TM: the new one of the component */
+                const double nr = p->use_reservoir ? (p->tetm ? nn[k][i] : n[0][i]) : 0.0;
+                const double sg = sqrt((p->R * nr + p->gamma_c) / (4.0 * p->dV));
+                psi[k][2 * i] += xi[2 * i] * sg;
+                psi[k][2 * i + 1] += xi[2 * i + 1] * sg;
+            }
+            if (p->use_reservoir && !p->tetm) n[0][i] = nn[0][i];
+        }
+    }
+    free(xi);
+    for (int k = 0; k < nc; k++) free(nn[k]);
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */

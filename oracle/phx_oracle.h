@@ -46,6 +46,8 @@ typedef struct {
     /* --imagTime amplitude; != 0: psi advances by -i dt and is normalised after every step
        (cuda_macro.cuh:178,225; solver_normalize_imag_time.cu:13-45) */
     double imag_time_amplitude;
+    /* domain lengths (set by orc_derive; the split-step iterator's wave numbers use them) */
+    double L_x, L_y;
 } orc_params;
 
 /* Fill derived constants exactly as SystemParameters::calculateAuto does.
@@ -101,6 +103,16 @@ int orc_iter_run(const orc_params* p, const char* key, double* psi_plus, double*
 int orc_iter_attempt(const orc_params* p, const char* key, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
                      const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
                      double dt, double noise_dt, int64_t step_index, double* err_sum, double* norm);
+/* Split-step Fourier iterator: `steps` SSFM::step() calls on full-grid fields
+   (split_step_fourier_method.cu:25-71, gp_scalar_linear_fourier / nonlinear / independent,
+   kernel_gp_compute.cuh:244-440).  Transforms: unnormalised 2D DFT of the N_r x N_c
+   row-major grid (orc_fft2d).  Pump / potential amplitudes pump_amp / pot_amp (the value at
+   the step start).  With -tetm the reservoirs stay at their input values, as in the
+   reference (its independent step never installs the new TE-TM reservoir). */
+int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
+                 const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
+                 double dt, int steps, int64_t step0);
+
 /* Imaginary-time normalisation of full-grid fields: psi_k *= sqrt(amp / (S_k dV)), S_k = sum of
    |psi_k|^2 over the grid (S_k < 1e-10 -> 1); psi_minus may be NULL.
    (solver_normalize_imag_time.cu:13-45) */

@@ -571,10 +571,18 @@ std::string finalize_config(SystemConfig& c) {
         if (strcasecmp(table[k].key, c.iterator.c_str()) == 0) found = &table[k];
         available += (k ? ", " : "") + std::string(table[k].key);
     }
-    if (!found) return "iterator '" + c.iterator + "' is not provided by this library (available: " + available + ")";
-    c.iterator = found->key;
-    if (c.adaptive && !found->can_adapt) c.adaptive = false;
-    c.halo = phx::rk_tableau_stages(c.adaptive ? found->adaptive : found->fixed);
+    if (strcasecmp(c.iterator.c_str(), "SSFM") == 0) {
+        /* split-step Fourier (split_step_fourier_method.cu:13-20): no halo, fixed step */
+        c.iterator = "SSFM";
+        c.adaptive = false;
+        c.halo = 0;
+    } else {
+        if (!found)
+            return "iterator '" + c.iterator + "' is not provided by this library (available: " + available + ", SSFM)";
+        c.iterator = found->key;
+        if (c.adaptive && !found->can_adapt) c.adaptive = false;
+        c.halo = phx::rk_tableau_stages(c.adaptive ? found->adaptive : found->fixed);
+    }
     /* imaginary time turns the reference's (complex-stored) reservoir complex; the reservoir here is real */
     if (c.imag_time_amplitude != 0.0 && c.use_reservoir)
         return "--imagTime with a reservoir (pump or initial reservoir) is not provided by this library";

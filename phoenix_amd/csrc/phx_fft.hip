@@ -145,10 +145,14 @@ __device__ __forceinline__ int fft_xcd_group(int b, int nblocks) {
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
-/* CB adjacent columns: forward FFT along the column, F <- F / (N_c N_r) * mask[shifted], inverse FFT. */
-template <typename real, int CB, int KB>
+/* CB adjacent columns: forward FFT along the column, a pointwise factor in k-space, inverse FFT.
+   PROP = false: F <- F / (N_c N_r) * mask[shifted] (the k-space filter);
+   PROP = true:  F <- F * colf[c] * rowf[r] (the split-step propagator; the tables carry 1 / (N_c N_r)). */
+template <typename real, int CB, int KB, bool PROP>
 __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw,
-                                                              const real* __restrict__ mask, int Nc, int Nr, int log2Nr, real Ntot) {
+                                                              const real* __restrict__ mask, const Cx<real>* __restrict__ rowf,
+                                                              const Cx<real>* __restrict__ colf, int Nc, int Nr, int log2Nr,
+                                                              real Ntot) {
     extern __shared__ unsigned char lds_raw[];
     Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
     const int c0 = fft_xcd_group(blockIdx.x, gridDim.x) * CB;
@@ -159,15 +163,23 @@ __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<rea
     }
     __syncthreads();
     stockham<real, -1, KB>(A, tw, Nr, log2Nr, CB);
-    /* mask of the quadrant-swapped grid (fft_shift_2D, kernel_fft.cuh:15-32), applied as data / N * mask */
-    for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
-        const int r = i / CB, j = i - r * CB; /* row-major walk: consecutive threads read consecutive mask cells */
-        const int rs = (r + (Nr >> 1)) & (Nr - 1);
-        const int cs = (c0 + j + (Nc >> 1)) & (Nc - 1);
-        const real m = mask[(size_t)rs * Nc + cs];
-        Cx<real>& f = A[j * Nr + r];
-        f.r = f.r / Ntot * m;
-        f.i = f.i / Ntot * m;
+    if constexpr (PROP) {
+        for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
+            const int r = i / CB, j = i - r * CB;
+            Cx<real>& f = A[j * Nr + r];
+            f = cmul(cmul(f, colf[c0 + j]), rowf[r]);
+        }
+    } else {
+        /* mask of the quadrant-swapped grid (fft_shift_2D, kernel_fft.cuh:15-32), applied as data / N * mask */
+        for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
+            const int r = i / CB, j = i - r * CB; /* row-major walk: consecutive threads read consecutive mask cells */
+            const int rs = (r + (Nr >> 1)) & (Nr - 1);
+            const int cs = (c0 + j + (Nc >> 1)) & (Nc - 1);
+            const real m = mask[(size_t)rs * Nc + cs];
+            Cx<real>& f = A[j * Nr + r];
+            f.r = f.r / Ntot * m;
+            f.i = f.i / Ntot * m;
+        }
     }
     __syncthreads();
     stockham<real, +1, KB>(A, tw, Nr, log2Nr, CB);
@@ -219,29 +231,33 @@ static hipError_t launch_rows(typename Vec2<real>::type* psi, const void* tw_c, 
     return hipGetLastError();
 }
 
-template <typename real, int CB>
-static hipError_t launch_cols(typename Vec2<real>::type* psi, const void* tw_r, const real* mask, int Nc, int Nr, hipStream_t s) {
+template <typename real, int CB, bool PROP>
+static hipError_t launch_cols(typename Vec2<real>::type* psi, const void* tw_r, const real* mask, const void* rowf, const void* colf,
+                              int Nc, int Nr, hipStream_t s) {
     const size_t lds = (size_t)CB * Nr * sizeof(Cx<real>);
     const int kb = bfly_bound(CB * (Nr / 4));
     const auto* tw = (const Cx<real>*)tw_r;
+    const auto* rf = (const Cx<real>*)rowf;
+    const auto* cf = (const Cx<real>*)colf;
     const int l = ilog2(Nr);
     const real ntot = (real)((double)Nc * (double)Nr);
     if (kb == 1)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 1>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, Nc, Nr, l, ntot);
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 1, PROP>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot);
     else if (kb == 2)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 2>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, Nc, Nr, l, ntot);
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 2, PROP>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot);
     else if (kb == 4)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 4>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, Nc, Nr, l, ntot);
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 4, PROP>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot);
     else if constexpr (max_bfly<real>() >= 8)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 8>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, Nc, Nr, l, ntot);
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 8, PROP>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-template <typename real>
-hipError_t launch_fft_filter(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, int Nc, int Nr,
-                             hipStream_t s) {
+/* rows forward -> columns (forward, k-space factor, inverse) -> rows inverse */
+template <typename real, bool PROP>
+static hipError_t fft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, const void* rowf,
+                                const void* colf, int Nc, int Nr, hipStream_t s) {
     hipError_t e = launch_rows<real, -1>(psi, tw_c, Nc, Nr, s);
     if (e != hipSuccess) return e;
     /* column block width: as wide as LDS and registers allow, but no wider than keeps
@@ -251,17 +267,31 @@ hipError_t launch_fft_filter(typename Vec2<real>::type* psi, const void* tw_c, c
         return per_col * cb <= 65536 && Nc % cb == 0 && cb * (Nr / 4) <= max_bfly<real>() * kFftThreads && (cb == 1 || Nc / cb >= 512);
     };
     if (fits(8))
-        e = launch_cols<real, 8>(psi, tw_r, mask, Nc, Nr, s);
+        e = launch_cols<real, 8, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
     else if (fits(4))
-        e = launch_cols<real, 4>(psi, tw_r, mask, Nc, Nr, s);
+        e = launch_cols<real, 4, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
     else if (fits(2))
-        e = launch_cols<real, 2>(psi, tw_r, mask, Nc, Nr, s);
+        e = launch_cols<real, 2, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
     else
-        e = launch_cols<real, 1>(psi, tw_r, mask, Nc, Nr, s);
+        e = launch_cols<real, 1, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
     if (e != hipSuccess) return e;
     return launch_rows<real, +1>(psi, tw_c, Nc, Nr, s);
 }
 
+template <typename real>
+hipError_t launch_fft_filter(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, int Nc, int Nr,
+                             hipStream_t s) {
+    return fft_roundtrip<real, false>(psi, tw_c, tw_r, mask, nullptr, nullptr, Nc, Nr, s);
+}
+
+template <typename real>
+hipError_t launch_fft_propagate(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const void* rowf, const void* colf,
+                                int Nc, int Nr, hipStream_t s) {
+    return fft_roundtrip<real, true>(psi, tw_c, tw_r, nullptr, rowf, colf, Nc, Nr, s);
+}
+
+template hipError_t launch_fft_propagate<float>(float2*, const void*, const void*, const void*, const void*, int, int, hipStream_t);
+template hipError_t launch_fft_propagate<double>(double2*, const void*, const void*, const void*, const void*, int, int, hipStream_t);
 template hipError_t launch_fft_filter<float>(float2*, const void*, const void*, const float*, int, int, hipStream_t);
 template hipError_t launch_fft_filter<double>(double2*, const void*, const void*, const double*, int, int, hipStream_t);
 

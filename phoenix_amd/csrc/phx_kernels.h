@@ -192,6 +192,31 @@ template <typename real>
 hipError_t launch_fft_filter(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, int Nc, int Nr,
                              hipStream_t s);
 
+/* Split-step linear propagator in place: psi <- IDFT(DFT(psi) * colf[c] * rowf[r]), with the
+   separable complex factors rowf (N_r), colf (N_c) computed on the host (they carry 1/(N_c N_r)). */
+template <typename real>
+hipError_t launch_fft_propagate(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const void* rowf, const void* colf,
+                                int Nc, int Nr, hipStream_t s);
+
+/* Pointwise parts of the split-step Fourier iterator (phx_ssfm.hip).  Index 0 = plus, 1 = minus. */
+template <typename real>
+struct SsfmArgs {
+    typename Vec2<real>::type* psi[2];
+    real* n_cur[2]; /* reservoir of the step start (nullptr without reservoir) */
+    real* n_nxt[2]; /* reservoir after the nonlinear step */
+    const real* pump[2];
+    const real* pot[2];
+    const typename Vec2<real>::type* noise; /* this step's noise field (stochastic only) */
+    size_t cells;
+    int tetm, stochastic;
+    real dt, pamp, vamp; /* temporal amplitudes of pump and potential at the step start */
+    PhxPhys<real> ph;
+};
+template <typename real>
+hipError_t launch_ssfm_nonlinear(const SsfmArgs<real>& a, hipStream_t s);
+template <typename real>
+hipError_t launch_ssfm_independent(const SsfmArgs<real>& a, hipStream_t s);
+
 /* Power of two in [4, 4096] whose two LDS row buffers fit in 64 KiB. */
 bool fft_supported(int N, size_t real_size);
 

@@ -75,7 +75,11 @@ struct phx_solver {
     double* partials = nullptr; /* per-wavefront (error, norm) of the adaptive kernel */
     double* sums = nullptr;     /* reduced (error, norm), device */
     double* sums_host = nullptr; /* pinned host copy */
-    const phx::IteratorInfo* iter = nullptr; /* registry entry of cfg.iterator */
+    const phx::IteratorInfo* iter = nullptr; /* registry entry of cfg.iterator (nullptr for SSFM) */
+    bool ssfm = false;                        /* split-step Fourier iterator */
+    void* ssfm_rowf = nullptr;                /* linear half-step factors along rows (N_r) and columns (N_c) */
+    void* ssfm_colf = nullptr;
+    double ssfm_dt = -1.0;                    /* dt the factors were made for */
     int max_rejections = 10000;
     double last_error = 0; /* E of the last adaptive attempt */
     /* time bookkeeping in the build's precision (system_parameters.hpp:33) */
@@ -147,8 +151,12 @@ int initialize_t(phx_solver* s) {
     const phx::IteratorInfo* table = phx::iterator_table(&n_it);
     for (int k = 0; k < n_it && !s->iter; k++)
         if (c.iterator == table[k].key) s->iter = &table[k];
-    if (!s->iter) return fail(PHX_ERR_UNSUPPORTED, "iterator '" + c.iterator + "' is not provided by this library");
-    if (c.adaptive && s->iter->can_adapt) {
+    s->ssfm = c.iterator == "SSFM";
+    if (!s->iter && !s->ssfm) return fail(PHX_ERR_UNSUPPORTED, "iterator '" + c.iterator + "' is not provided by this library");
+    if (s->ssfm && (!phx::fft_supported(c.N_c, sizeof(real)) || !phx::fft_supported(c.N_r, sizeof(real))))
+        return fail(PHX_ERR_UNSUPPORTED, "SSFM needs power-of-two grid sizes between 4 and " +
+                                             std::to_string(sizeof(real) == 4 ? 8192 : 4096));
+    if (c.adaptive && s->iter && s->iter->can_adapt) {
         const int nw = phx::rk_generic_waves(s->iter->adaptive, c.N_c, c.N_r);
         PHX_HIP(hipMalloc((void**)&s->partials, 2 * sizeof(double) * (size_t)nw));
         s->allocs.push_back(s->partials);
@@ -184,7 +192,11 @@ int initialize_t(phx_solver* s) {
         if (int r = dev_alloc(s, (void**)&s->norm_partials, 2 * sizeof(double) * phx::imag_norm_blocks(N))) return r;
         if (int r = dev_alloc(s, (void**)&s->norm_sums, 2 * sizeof(double))) return r;
     }
-    if (c.use_fft_mask) {
+    if (s->ssfm) {
+        if (int r = dev_alloc(s, &s->ssfm_rowf, 2 * (size_t)c.N_r * sizeof(real))) return r;
+        if (int r = dev_alloc(s, &s->ssfm_colf, 2 * (size_t)c.N_c * sizeof(real))) return r;
+    }
+    if (c.use_fft_mask || s->ssfm) {
         for (int d = 0; d < 2; d++) {
             const int L = d == 0 ? c.N_c : c.N_r;
             std::vector<std::complex<double>> w(L);
@@ -353,6 +365,58 @@ int fft_filter_t(phx_solver* s) {
     return PHX_OK;
 }
 
+/* SSFM::step (split_step_fourier_method.cu:25-71): linear half step in k-space, nonlinear
+   full step in real space, linear half step, noise.  The linear factor of a cell is
+   exp(-i hbar k^2 / (2 m) dt / 2) / (N_c N_r) with k = 2 * 3.1415926535 * (index or
+   index - N) / L (gp_scalar_linear_fourier, kernel_gp_compute.cuh:244-262); it is separable
+   and kept as a row table and a column table, remade when dt changes.  psi is updated in
+   place; the reservoir goes through the spare buffer. */
+template <typename real>
+int ssfm_step(phx_solver* s, double t_pre) {
+    auto& c = s->cfg;
+    using v2 = typename phx::Vec2<real>::type;
+    if (s->ssfm_dt != s->dt) {
+        const double dt = s->dt, lin = c.h_bar_s / 2.0 / c.m_eff;
+        std::vector<std::complex<double>> rowf(c.N_r), colf(c.N_c);
+        for (int r = 0; r < c.N_r; r++) {
+            const double ky = 2.0 * 3.1415926535 * (double)(r <= c.N_r / 2 ? r : r - c.N_r) / c.L_y;
+            rowf[r] = std::polar(1.0, -lin * ky * ky * dt / 2.0) / ((double)c.N_c * (double)c.N_r);
+        }
+        for (int col = 0; col < c.N_c; col++) {
+            const double kx = 2.0 * 3.1415926535 * (double)(col <= c.N_c / 2 ? col : col - c.N_c) / c.L_x;
+            colf[col] = std::polar(1.0, -lin * kx * kx * dt / 2.0);
+        }
+        if (int r = upload_complex<real>(s, s->ssfm_rowf, rowf)) return r;
+        if (int r = upload_complex<real>(s, s->ssfm_colf, colf)) return r;
+        s->ssfm_dt = s->dt;
+    }
+    const int ncomp = c.tetm ? 2 : 1;
+    phx::SsfmArgs<real> a{};
+    for (int k = 0; k < ncomp; k++) {
+        a.psi[k] = (v2*)s->psi[k][s->cur];
+        a.n_cur[k] = c.use_reservoir ? (real*)s->n[k][s->cur] : nullptr;
+        a.n_nxt[k] = c.use_reservoir ? (real*)s->n[k][s->cur ^ 1] : nullptr;
+        a.pump[k] = (const real*)s->pump[k];
+        a.pot[k] = (const real*)s->pot[k];
+    }
+    a.noise = (const v2*)s->noise;
+    a.cells = s->cells();
+    a.tetm = c.tetm;
+    a.stochastic = c.use_stochastic;
+    a.dt = (real)s->dt;
+    static const double w0[1] = {0.0};
+    stage_amps<real>(c.pump, t_pre, s->dt, w0, 1, &a.pamp);
+    stage_amps<real>(c.potential, t_pre, s->dt, w0, 1, &a.vamp);
+    a.ph = phx::make_phys<real>(c);
+    for (int k = 0; k < ncomp; k++)
+        PHX_HIP(phx::launch_fft_propagate<real>(a.psi[k], s->tw_c, s->tw_r, s->ssfm_rowf, s->ssfm_colf, c.N_c, c.N_r, s->stream));
+    PHX_HIP(phx::launch_ssfm_nonlinear<real>(a, s->stream));
+    for (int k = 0; k < ncomp; k++)
+        PHX_HIP(phx::launch_fft_propagate<real>(a.psi[k], s->tw_c, s->tw_r, s->ssfm_rowf, s->ssfm_colf, c.N_c, c.N_r, s->stream));
+    PHX_HIP(phx::launch_ssfm_independent<real>(a, s->stream));
+    return PHX_OK;
+}
+
 /* Autotuner of the fused RK4 launch shape: time each candidate tile height on this
    grid (writing into the spare buffers, so the state is untouched) and keep the
    fastest.  Results are cached per (precision, grid, feature set) for the process.
@@ -420,9 +484,11 @@ int iterate_t(phx_solver* s, bool force_fixed) {
                                               (real)(c.stochastic_amplitude * std::sqrt(s->dt)), s->stream));
     }
     s->t = s->rnd(s->t + s->dt);   /* t += dt before the step (solver.cu:50) */
-    const bool adaptive = c.adaptive && s->iter->can_adapt && !force_fixed;
+    const bool adaptive = c.adaptive && s->iter && s->iter->can_adapt && !force_fixed;
     const bool imag = c.imag_time_amplitude != 0.0;
-    if (c.iterator == "RK4" && !c.tetm && !c.use_stochastic && !imag) {
+    if (s->ssfm) {
+        if (int r = ssfm_step<real>(s, t_pre)) return r;
+    } else if (c.iterator == "RK4" && !c.tetm && !c.use_stochastic && !imag) {
         if (!s->tuned) {
             if (int r = autotune_rk4<real>(s)) return r;
             s->tuned = true;

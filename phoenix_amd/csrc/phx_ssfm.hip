@@ -1,0 +1,124 @@
+/*
+ * phx_ssfm.hip -- pointwise parts of the split-step Fourier iterator (SSFM::step,
+ * split_step_fourier_method.cu:25-71); the linear half steps run through the LDS FFT
+ * (launch_fft_propagate, phx_fft.hip).
+ *
+ *   ssfm_nonlinear    : gp_scalar_nonlinear (kernel_gp_compute.cuh:264-382): per cell
+ *                       psi <- psi exp(dt (R n - gamma_c) / 2) exp(-i dt A / hbar),
+ *                       A = g_c |psi|^2 + V a_V + g_r n (+ g_pm |psi_other|^2 with TE-TM)
+ *                       (- g_c / dV with noise), and the explicit Euler reservoir step
+ *                       n' = n + dt (-(gamma_r + R |psi|^2) n + P a_P (+ R n / dV with noise));
+ *                       the TE-TM splitting delta_LT is not part of it (kernel_gp_compute.cuh:320,357).
+ *   ssfm_independent  : gp_scalar_independent (kernel_gp_compute.cuh:387-440): psi += xi
+ *                       sqrt((R n + gamma_c) / (4 dV)); the scalar model takes the reservoir of
+ *                       the step start and then installs the new one; the TE-TM model takes
+ *                       the new reservoir of each component and keeps the old one (the reference
+ *                       never copies it back, so its TE-TM reservoirs do not evolve under SSFM).
+ * One thread per cell, coalesced; the fields are the solver's full-grid arrays.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "custom_rhs.h"
+#include "phx_kernels.h"
+#include "phx_noise.h"
+
+namespace phx {
+
+namespace {
+__device__ __forceinline__ void phx_sincos_dev(float x, float& s, float& c) { sincosf(x, &s, &c); }
+__device__ __forceinline__ void phx_sincos_dev(double x, double& s, double& c) { sincos(x, &s, &c); }
+__device__ __forceinline__ float phx_exp_dev(float x) { return expf(x); }
+__device__ __forceinline__ double phx_exp_dev(double x) { return exp(x); }
+} // namespace
+
+template <typename real, bool TETM>
+__global__ __launch_bounds__(256) void ssfm_nonlinear(SsfmArgs<real> a) {
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.cells) return;
+    constexpr int NC = TETM ? 2 : 1;
+    const PhxPhys<real>& ph = a.ph;
+    typename Vec2<real>::type psi[NC];
+    real nrm[NC], n[NC];
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        psi[k] = a.psi[k][i];
+        nrm[k] = phx_fma(psi[k].x, psi[k].x, psi[k].y * psi[k].y);
+        n[k] = a.n_cur[k] ? a.n_cur[k][i] : real(0);
+    }
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        real A = ph.g_c * nrm[k];
+        if (a.pot[k]) A += a.pot[k][i] * a.vamp;
+        A += ph.g_r * n[k];
+        if (TETM) A += ph.g_pm * nrm[1 - k];
+        if (a.stochastic) A -= ph.g_c / ph.dV;
+        const real er = a.dt * (real(0.5) * (ph.R * n[k] - ph.gamma_c));
+        const real ei = -(A * ph.one_over_h_bar_s * a.dt);
+        real sn, cs;
+        phx_sincos_dev(ei, sn, cs);
+        const real m = phx_exp_dev(er);
+        const real fr = m * cs, fi = m * sn;
+        typename Vec2<real>::type o;
+        o.x = phx_fma(psi[k].x, fr, -(psi[k].y * fi));
+        o.y = phx_fma(psi[k].x, fi, psi[k].y * fr);
+        a.psi[k][i] = o;
+        if (a.n_nxt[k]) {
+            real rn = -(ph.gamma_r * n[k]);
+            rn -= ph.R * nrm[k] * n[k];
+            if (a.pump[k]) rn += a.pump[k][i] * a.pamp;
+            if (a.stochastic) rn += ph.R * n[k] / ph.dV;
+            a.n_nxt[k][i] = phx_fma(rn, a.dt, n[k]);
+        }
+    }
+}
+
+template <typename real, bool TETM>
+__global__ __launch_bounds__(256) void ssfm_independent(SsfmArgs<real> a) {
+    const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
+    if (i >= a.cells) return;
+    constexpr int NC = TETM ? 2 : 1;
+    const PhxPhys<real>& ph = a.ph;
+    const real n_old = a.n_cur[0] ? a.n_cur[0][i] : real(0);
+#pragma unroll
+    for (int k = 0; k < NC; k++) {
+        if (a.stochastic) {
+            const real nr = TETM ? (a.n_nxt[k] ? a.n_nxt[k][i] : real(0)) : n_old;
+            const real sg = phx_sqrt(phx_fma(ph.R, nr, ph.gamma_c) / (real(4) * ph.dV));
+            const auto xi = a.noise[i];
+            auto p = a.psi[k][i];
+            p.x = phx_fma(xi.x, sg, p.x);
+            p.y = phx_fma(xi.y, sg, p.y);
+            a.psi[k][i] = p;
+        }
+    }
+    /* scalar model: the new reservoir becomes current (TE-TM keeps the old one) */
+    if (!TETM && a.n_nxt[0]) a.n_cur[0][i] = a.n_nxt[0][i];
+}
+
+template <typename real>
+hipError_t launch_ssfm_nonlinear(const SsfmArgs<real>& a, hipStream_t s) {
+    const dim3 grid((unsigned)((a.cells + 255) / 256));
+    if (a.tetm)
+        hipLaunchKernelGGL((ssfm_nonlinear<real, true>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((ssfm_nonlinear<real, false>), grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename real>
+hipError_t launch_ssfm_independent(const SsfmArgs<real>& a, hipStream_t s) {
+    const dim3 grid((unsigned)((a.cells + 255) / 256));
+    if (a.tetm)
+        hipLaunchKernelGGL((ssfm_independent<real, true>), grid, dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((ssfm_independent<real, false>), grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+template hipError_t launch_ssfm_nonlinear<float>(const SsfmArgs<float>&, hipStream_t);
+template hipError_t launch_ssfm_nonlinear<double>(const SsfmArgs<double>&, hipStream_t);
+template hipError_t launch_ssfm_independent<float>(const SsfmArgs<float>&, hipStream_t);
+template hipError_t launch_ssfm_independent<double>(const SsfmArgs<double>&, hipStream_t);
+
+} // namespace phx

@@ -1,0 +1,74 @@
+"""GPU parity of the split-step Fourier iterator (--iterator SSFM / -ssfm) against the oracle
+(orc_ssfm_run): LDS FFT round trips with the separable linear propagator, the pointwise
+nonlinear step with the Euler reservoir update, and the noise step.
+
+Bars: FP64 relative L2 <= 1e-12, FP32 <= 1e-5 (FP32 transforms of 64..256 points round at
+about 1e-7 per pass).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+TOL = {"fp64": 1e-12, "fp32": 1e-5}
+PTH = 0.15 * 0.225 / 0.01
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("Nc,Nr,tetm,noise", [(64, 64, False, False), (128, 64, False, True), (64, 128, True, False),
+                                             (64, 64, True, True)])
+def test_ssfm_matches_oracle(Nc, Nr, tetm, noise, prec):
+    from phoenix_amd._lib import Solver
+    L, dt, steps, seed = 31.5, 0.02, 25, 31
+    Ly = L * (Nr - 1) / (Nc - 1)
+    say(f"[ssfm] {Nc}x{Nr} tetm={tetm} noise={noise} prec={prec}")
+    dx = L / (Nc - 1)
+    X, Y = np.meshgrid(-L / 2 + dx * np.arange(Nc), -Ly / 2 + dx * np.arange(Nr))
+    P = 1.5 * PTH * np.exp(-0.5 * (X ** 2 + Y ** 2) / 6.0 ** 2)
+    V = 2e-3 * np.exp(-0.5 * ((X - 3) ** 2 + (Y + 2) ** 2) / 4.0 ** 2)
+    rng = np.random.default_rng(Nc + Nr + tetm)
+    init = {"psi_plus": 0.05 * (rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc))),
+            "n_plus": 0.1 * rng.random((Nr, Nc)), "pump_plus": P, "potential_plus": V}
+    if tetm:
+        init.update(psi_minus=0.05 * (rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc))),
+                    n_minus=0.1 * rng.random((Nr, Nc)), pump_minus=0.6 * P, potential_minus=V[::-1].copy())
+    args = ["--N", Nc, Nr, "--L", L, Ly, "--boundary", "periodic", "periodic", "--outEvery", 1e9, "--fftEvery", 1e9,
+            "--pump", 1.0, "add", 10, 10, 0, 0, "both", 1, "none", "gauss+noDivide",
+            "--potential", 1.0, "add", 10, 10, 0, 0, "both", 1, "none", "gauss+noDivide",
+            "-ssfm" if noise else "--iterator", *([] if noise else ["SSFM"]), "--tstep", dt, "--tmax", (steps - 0.5) * dt]
+    if tetm:
+        args += ["-tetm"]
+    if noise:
+        args += ["--dw", 0.05, "--initRandom", 0.0, seed]
+    s = Solver([str(a) for a in args], prec)
+    for k, v in init.items():
+        s.set(k, v)
+    done = s.step(steps + 5)
+    out = {k: s.get(k).astype(np.complex128 if k.startswith("psi") else np.float64) for k in init
+           if k.startswith(("psi", "n_"))}
+    info = s.info
+    s.close()
+    assert done == steps and info["iterator"] == "SSFM"
+    p = O.params(Nc, Nr, L, Ly, tetm=tetm, pump=True, potential=True, stochastic=0.05 if noise else 0.0,
+                 noise_seed=seed)
+    kw = dict(pump=P, pot=V)
+    if tetm:
+        kw.update(psi_minus=init["psi_minus"], n_minus=init["n_minus"], pump_minus=init["pump_minus"],
+                  pot_minus=init["potential_minus"])
+    o = O.ssfm_run(p, init["psi_plus"], init["n_plus"], dt=dt, steps=steps, **kw)
+    ref = dict(zip(["psi_plus", "n_plus", "psi_minus", "n_minus"], o))
+    errs = {k: rel_l2(out[k], ref[k]) for k in out}
+    say("  " + " ".join(f"{k} {e:.2e}" for k, e in errs.items()))
+    assert max(errs.values()) <= TOL[prec]
