@@ -335,18 +335,26 @@ def time_adjoint(N: int = 512, steps: int = 2000, K: int = 50, batch: int = 16, 
             "gradient_over_forward": round((ms_f + ms_b) / ms_plain, 3), "finite": finite}
 
 
-def cpu_baseline(N: int, L: float, steps: int):
-    """The FP64 oracle (single-threaded C restatement of the reference CPU path) on a bounded sample."""
+def cpu_baseline(N: int, L: float, seconds: float = 10.0, subgrids: tuple = (4, 4)):
+    """The FP64 oracle (C restatement of the reference CPU path: padded subgrids with halos, OpenMP loop
+    over subgrids as in the reference CPU build) on the host cores, on a bounded sample: a probe of 10
+    steps sets the step count for about `seconds` of work (20 to 2000 steps)."""
     from oracle import oracle as O
-    p = O.params(N, N, L, L, periodic=(True, True), pump=True)
+    cores = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    threads = O.set_threads(min(cores, int(omp)) if omp.isdigit() and int(omp) > 0 else cores)
+    p = O.params(N, N, L, L, periodic=(True, True), pump=True, subgrids=subgrids)
     psi, n, P = workload_inputs(N, L)
-    O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=1)  # warm-up (page-in)
+    t0 = time.perf_counter()
+    O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=10)  # probe (also pages the workspace in)
+    probe = (time.perf_counter() - t0) / 10
+    steps = int(min(2000, max(20, seconds / max(probe, 1e-6))))
     t0 = time.perf_counter()
     O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=steps)
     dt = time.perf_counter() - t0
-    return dict(value=1e6 * dt / steps, unit="µs/step", cores=1, kind="port",
-                sample=f"FP64 oracle (C, 1 thread, reference subgrid+halo structure), {N}x{N}, {steps} RK4 steps, "
-                       f"{dt:.1f} s")
+    return dict(value=round(1e6 * dt / steps, 1), unit="µs/step", cores=threads, kind="port",
+                sample=f"FP64 oracle (C, OpenMP over {subgrids[0]}x{subgrids[1]} subgrids with 4-cell halos, "
+                       f"{threads} threads on {cores} host cores), {N}x{N}, {steps} RK4 steps, {dt:.1f} s")
 
 
 def main():
@@ -355,7 +363,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None, help="time one precision only (profiling)")
     ap.add_argument("--kernel-samples", type=int, default=200, help="single-step launches timed with event pairs")
@@ -440,7 +448,7 @@ def main():
     if not a.no_spinor and a.only is None:
         out["spinor_dp45"] = time_spinor()
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(N, L, a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(N, L, a.cpu_seconds)
     print(json.dumps(out), flush=True)
 
 

@@ -57,6 +57,8 @@ def lib():
         dp = C.POINTER(C.c_double)
         P = C.POINTER(OrcParams)
         L.orc_defaults.argtypes = [P]
+        L.orc_set_threads.argtypes = [C.c_int]
+        L.orc_set_threads.restype = C.c_int
         L.orc_derive.argtypes = [P] + [C.c_double] * 7
         L.orc_derive.restype = C.c_double
         L.orc_rk4_run.argtypes = [P, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_int, C.c_int64]
@@ -84,6 +86,11 @@ def lib():
         L.orc_iter_adaptive_run.restype = C.c_int64
         _lib = L
     return _lib
+
+
+def set_threads(n: int = 0) -> int:
+    """Threads of the oracle's OpenMP subgrid loops (n <= 0: query); returns the count in use."""
+    return int(lib().orc_set_threads(int(n)))
 
 
 # reference defaults: system.cpp:20-23 (SI constants)

@@ -8,16 +8,36 @@
  * Every function cites the reference unit it restates.  The structure is the
  * reference CPU path: padded subgrids with halo = k_max, one halo sync per
  * step, stage s on extent (halo - s), full-region final sum.
+ *
+ * Threads: like the reference CPU build, which runs each step's per-subgrid
+ * work in an OpenMP loop over subgrids (cuda_macro.cuh:435), the per-subgrid
+ * loops here are OpenMP loops over subgrids.  Every subgrid writes only its
+ * own padded cells, so results do not depend on the thread count; the error
+ * and norm reductions stay serial (fixed order).
  */
 #include "phx_oracle.h"
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 /* ------------------------------------------------------------------------ */
 /* parameters                                                               */
 /* ------------------------------------------------------------------------ */
+
+/* Thread count of the OpenMP subgrid loops (1 without OpenMP); n <= 0 keeps the current one. */
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
 
 /* SystemParameters::SystemParameters() defaults (system.cpp:18-72). */
 void orc_defaults(orc_params* p) {
@@ -201,6 +221,7 @@ static long global_cell(const ws_t* w, int g, int tr, int tc) {
 static void sync_halos(const ws_t* w, double* f, int elem) {
     const orc_params* p = w->p;
     const int h = w->h, sgc = w->sgc, sgr = w->sgr;
+    #pragma omp parallel for schedule(static)
     for (int g = 0; g < w->S; g++) {
         int R = g / p->subgrids_columns, C = g % p->subgrids_columns;
         for (int tr = 0; tr < sgr + 2 * h; tr++)
@@ -238,6 +259,7 @@ static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
     const double pamp = p->pump_amp * (1.0 - wt) + wt * p->pump_amp_next;
     const double vamp = p->pot_amp * (1.0 - wt) + wt * p->pot_amp_next;
     const int stoch = p->use_stochastic && p->stochastic_amplitude > 0.0;
+    #pragma omp parallel for schedule(static)
     for (int g = 0; g < w->S; g++) {
         size_t base = (size_t)g * w->N2h;
         size_t kbase = ((size_t)(index - 1) * w->S + g) * w->N2h;
@@ -382,6 +404,7 @@ static void sum_to_buffer(ws_t* w, int index, int nw, const double* wts) {
     double wsum = 0.0;
     for (int j = 0; j < nw; j++) wsum += wts[j];
     const double dt = w->dt;
+    #pragma omp parallel for schedule(static)
     for (int g = 0; g < w->S; g++)
         for (int r = w->h - ch; r < w->h + w->sgr + ch; r++)
             for (int c = w->h - ch; c < w->h + w->sgc + ch; c++) {
@@ -413,6 +436,7 @@ static void final_sum(ws_t* w, int index, int nw, const double* wts) {
     double wsum = 0.0;
     for (int j = 0; j < nw; j++) wsum += wts[j];
     const double dt = w->dt;
+    #pragma omp parallel for schedule(static)
     for (int g = 0; g < w->S; g++)
         for (int r = w->h - ch; r < w->h + w->sgr + ch; r++)
             for (int c = w->h - ch; c < w->h + w->sgc + ch; c++) {
@@ -454,6 +478,7 @@ static void gen_noise(ws_t* w, int64_t step, double dt) {
     const orc_params* p = w->p;
     if (!(p->use_stochastic && p->stochastic_amplitude > 0.0)) return;
     const double a = p->stochastic_amplitude * sqrt(dt);
+    #pragma omp parallel for schedule(static)
     for (int g = 0; g < w->S; g++)
         for (int tr = 0; tr < w->sgr + 2 * w->h; tr++)
             for (int tc = 0; tc < w->sgc + 2 * w->h; tc++) {

@@ -24,6 +24,9 @@
 extern "C" {
 #endif
 
+/* Threads of the OpenMP subgrid loops; n <= 0 only queries.  Returns the count in use. */
+int orc_set_threads(int n);
+
 typedef struct {
     /* geometry */
     int N_c, N_r;
