@@ -32,18 +32,27 @@ __device__ __forceinline__ int xcd_contiguous(int b, int nblocks) {
     return start + (b >> 3);
 }
 
-/* Row index of a tile row: inside the grid, wrapped once (periodic; a grid shorter
-   than the tile's reach falls back to a full modulo), or outside (zero boundary). */
-__device__ __forceinline__ int tile_row(int g, int Nr, int periodic, bool& in) {
-    in = true;
-    if (g >= 0 && g < Nr) return g;
-    if (!periodic) {
-        in = false;
-        return 0;
-    }
-    g += g < 0 ? Nr : -Nr;
-    if (g < 0 || g >= Nr) g = ((g % Nr) + Nr) % Nr;
-    return g;
+/* Periodic row wrap without branches, so that a fully unrolled row loop stays one
+   basic block (a branch there splits it, and DPP moves then no longer fold into the
+   adds that consume them).  x mod Nr for x = g + bias >= 0 is x - Nr * q with
+   q = mulhi(x, ceil(2^32 / Nr)), exact whenever x * Nr < 2^32; g >= -halo, and
+   bias = Nr (Nr >= halo) or Nr * halo keeps x below 3 Nr + the tile height. */
+struct RowWrap {
+    uint32_t nr, magic, bias;
+};
+__device__ __forceinline__ RowWrap make_row_wrap(int Nr, int halo) {
+    RowWrap w;
+    w.nr = (uint32_t)Nr;
+    w.magic = 0xFFFFFFFFu / (uint32_t)Nr + 1u; /* ceil(2^32 / Nr) for Nr >= 2 */
+    w.bias = Nr >= halo ? (uint32_t)Nr : (uint32_t)(Nr * halo);
+    return w;
+}
+/* Row of tile row g: wrapped into [0, Nr) (exact modulo for periodic grids; for a
+   zero boundary `in` says whether g lies in the grid, the wrapped row is then unused). */
+__device__ __forceinline__ uint32_t tile_row(int g, const RowWrap& w, int periodic, bool& in) {
+    in = periodic || (g >= 0 && g < (int)w.nr);
+    const uint32_t x = (uint32_t)g + w.bias;
+    return x - __umulhi(x, w.magic) * w.nr;
 }
 
 template <int CTRL>

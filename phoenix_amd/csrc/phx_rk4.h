@@ -1,0 +1,350 @@
+/*
+ * phx_rk4.h -- fused RK4 step of the scalar GP + reservoir model for gfx950.
+ *
+ * One launch = one full RungeKutta4::step (runge_kutta.cu:74-94): halo sync,
+ * K1..K4 on shrinking extents, three intermediate sums and the final sum,
+ * which the reference runs as 2 + 12*S separate kernels over global memory
+ * (cuda_macro.cuh:306-353).  Here every intermediate value stays in registers.
+ *
+ * Mapping (one wavefront = one tile, no inter-wave synchronisation):
+ *   - a wave owns a strip of 64 extended columns (56 output columns plus a
+ *     4-cell halo either side), one lane per column;
+ *   - it streams TY+8 extended rows top to bottom; after row i is available it
+ *     evaluates stage 1 at row i-1, stage 2 at i-2, stage 3 at i-3 and stage 4
+ *     at i-4 and writes output row i-4.  Stage s is therefore evaluated on rows
+ *     [s, TY+7-s] and is valid on columns [s, 63-s]: the extents that
+ *     GENERATE_SUBGRID_INDEX gives with current_halo = 4 - s
+ *     (cuda_macro.cuh:43-49,112-113);
+ *   - north/south neighbours come from the register window of the previous
+ *     stage, east/west neighbours from the adjacent lanes through DPP wave
+ *     shifts (one VALU op, no LDS round trip);
+ *   - rows are read with raw buffer loads PF rows ahead of their first use.
+ *     Each row gets its own buffer descriptor (scalar registers); a row outside
+ *     a zero-boundary domain gets an empty descriptor and a column outside it an
+ *     out-of-range offset, so the hardware returns 0 for psi, n, P, V -- which is
+ *     what synchronize_halos writes into the reference's halos
+ *     (kernel_halo.cuh:66-69).  Periodic boundaries wrap the row / column index.
+ * Arithmetic uses explicit fused multiply-adds and no automatic contraction, so
+ * a cell's result does not depend on the tile shape or its place in the tile.
+ * HBM traffic per step and cell: read psi, n, P (, V) once, write psi, n once;
+ * the 8-row / 8-column halo re-reads are served from L2.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "custom_rhs.h"
+#include "phx_device.h"
+#include "phx_kernels.h"
+
+namespace phx {
+
+constexpr int kHalo = 4;
+constexpr int kOutW = kWave - 2 * kHalo; /* 56 */
+
+/* Neighbour part of the stencil: kin_x (W + (E + r (N+S))) with r = kin_y / kin_x,
+   both east/west moves folded into DPP adds.  ISO (dx == dy, r == 1 exactly) drops
+   the multiply by r; the two forms give identical bits for dx == dy, so a grid gets
+   the same result from either kernel (an ensemble mixing both uses the general one). */
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool PERY>
+__device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int band) {
+    using v2 = typename Vec2<real>::type;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int x0 = strip * kOutW;
+    const int y0 = band * TY;
+    const PhxPhys<real>& ph = a.ph;
+    const int Nc = a.Nc, Nr = a.Nr;
+
+    /* column of this lane (wrapped for periodic x); out-of-range offset outside a zero-BC domain */
+    int gcol = x0 - kHalo + lane;
+    bool col_in = true;
+    if (gcol < 0 || gcol >= Nc) {
+        if (a.periodic_x)
+            gcol = ((gcol % Nc) + Nc) % Nc;
+        else
+            col_in = false;
+    }
+    const uint32_t off_r = col_in ? (uint32_t)gcol * (uint32_t)sizeof(real) : kOutOfRange;
+    const uint32_t off_c = col_in ? (uint32_t)gcol * (uint32_t)sizeof(v2) : kOutOfRange;
+    /* store offsets: halo lanes and lanes past the last column write nothing */
+    const bool out_lane = lane >= kHalo && lane < kWave - kHalo && (x0 - kHalo + lane) < Nc;
+    const uint32_t so_r = out_lane ? (uint32_t)(x0 - kHalo + lane) * (uint32_t)sizeof(real) : kOutOfRange;
+    const uint32_t so_c = out_lane ? (uint32_t)(x0 - kHalo + lane) * (uint32_t)sizeof(v2) : kOutOfRange;
+    /* one descriptor per array; a row is an SGPR offset.  Rows outside a zero-BC
+       domain read through a descriptor of size 0 (zeros). */
+    const uint32_t row_r = (uint32_t)Nc * (uint32_t)sizeof(real);
+    const uint32_t row_c = (uint32_t)Nc * (uint32_t)sizeof(v2);
+    const uint32_t all_r = row_r * (uint32_t)Nr, all_c = row_c * (uint32_t)Nr;
+    const rsrc_t w_psi = row_rsrc(a.psi_out, all_c), w_n = row_rsrc(a.n_out, all_r);
+    /* the same arrays with no records: output rows past the grid end store through them (dropped) */
+    const rsrc_t w_psi_none = row_rsrc(a.psi_out, 0u), w_n_none = row_rsrc(a.n_out, 0u);
+    const RowWrap wrap = make_row_wrap(Nr, kHalo);
+
+    constexpr int E = TY + 2 * kHalo;
+    real p0r[E], p0i[E], n0[E], pp[E], vv[E];
+    real y1r[E], y1i[E], y2r[E], y2i[E], y3r[E], y3i[E];
+    real yn1[E], yn2[E], yn3[E];
+    real acr[E], aci[E], acn[E]; /* psi0 + dt/6 k1 + dt/3 k2 + dt/3 k3, built stage by stage */
+
+    const real dth = a.dt * real(0.5);
+    const real dt1 = a.dt;
+    const real dt6 = a.dt / real(6);
+    const real dt3 = a.dt / real(3);
+    PhxPhys<real> phv = ph;
+    phv.mhalf_gamma_c = in_vgpr(ph.mhalf_gamma_c);
+    phv.gamma_r = in_vgpr(ph.gamma_r);
+    const real kin_x = ph.kin_x, kin_yx = ph.kin_yx;
+
+    auto load_row = [&](int r) {
+        bool in;
+        const uint32_t grow = tile_row(y0 - kHalo + r, wrap, PERY, in); /* wave-uniform */
+        const uint32_t sc = grow * row_c, sr = grow * row_r;
+        const uint32_t nc = in ? all_c : 0u, nr = in ? all_r : 0u;
+        const auto x = load_cplx_s<real>(row_rsrc(a.psi_in, nc), off_c, sc);
+        p0r[r] = x.x;
+        p0i[r] = x.y;
+        if (RES) n0[r] = load_real_s<real>(row_rsrc(a.n_in, nr), off_r, sr);
+        if (PUMP) pp[r] = load_real_s<real>(row_rsrc(a.pump, nr), off_r, sr);
+        if (POT) vv[r] = load_real_s<real>(row_rsrc(a.pot, nr), off_r, sr);
+    };
+
+    /* neighbour part of the kinetic stencil: centre c, north+south ns */
+    auto nb = [&](real c, real ns) {
+        if constexpr (ISO)
+            return kin_x * (west(c) + (east(c) + ns));
+        else
+            return kin_x * (west(c) + (east(c) + kin_yx * ns));
+    };
+    /* one RHS evaluation: centre (cr, ci), north+south sums, reservoir, pump*amp, V*amp/hbar */
+    auto rhs = [&](real cr, real ci, real nsr, real nsi, real nv, real pmp, real poth, real& kr, real& ki, real& kn) {
+        phx_rhs_scalar<real>(phv, cr, ci, nb(cr, nsr), nb(ci, nsi), RES ? nv : real(0), PUMP ? pmp : real(0), POT ? poth : real(0),
+                             kr, ki, kn);
+    };
+
+#pragma unroll
+    for (int it = -PF; it < E; ++it) {
+        if (it + PF < E) load_row(it + PF);
+        if (it < 0) continue;
+        /* ---- stage 1 at row e = it-1, input psi0 */
+        {
+            const int e = it - 1;
+            if (e >= 1 && e <= E - 2) {
+                real kr, ki, kn;
+                rhs(p0r[e], p0i[e], p0r[e - 1] + p0r[e + 1], p0i[e - 1] + p0i[e + 1], n0[e], pp[e] * a.pamp[0],
+                    vv[e] * a.vamp_h[0], kr, ki, kn);
+                y1r[e] = phx_fma(dth, kr, p0r[e]);
+                y1i[e] = phx_fma(dth, ki, p0i[e]);
+                acr[e] = phx_fma(dt6, kr, p0r[e]);
+                aci[e] = phx_fma(dt6, ki, p0i[e]);
+                if (RES) {
+                    yn1[e] = phx_fma(dth, kn, n0[e]);
+                    acn[e] = phx_fma(dt6, kn, n0[e]);
+                }
+            }
+        }
+        /* ---- stage 2 at row e = it-2, input y1 */
+        {
+            const int e = it - 2;
+            if (e >= 2 && e <= E - 3) {
+                real kr, ki, kn;
+                rhs(y1r[e], y1i[e], y1r[e - 1] + y1r[e + 1], y1i[e - 1] + y1i[e + 1], RES ? yn1[e] : real(0),
+                    pp[e] * a.pamp[1], vv[e] * a.vamp_h[1], kr, ki, kn);
+                y2r[e] = phx_fma(dth, kr, p0r[e]);
+                y2i[e] = phx_fma(dth, ki, p0i[e]);
+                acr[e] = phx_fma(dt3, kr, acr[e]);
+                aci[e] = phx_fma(dt3, ki, aci[e]);
+                if (RES) {
+                    yn2[e] = phx_fma(dth, kn, n0[e]);
+                    acn[e] = phx_fma(dt3, kn, acn[e]);
+                }
+            }
+        }
+        /* ---- stage 3 at row e = it-3, input y2 */
+        {
+            const int e = it - 3;
+            if (e >= 3 && e <= E - 4) {
+                real kr, ki, kn;
+                rhs(y2r[e], y2i[e], y2r[e - 1] + y2r[e + 1], y2i[e - 1] + y2i[e + 1], RES ? yn2[e] : real(0),
+                    pp[e] * a.pamp[2], vv[e] * a.vamp_h[2], kr, ki, kn);
+                y3r[e] = phx_fma(dt1, kr, p0r[e]);
+                y3i[e] = phx_fma(dt1, ki, p0i[e]);
+                acr[e] = phx_fma(dt3, kr, acr[e]);
+                aci[e] = phx_fma(dt3, ki, aci[e]);
+                if (RES) {
+                    yn3[e] = phx_fma(dt1, kn, n0[e]);
+                    acn[e] = phx_fma(dt3, kn, acn[e]);
+                }
+            }
+        }
+        /* ---- stage 4 at row e = it-4, input y3; psi0 + dt/6 (k1 + 2k2 + 2k3 + k4) */
+        {
+            const int e = it - 4;
+            if (e >= 4 && e <= E - 5) {
+                real kr, ki, kn;
+                rhs(y3r[e], y3i[e], y3r[e - 1] + y3r[e + 1], y3i[e - 1] + y3i[e + 1], RES ? yn3[e] : real(0),
+                    pp[e] * a.pamp[3], vv[e] * a.vamp_h[3], kr, ki, kn);
+                /* rows past the last grid row store through an empty descriptor: no branch */
+                const int grow = y0 + e - kHalo;
+                const bool row_ok = grow < Nr;
+                v2 o;
+                o.x = phx_fma(dt6, kr, acr[e]);
+                o.y = phx_fma(dt6, ki, aci[e]);
+                store_cplx_s(row_ok ? w_psi : w_psi_none, so_c, (uint32_t)grow * row_c, o);
+                if (RES) store_real_s(row_ok ? w_n : w_n_none, so_r, (uint32_t)grow * row_r, phx_fma(dt6, kn, acn[e]));
+            }
+        }
+    }
+}
+
+/* one grid: tile (strip, band) from the block index, 4 vertically stacked bands per block */
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool PERY>
+__global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int strip = tile % a.nstrips;
+    const int band = (tile / a.nstrips) * kWavesPerBlock + wave;
+    if (band >= a.nbands) return;
+    rk4_tile<real, TY, ISO, PF, RES, PUMP, POT, PERY>(a, strip, band);
+}
+
+/* ensemble: one flattened list of wavefront tiles over all instances; each wave looks up
+   its instance and steps one tile of it (instances of any size in the same launch).
+   Without an instance table: `items` grids of the common size at a fixed element stride. */
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool PERY>
+__global__ __launch_bounds__(256) void rk4_batched_fused(Rk4BatchArgs<real> b) {
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int gw = xcd_contiguous(blockIdx.x, gridDim.x) * kWavesPerBlock + wave;
+    if (gw >= b.ntiles) return;
+    Rk4Args<real> a = b.common;
+    int local;
+    if (b.entries) {
+        const int inst = __builtin_amdgcn_readfirstlane(b.tile_inst[gw]);
+        const Rk4BatchEntry<real>& en = b.entries[inst];
+        a.psi_in = en.psi_in;
+        a.n_in = en.n_in;
+        a.psi_out = en.psi_out;
+        a.n_out = en.n_out;
+        a.pump = en.pump;
+        a.pot = en.pot;
+        a.Nc = en.Nc;
+        a.Nr = en.Nr;
+        a.nstrips = en.nstrips;
+        a.ph.kin_c = en.kin_c;
+        a.ph.kin_x = en.kin_x;
+        a.ph.kin_y = en.kin_y;
+        a.ph.kin_yx = en.kin_yx;
+        local = gw - en.tile0;
+    } else {
+        const int inst = gw / b.tiles_per_item;
+        const size_t so = (size_t)inst * b.item_stride, po = (size_t)inst * b.param_stride;
+        a.psi_in += so;
+        a.psi_out += so;
+        if (RES) {
+            a.n_in += so;
+            a.n_out += so;
+        }
+        if (PUMP) a.pump += po;
+        if (POT) a.pot += po;
+        a.nstrips = rk4_strips(a.Nc);
+        local = gw - inst * b.tiles_per_item;
+    }
+    rk4_tile<real, TY, ISO, PF, RES, PUMP, POT, PERY>(a, local % a.nstrips, local / a.nstrips);
+}
+
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT>
+inline hipError_t launch_ty(const Rk4Args<real>& a0, hipStream_t s) {
+    Rk4Args<real> a = a0;
+    a.nstrips = (a.Nc + kOutW - 1) / kOutW;
+    a.nbands = (a.Nr + TY - 1) / TY;
+    const int bandgroups = (a.nbands + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int nblocks = a.nstrips * bandgroups;
+    /* periodic rows: a separate kernel, whose row descriptors are loop invariant (fewer
+       scalar registers than the zero-boundary form, which selects an empty descriptor for
+       rows outside the grid) */
+    if (a.periodic_y)
+        hipLaunchKernelGGL((rk4_scalar_fused<real, TY, ISO, PF, RES, PUMP, POT, true>), dim3(nblocks),
+                           dim3(kWave * kWavesPerBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((rk4_scalar_fused<real, TY, ISO, PF, RES, PUMP, POT, false>), dim3(nblocks),
+                           dim3(kWave * kWavesPerBlock), 0, s, a);
+    return hipGetLastError();
+}
+
+/* Tile height TY in {8, 10, 12, 16, 20, 24} (others: 16). */
+template <typename real, bool ISO, int PF, bool RES, bool PUMP, bool POT>
+inline hipError_t launch_tys(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
+    switch (t.ty) {
+        case 8: return launch_ty<real, 8, ISO, PF, RES, PUMP, POT>(a, s);
+        case 10: return launch_ty<real, 10, ISO, PF, RES, PUMP, POT>(a, s);
+        case 12: return launch_ty<real, 12, ISO, PF, RES, PUMP, POT>(a, s);
+        case 20: return launch_ty<real, 20, ISO, PF, RES, PUMP, POT>(a, s);
+        case 24: return launch_ty<real, 24, ISO, PF, RES, PUMP, POT>(a, s);
+        default: return launch_ty<real, 16, ISO, PF, RES, PUMP, POT>(a, s);
+    }
+}
+
+/* Rows loaded ahead: 2, or 3 / 4 for the pumped reservoir model with dx == dy
+   (the benchmark configuration; chosen through Rk4Tune::pf). */
+template <typename real, bool RES, bool PUMP, bool POT>
+inline hipError_t launch_flags(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
+    const bool iso = a.ph.kin_yx == real(1);
+    if constexpr (RES && PUMP && !POT) {
+        if (iso && t.pf == 3) return launch_tys<real, true, 3, RES, PUMP, POT>(a, t, s);
+        if (iso && t.pf >= 4) return launch_tys<real, true, 4, RES, PUMP, POT>(a, t, s);
+    }
+    return iso ? launch_tys<real, true, 2, RES, PUMP, POT>(a, t, s) : launch_tys<real, false, 2, RES, PUMP, POT>(a, t, s);
+}
+
+template <typename real>
+hipError_t launch_rk4_scalar(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
+    if ((size_t)a.Nc * (size_t)a.Nr * sizeof(typename Vec2<real>::type) > kMaxArrayBytes) return hipErrorInvalidValue;
+    const bool res = a.n_in != nullptr;
+    const bool pump = res && a.pump != nullptr;
+    const bool pot = a.pot != nullptr;
+    if (!res) return pot ? launch_flags<real, false, false, true>(a, t, s) : launch_flags<real, false, false, false>(a, t, s);
+    if (pump) return pot ? launch_flags<real, true, true, true>(a, t, s) : launch_flags<real, true, true, false>(a, t, s);
+    return pot ? launch_flags<real, true, false, true>(a, t, s) : launch_flags<real, true, false, false>(a, t, s);
+}
+
+
+/* ---------------------------------------------------------------- ensembles */
+template <typename real, int TY, bool ISO, bool RES, bool PUMP, bool POT>
+inline hipError_t launch_batch_ty(const Rk4BatchArgs<real>& b, hipStream_t s) {
+    const int nblocks = (b.ntiles + kWavesPerBlock - 1) / kWavesPerBlock;
+    if (b.common.periodic_y)
+        hipLaunchKernelGGL((rk4_batched_fused<real, TY, ISO, 2, RES, PUMP, POT, true>), dim3(nblocks),
+                           dim3(kWave * kWavesPerBlock), 0, s, b);
+    else
+        hipLaunchKernelGGL((rk4_batched_fused<real, TY, ISO, 2, RES, PUMP, POT, false>), dim3(nblocks),
+                           dim3(kWave * kWavesPerBlock), 0, s, b);
+    return hipGetLastError();
+}
+
+template <typename real, bool ISO, bool RES, bool PUMP, bool POT>
+inline hipError_t launch_batch_tys(const Rk4BatchArgs<real>& b, int ty, hipStream_t s) {
+    switch (ty) {
+        case 8: return launch_batch_ty<real, 8, ISO, RES, PUMP, POT>(b, s);
+        case 10: return launch_batch_ty<real, 10, ISO, RES, PUMP, POT>(b, s);
+        case 16: return launch_batch_ty<real, 16, ISO, RES, PUMP, POT>(b, s);
+        case 20: return launch_batch_ty<real, 20, ISO, RES, PUMP, POT>(b, s);
+        default: return launch_batch_ty<real, 12, ISO, RES, PUMP, POT>(b, s);
+    }
+}
+
+template <typename real, bool RES, bool PUMP, bool POT>
+inline hipError_t launch_batch_flags(const Rk4BatchArgs<real>& b, int ty, hipStream_t s) {
+    return b.iso ? launch_batch_tys<real, true, RES, PUMP, POT>(b, ty, s) : launch_batch_tys<real, false, RES, PUMP, POT>(b, ty, s);
+}
+
+template <typename real>
+hipError_t launch_rk4_batched(const Rk4BatchArgs<real>& b, int ty, bool res, bool pump, bool pot, hipStream_t s) {
+    /* (instance sizes are checked when the ensemble is created) */
+    if (!b.entries && (size_t)b.common.Nc * (size_t)b.common.Nr * sizeof(typename Vec2<real>::type) > kMaxArrayBytes)
+        return hipErrorInvalidValue;
+    if (!res) return pot ? launch_batch_flags<real, false, false, true>(b, ty, s) : launch_batch_flags<real, false, false, false>(b, ty, s);
+    if (pump) return pot ? launch_batch_flags<real, true, true, true>(b, ty, s) : launch_batch_flags<real, true, true, false>(b, ty, s);
+    return pot ? launch_batch_flags<real, true, false, true>(b, ty, s) : launch_batch_flags<real, true, false, false>(b, ty, s);
+}
+
+
+} // namespace phx

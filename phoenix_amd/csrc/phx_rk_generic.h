@@ -85,6 +85,7 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
     const uint32_t all_r = row_r * (uint32_t)Nr, all_c = row_c * (uint32_t)Nr;
     rsrc_t d_psi[NC], d_n[NC], d_p[NC], d_v[NC], o_psi[NC], o_n[NC];
     const rsrc_t d_xi = row_rsrc(a.noise, STOCH ? all_c : 0u);
+    const RowWrap wrap = make_row_wrap(Nr, H);
 #pragma unroll
     for (int k = 0; k < NC; k++) {
         d_psi[k] = row_rsrc(a.psi_in[k], all_c);
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
 
     auto load_row = [&](int r) {
         bool in;
-        const uint32_t grow = (uint32_t)tile_row(y0 - H + r, Nr, a.periodic_y, in); /* wave-uniform */
+        const uint32_t grow = tile_row(y0 - H + r, wrap, a.periodic_y, in); /* wave-uniform */
         const uint32_t vc = in ? off_c + grow * row_c : kOutOfRange;
         const uint32_t vr = in ? off_r + grow * row_r : kOutOfRange;
 #pragma unroll
@@ -268,13 +269,13 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
                 out0r = o_r;
                 out0i = o_i;
             }
-            if (grow < Nr) {
-                v2 o;
-                o.x = o_r;
-                o.y = o_i;
-                store_cplx_s(o_psi[k], so_c + (uint32_t)grow * row_c, 0u, o);
-                if (RES) store_real_s(o_n[k], so_r + (uint32_t)grow * row_r, 0u, phx_fma(dt, sn, n0[k][e]));
-            }
+            /* a row past the grid end has an offset beyond the array (launch_gen checks that
+               it does not wrap): the store is dropped, no branch */
+            v2 o;
+            o.x = o_r;
+            o.y = o_i;
+            store_cplx_s(o_psi[k], so_c + (uint32_t)grow * row_c, 0u, o);
+            if (RES) store_real_s(o_n[k], so_r + (uint32_t)grow * row_r, 0u, phx_fma(dt, sn, n0[k][e]));
         }
         if constexpr (TAB::ERR) {
             /* runge_sum_to_error (kernel_summation.cuh:270-289): |dt sum_j e_j K_j(psi+)|^2;
@@ -288,10 +289,9 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
             }
             const double er = (double)(dt * sr), ei = (double)(dt * si);
             const double ee = er * er + ei * ei;
-            if (grow < Nr) {
-                err_acc += out_lane ? (TETM ? 2.0 * ee : ee) : 0.0;
-                nrm_acc += out_lane ? (double)out0r * (double)out0r + (double)out0i * (double)out0i : 0.0;
-            }
+            const bool cell = out_lane && grow < Nr;
+            err_acc += cell ? (TETM ? 2.0 * ee : ee) : 0.0;
+            nrm_acc += cell ? (double)out0r * (double)out0r + (double)out0i * (double)out0i : 0.0;
         }
     };
 
@@ -323,7 +323,9 @@ template <typename real, typename TAB, int TY, bool TETM, bool RES, bool STOCH>
 static hipError_t launch_gen(const RkGenArgs<real>& a0, hipStream_t s) {
     RkGenArgs<real> a = a0;
     /* whole-array descriptors: every field must stay below the 32-bit offset range */
-    if ((size_t)a.Nc * (size_t)a.Nr * sizeof(typename Vec2<real>::type) > kMaxArrayBytes) return hipErrorInvalidValue;
+    /* (rows up to TY + S past the grid end are addressed, and dropped, by the stores) */
+    if ((size_t)a.Nc * (size_t)(a.Nr + TY + TAB::S) * sizeof(typename Vec2<real>::type) > kMaxArrayBytes)
+        return hipErrorInvalidValue;
     const int outw = kWave - 2 * TAB::S;
     a.nstrips = (a.Nc + outw - 1) / outw;
     a.nbands = (a.Nr + TY - 1) / TY;

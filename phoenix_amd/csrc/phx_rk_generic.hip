@@ -1,7 +1,7 @@
 /*
  * phx_rk_generic.hip -- instantiations of the fused Runge-Kutta kernel
  * (phx_rk_generic.h) for RK4 (with noise or TE-TM; the plain scalar RK4 has its own
- * hand-tuned kernel in phx_rk4.hip) and Dormand-Prince 5(4) fixed and adaptive, the
+ * hand-tuned kernel in phx_rk4.h) and Dormand-Prince 5(4) fixed and adaptive, the
  * deterministic reduction of the adaptive error partials, the per-step noise field, the
  * imaginary-time normalisation, and the dispatch over all tableaux (the other groups are instantiated in phx_rk_tab3.hip,
  * phx_rk_tab4.hip, phx_rk_tab6.hip and phx_rk_tab6f.hip).
