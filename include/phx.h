@@ -73,6 +73,8 @@ typedef struct {
     int64_t rejected;           /* rejected adaptive attempts so far */
     int32_t tile_rows;          /* output rows per wavefront tile of the fused RK4 kernel (0 before the first step) */
     int32_t prefetch_rows;      /* rows it loads ahead of use */
+    int32_t staged;             /* 1: the stage-by-stage path runs the step (K's in HBM: pulses, several
+                                   non-constant temporal groups, complex potential envelopes, DP85, NSRK78) */
 } phx_info;
 
 /* Parse PHOENIX flags (argv[0] is the program name and is skipped), including

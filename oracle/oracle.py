@@ -19,6 +19,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libphx_oracle.so")
 
 
+MAX_GROUPS = 8  # ORC_MAX_GROUPS
+
+
 class OrcParams(C.Structure):
     _fields_ = [
         ("N_c", C.c_int), ("N_r", C.c_int),
@@ -36,6 +39,9 @@ class OrcParams(C.Structure):
         ("noise_seed", C.c_uint64),
         ("imag_time_amplitude", C.c_double),
         ("L_x", C.c_double), ("L_y", C.c_double),
+        ("src_groups", C.c_int * 3),
+        ("src_field", C.c_void_p * 2 * MAX_GROUPS * 3),
+        ("src_amp", C.c_double * 2 * MAX_GROUPS * 3), ("src_amp_next", C.c_double * 2 * MAX_GROUPS * 3),
     ]
 
 
@@ -119,6 +125,31 @@ def params(N_c, N_r, L_x, L_y, *, periodic=(True, True), subgrids=(1, 1), tetm=F
     p.stochastic_amplitude = stochastic
     p.noise_seed = noise_seed
     return p
+
+
+SOURCES = {"pump": 0, "potential": 1, "pulse": 2}
+
+
+def set_chain(p: OrcParams, source: str, groups):
+    """Source chain of `source` ("pump", "potential", "pulse"): one entry per temporal group,
+    (field_plus, field_minus or None, amp, amp_next) with complex temporal amplitudes at t and
+    t + dt; pump and potential fields real (N_r, N_c), pulse fields complex.  The arrays are
+    kept alive on `p`."""
+    q = SOURCES[source]
+    keep = p.__dict__.setdefault("_chain_arrays", {})
+    keep[q] = []
+    p.src_groups[q] = len(groups)
+    for g, (fp, fm, a0, a1) in enumerate(groups):
+        for comp, f in enumerate((fp, fm)):
+            if f is None:
+                p.src_field[q][g][comp] = None
+                continue
+            arr = _c128(f).view(np.float64) if q == 2 else _f64(f)
+            arr = np.ascontiguousarray(arr)
+            keep[q].append(arr)
+            p.src_field[q][g][comp] = arr.ctypes.data
+        p.src_amp[q][g][0], p.src_amp[q][g][1] = complex(a0).real, complex(a0).imag
+        p.src_amp_next[q][g][0], p.src_amp_next[q][g][1] = complex(a1).real, complex(a1).imag
 
 
 def _dp(a):

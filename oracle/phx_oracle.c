@@ -132,6 +132,7 @@ typedef struct {
     double *n[2], *bn[2], *kn[2];           /* real */
     double *pump[2], *pot[2];               /* real */
     double *xi;                             /* complex noise per padded cell, per subgrid */
+    double *src[3][ORC_MAX_GROUPS][2];      /* source chains per group and component (padded) */
     double dt;
 } ws_t;
 
@@ -163,6 +164,11 @@ static int ws_init(ws_t* w, const orc_params* p, int kmax) {
         w->pot[c] = xcalloc(cells, sizeof(double));
     }
     w->xi = xcalloc(2 * cells, sizeof(double));
+    for (int q = 0; q < 3; q++) {
+        if (p->src_groups[q] < 0 || p->src_groups[q] > ORC_MAX_GROUPS) return -1;
+        for (int g = 0; g < p->src_groups[q]; g++)
+            for (int c = 0; c < w->ncomp; c++) w->src[q][g][c] = xcalloc((q == 2 ? 2 : 1) * cells, sizeof(double));
+    }
     return 0;
 }
 
@@ -173,6 +179,9 @@ static void ws_free(ws_t* w) {
         free(w->pump[c]); free(w->pot[c]);
     }
     free(w->xi);
+    for (int q = 0; q < 3; q++)
+        for (int g = 0; g < ORC_MAX_GROUPS; g++)
+            for (int c = 0; c < 2; c++) free(w->src[q][g][c]);
 }
 
 /* full grid -> padded subgrid interiors (kernel_halo.cuh:7-24) */
@@ -251,6 +260,55 @@ static void sync_halos(const ws_t* w, double* f, int elem) {
 #define MI_RE(re, im) (im)
 #define MI_IM(re, im) (-(re))
 
+/* a_g(w) = amp (1 - w) + w amp_next of group g of source q (kernel_gp_compute.cuh:79) */
+static void src_amp(const orc_params* p, int q, int g, double wt, double* re, double* im) {
+    *re = p->src_amp[q][g][0] * (1.0 - wt) + wt * p->src_amp_next[q][g][0];
+    *im = p->src_amp[q][g][1] * (1.0 - wt) + wt * p->src_amp_next[q][g][1];
+}
+
+/* potential chain: wf += (1/hbar) (V_g a_g(w)) (-i psi) for every group, in group order
+   (kernel_gp_compute.cuh:88-93,146-150); (mr, mi) = -i psi */
+static void add_pot_chain(const ws_t* w, int comp, size_t i, double wt, double mr, double mi, double* wr, double* wi) {
+    const orc_params* p = w->p;
+    for (int g = 0; g < p->src_groups[1]; g++) {
+        double ar, ai;
+        src_amp(p, 1, g, wt, &ar, &ai);
+        const double v = w->src[1][g][comp][i];
+        const double tr = p->one_over_h_bar_s * (v * ar), ti = p->one_over_h_bar_s * (v * ai);
+        *wr += tr * mr - ti * mi;
+        *wi += tr * mi + ti * mr;
+    }
+}
+
+/* pulse chain: wf += ((1/hbar) F_g) a_g(w) (kernel_gp_compute.cuh:94-100,160-164) */
+static void add_pulse_chain(const ws_t* w, int comp, size_t i, double wt, double* wr, double* wi) {
+    const orc_params* p = w->p;
+    for (int g = 0; g < p->src_groups[2]; g++) {
+        double ar, ai;
+        src_amp(p, 2, g, wt, &ar, &ai);
+        const double* f = w->src[2][g][comp];
+        const double tr = p->one_over_h_bar_s * f[2 * i], ti = p->one_over_h_bar_s * f[2 * i + 1];
+        *wr += tr * ar - ti * ai;
+        *wi += tr * ai + ti * ar;
+    }
+}
+
+/* pump term of k_n: the single field at its amplitude, or the chain sum_g P_g Re a_g(w)
+   (kernel_gp_compute.cuh:70-75) */
+static double pump_term(const ws_t* w, int comp, size_t i, double wt, double pamp, double kn) {
+    const orc_params* p = w->p;
+    if (p->src_groups[0] > 0) {
+        for (int g = 0; g < p->src_groups[0]; g++) {
+            double ar, ai;
+            src_amp(p, 0, g, wt, &ar, &ai);
+            kn += w->src[0][g][comp][i] * ar;
+        }
+        return kn;
+    }
+    if (p->use_pump) kn += w->pump[comp][i] * pamp;
+    return kn;
+}
+
 static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
     const orc_params* p = w->p;
     const int ch = w->h - index;
@@ -288,15 +346,18 @@ static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
                         wr += 0.5 * p->R * nv * pr;
                         wi += 0.5 * p->R * nv * pi;
                         double kn = -(p->gamma_r + p->R * nrm) * nv;
-                        if (p->use_pump) kn += w->pump[0][i] * pamp;
+                        kn = pump_term(w, 0, i, wt, pamp, kn);
                         if (stoch) kn += p->R * nv / p->dV;
                         w->kn[0][ki] = kn;
                     }
-                    if (p->use_potential) {
+                    if (p->src_groups[1] > 0) {
+                        add_pot_chain(w, 0, i, wt, mr, mi, &wr, &wi);
+                    } else if (p->use_potential) {
                         const double v = w->pot[0][i] * vamp;
                         wr += ohb * v * mr;
                         wi += ohb * v * mi;
                     }
+                    add_pulse_chain(w, 0, i, wt, &wr, &wi);
                     if (stoch) {
                         wr -= ohb * p->g_c * mr / p->dV;
                         wi -= ohb * p->g_c * mi / p->dV;
@@ -337,7 +398,9 @@ static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
                         const double nself = comp ? nrmm : nrmp, nother = comp ? nrmp : nrmm;
                         double res_r = ohb * p->m_eff_scaled * MI_RE(rr, ri);
                         double res_i = ohb * p->m_eff_scaled * MI_IM(rr, ri);
-                        if (p->use_potential) {
+                        if (p->src_groups[1] > 0) {
+                            add_pot_chain(w, comp, i, wt, mr, mi, &res_r, &res_i);
+                        } else if (p->use_potential) {
                             const double v = w->pot[comp][i] * vamp;
                             res_r += ohb * v * mr;
                             res_i += ohb * v * mi;
@@ -349,6 +412,7 @@ static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
                         res_r += ohb * p->g_pm * nother * mr; res_i += ohb * p->g_pm * nother * mi;
                         res_r += ohb * p->delta_LT * MI_RE(cr, ci);
                         res_i += ohb * p->delta_LT * MI_IM(cr, ci);
+                        add_pulse_chain(w, comp, i, wt, &res_r, &res_i);
                         if (stoch) {
                             res_r -= ohb * p->g_c * mr / p->dV;
                             res_i -= ohb * p->g_c * mi / p->dV;
@@ -356,7 +420,7 @@ static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
                         w->kpsi[comp][2 * ki] = res_r;
                         w->kpsi[comp][2 * ki + 1] = res_i;
                         double kn = -(p->gamma_r + p->R * nself) * nv;
-                        if (p->use_pump) kn += w->pump[comp][i] * pamp;
+                        kn = pump_term(w, comp, i, wt, pamp, kn);
                         if (stoch) kn += p->R * nv / p->dV;
                         w->kn[comp][ki] = kn;
                     }
@@ -399,7 +463,7 @@ static void add_dw(const ws_t* w, size_t i, double wsum, double* out_r, double* 
 static void sum_to_buffer(ws_t* w, int index, int nw, const double* wts) {
     const int ch = w->h - index;
     const int imag = w->p->imag_time_amplitude != 0.0;
-    int slot[8];
+    int slot[13];
     for (int j = 0; j < nw; j++) slot[j] = (nw == 1) ? index - 1 : j;
     double wsum = 0.0;
     for (int j = 0; j < nw; j++) wsum += wts[j];
@@ -431,7 +495,7 @@ static void sum_to_buffer(ws_t* w, int index, int nw, const double* wts) {
 static void final_sum(ws_t* w, int index, int nw, const double* wts) {
     const int ch = w->h;
     const int imag = w->p->imag_time_amplitude != 0.0;
-    int slot[8];
+    int slot[13];
     for (int j = 0; j < nw; j++) slot[j] = (nw == 1) ? index - 1 : j;
     double wsum = 0.0;
     for (int j = 0; j < nw; j++) wsum += wts[j];
@@ -496,13 +560,22 @@ static void load_state(ws_t* w, double* psi_plus, double* psi_minus, double* n_p
     const orc_params* p = w->p;
     to_sub(w, w->psi[0], psi_plus, 2);
     if (p->use_reservoir) to_sub(w, w->n[0], n_plus, 1);
-    if (p->use_pump) { to_sub(w, w->pump[0], pump_plus, 1); sync_halos(w, w->pump[0], 1); }
-    if (p->use_potential) { to_sub(w, w->pot[0], pot_plus, 1); sync_halos(w, w->pot[0], 1); }
+    /* (a source given as a chain has no single field: NULL) */
+    if (p->use_pump && pump_plus) { to_sub(w, w->pump[0], pump_plus, 1); sync_halos(w, w->pump[0], 1); }
+    if (p->use_potential && pot_plus) { to_sub(w, w->pot[0], pot_plus, 1); sync_halos(w, w->pot[0], 1); }
+    for (int q = 0; q < 3; q++)
+        for (int g = 0; g < p->src_groups[q]; g++)
+            for (int c = 0; c < w->ncomp; c++) {
+                const int el = q == 2 ? 2 : 1;
+                if (!p->src_field[q][g][c]) continue;
+                to_sub(w, w->src[q][g][c], p->src_field[q][g][c], el);
+                sync_halos(w, w->src[q][g][c], el);
+            }
     if (p->tetm) {
         to_sub(w, w->psi[1], psi_minus, 2);
         if (p->use_reservoir) to_sub(w, w->n[1], n_minus, 1);
-        if (p->use_pump) { to_sub(w, w->pump[1], pump_minus, 1); sync_halos(w, w->pump[1], 1); }
-        if (p->use_potential) { to_sub(w, w->pot[1], pot_minus, 1); sync_halos(w, w->pot[1], 1); }
+        if (p->use_pump && pump_minus) { to_sub(w, w->pump[1], pump_minus, 1); sync_halos(w, w->pump[1], 1); }
+        if (p->use_potential && pot_minus) { to_sub(w, w->pot[1], pot_minus, 1); sync_halos(w, w->pot[1], 1); }
     }
 }
 
@@ -681,7 +754,7 @@ typedef struct {
     int index;
     double w;
     int nw;
-    double wts[8];
+    double wts[13];
 } orc_op;
 
 typedef struct {
@@ -689,8 +762,8 @@ typedef struct {
     int halo;        /* halo_size_ of the reference iterator */
     double power;    /* adaptTimeStep exponent; 0: no variable time step */
     int nfixed, nadapt;
-    orc_op fixed[16];
-    orc_op adapt[16];
+    orc_op fixed[28];
+    orc_op adapt[28];
 } orc_iter;
 
 #define K1_ {'K', 1, 0.0, 0, {0}}
@@ -819,6 +892,112 @@ static const orc_iter ITERS[] = {
       {'E', 7, 0, 7,
        {35.0 / 384.0 - 5179.0 / 57600.0, 0.0, 500.0 / 1113.0 - 7571.0 / 16695.0, 125.0 / 192.0 - 393.0 / 640.0,
         -2187.0 / 6784.0 + 92097.0 / 339200.0, 11.0 / 84.0 - 187.0 / 2100.0, -1.0 / 40.0}}}},
+    /* NSRK78::step (NSRK78.cu:22-116); its ERROR_K weights as passed there */
+    {"NSRK78", 13, 1.0 / 8.0, 26, 27,
+     {K1_,
+      {'I', 1, 0, 1, {1.0 / 1000.0}},
+      KB(2, 0.001),
+      {'I', 2, 0, 2, {-491.0 / 81.0, 500.0 / 81.0}},
+      KB(3, 0.1111111111),
+      {'I', 3, 0, 3, {1.0 / 24.0, 0.0, 1.0 / 8.0}},
+      KB(4, 0.1666666667),
+      {'I', 4, 0, 4, {5.0 / 12.0, 0.0, -25.0 / 16.0, 25.0 / 16.0}},
+      KB(5, 0.4166666667),
+      {'I', 5, 0, 5, {1.0 / 20.0, 0.0, 0.0, 1.0 / 4.0, 1.0 / 5.0}},
+      KB(6, 0.5),
+      {'I', 6, 0, 6, {-43.0 / 180.0, 0.0, 0.0, 127.0 / 108.0, -329.0 / 135.0, 7.0 / 3.0}},
+      KB(7, 0.8333333333),
+      {'I', 7, 0, 7, {27931.0 / 240300.0, 0.0, 0.0, -631.0 / 16020.0, 2459.0 / 6675.0, -3572.0 / 12015.0, 5.0 / 267.0}},
+      KB(8, 0.1666666667),
+      {'I', 8, 0, 8, {26114.0 / 12015.0, 0.0, 0.0, -7480.0 / 801.0, 67264.0 / 4005.0, -30640.0 / 2403.0, 1051.0 / 1335.0, 3.0}},
+      KB(9, 0.6666666667),
+      {'I', 9, 0, 9, {33096587331.0 / 17382812500.0, 0.0, 0.0, -1828977848.0 / 173828125.0, 62801809904.0 / 4345703125.0, -9389764774.0 / 869140625.0, 6380757669.0 / 8691406250.0, 98417891.0 / 19531250.0, -1692691.0 / 39062500.0}},
+      KB(10, 0.76),
+      {'I', 10, 0, 10, {-1456295425347.0 / 2642187500000.0, 0.0, 0.0, 110740056.0 / 34765625.0, -21221682384.0 / 4345703125.0, 58859060169.0 / 13906250000.0, -177381525069.0 / 1529687500000.0, -28942485159.0 / 27812500000.0, -1272297.0 / 312500000.0, 5151.0 / 297616.0}},
+      KB(11, 0.84),
+      {'I', 11, 0, 11, {844300798.0 / 137013275.0, 0.0, 0.0, -9997568.0 / 206035.0, 49636624.0 / 1030175.0, -3358834871.0 / 91067470.0, -40456983.0 / 1813108.0, 495817135.0 / 16647628.0, -149375.0 / 84266.0, 7470703125.0 / 1567431866.0, 1562500000.0 / 72342361.0}},
+      KB(12, 1.0),
+      {'I', 12, 0, 12, {-26225423.0 / 37371100.0, 0.0, 0.0, 807744.0 / 98345.0, -205584.0 / 37825.0, 5882202.0 / 1278485.0, -84543.0 / 432718.0, -223415.0 / 39338.0, -3625.0 / 6188.0, 292968750.0 / 374084711.0, 0.0, 0.0}},
+      KB(13, 1.0),
+      {'F', 13, 0, 13, {4241.0 / 88200.0, 0.0, 0.0, 0.0, 0.0, 9946.0 / 23205.0, -10449.0 / 1925.0, 415449.0 / 1573075.0, -2025.0 / 5096.0, 48828125.0 / 44900856.0, 48828125.0 / 9843561.0, 463.0 / 12600.0, 0.0}}},
+     {K1_,
+      {'I', 1, 0, 1, {1.0 / 1000.0}},
+      KB(2, 0.001),
+      {'I', 2, 0, 2, {-491.0 / 81.0, 500.0 / 81.0}},
+      KB(3, 0.1111111111),
+      {'I', 3, 0, 3, {1.0 / 24.0, 0.0, 1.0 / 8.0}},
+      KB(4, 0.1666666667),
+      {'I', 4, 0, 4, {5.0 / 12.0, 0.0, -25.0 / 16.0, 25.0 / 16.0}},
+      KB(5, 0.4166666667),
+      {'I', 5, 0, 5, {1.0 / 20.0, 0.0, 0.0, 1.0 / 4.0, 1.0 / 5.0}},
+      KB(6, 0.5),
+      {'I', 6, 0, 6, {-43.0 / 180.0, 0.0, 0.0, 127.0 / 108.0, -329.0 / 135.0, 7.0 / 3.0}},
+      KB(7, 0.8333333333),
+      {'I', 7, 0, 7, {27931.0 / 240300.0, 0.0, 0.0, -631.0 / 16020.0, 2459.0 / 6675.0, -3572.0 / 12015.0, 5.0 / 267.0}},
+      KB(8, 0.1666666667),
+      {'I', 8, 0, 8, {26114.0 / 12015.0, 0.0, 0.0, -7480.0 / 801.0, 67264.0 / 4005.0, -30640.0 / 2403.0, 1051.0 / 1335.0, 3.0}},
+      KB(9, 0.6666666667),
+      {'I', 9, 0, 9, {33096587331.0 / 17382812500.0, 0.0, 0.0, -1828977848.0 / 173828125.0, 62801809904.0 / 4345703125.0, -9389764774.0 / 869140625.0, 6380757669.0 / 8691406250.0, 98417891.0 / 19531250.0, -1692691.0 / 39062500.0}},
+      KB(10, 0.76),
+      {'I', 10, 0, 10, {-1456295425347.0 / 2642187500000.0, 0.0, 0.0, 110740056.0 / 34765625.0, -21221682384.0 / 4345703125.0, 58859060169.0 / 13906250000.0, -177381525069.0 / 1529687500000.0, -28942485159.0 / 27812500000.0, -1272297.0 / 312500000.0, 5151.0 / 297616.0}},
+      KB(11, 0.84),
+      {'I', 11, 0, 11, {844300798.0 / 137013275.0, 0.0, 0.0, -9997568.0 / 206035.0, 49636624.0 / 1030175.0, -3358834871.0 / 91067470.0, -40456983.0 / 1813108.0, 495817135.0 / 16647628.0, -149375.0 / 84266.0, 7470703125.0 / 1567431866.0, 1562500000.0 / 72342361.0}},
+      KB(12, 1.0),
+      {'I', 12, 0, 12, {-26225423.0 / 37371100.0, 0.0, 0.0, 807744.0 / 98345.0, -205584.0 / 37825.0, 5882202.0 / 1278485.0, -84543.0 / 432718.0, -223415.0 / 39338.0, -3625.0 / 6188.0, 292968750.0 / 374084711.0, 0.0, 0.0}},
+      KB(13, 1.0),
+      {'I', 13, 0, 13, {4241.0 / 88200.0, 0.0, 0.0, 0.0, 0.0, 9946.0 / 23205.0, -10449.0 / 1925.0, 415449.0 / 1573075.0, -2025.0 / 5096.0, 48828125.0 / 44900856.0, 48828125.0 / 9843561.0, 463.0 / 12600.0, 0.0}},
+      {'E', 13, 0, 13, {4241.0 / 88200.0 - 3799.0 / 79800.0, 0.0, 0.0, 0.0, 0.0, 9946.0 / 23205.0 - 538.0 / 1365.0, -10449.0 / 1925.0 - 351.0 / 1925.0, 415449.0 / 1573075.0 - 4149.0 / 15575.0, -2025.0 / 5096.0 - (-45.0 / 392.0), 48828125.0 / 44900856.0 - 48828125.0 / 284372088.0, 48828125.0 / 9843561.0 - 0.0, 463.0 / 12600.0 - 0.0, 0.0 - 221.0 / 4200.0}}}},
+    /* DormandPrince85 coefficients (dormand_prince.cu:128-205); the reference swaps its fixed and adaptive branches: here the fixed step applies b and the adaptive step estimates its error with b - b5, b5 the weights its ERROR_K call passes */
+    {"DP85", 12, 1.0 / 4.0, 24, 25,
+     {K1_,
+      {'I', 1, 0, 1, {5.260015195876773e-2}},
+      KB(2, 0.05260015196),
+      {'I', 2, 0, 2, {1.9725056984537899e-2, 5.9175170953613698e-2}},
+      KB(3, 0.07890022794),
+      {'I', 3, 0, 3, {2.9587585476806849e-2, 0.0, 8.8762756430420548e-2}},
+      KB(4, 0.1183503419),
+      {'I', 4, 0, 4, {2.4136513415926669e-1, 0.0, -8.8454947932828610e-1, 9.2483400326179200e-1}},
+      KB(5, 0.2816496581),
+      {'I', 5, 0, 5, {3.7037037037037037e-2, 0.0, 0.0, 1.7082860872947387e-1, 1.2546768756682243e-1}},
+      KB(6, 0.3333333333),
+      {'I', 6, 0, 6, {3.7109375e-2, 0.0, 0.0, 1.7025221101954404e-1, 6.0216538980455961e-2, -1.7578125e-2}},
+      KB(7, 0.25),
+      {'I', 7, 0, 7, {3.7092000118504793e-2, 0.0, 0.0, 1.7038392571223999e-1, 1.0726203044637328e-1, -1.5319437748624402e-2, 8.2737891638140229e-3}},
+      KB(8, 0.3076923077),
+      {'I', 8, 0, 8, {6.2411095871607572e-1, 0.0, 0.0, -3.3608926294469413e+0, -8.6821934684172601e-1, 2.7592099699446708e+1, 2.0154067550477893e+1, -4.3489884181069959e+1}},
+      KB(9, 0.6512820513),
+      {'I', 9, 0, 9, {4.7766253643826437e-1, 0.0, 0.0, -2.4881146199716676e+0, -5.9029082683684300e-1, 2.1230051448181194e+1, 1.5279233632882424e+1, -3.3288210968984863e+1, -2.0331201708508626e-2}},
+      KB(10, 0.6),
+      {'I', 10, 0, 10, {-9.3714243008598733e-1, 0.0, 0.0, 5.1863724288440637e+0, 1.0914373489967296e+0, -8.1497870107469261e+0, -1.8520065659996960e+1, 2.2739487099350504e+1, 2.4936055526796524e+0, -3.0467644718982195e+0}},
+      KB(11, 0.8571428571),
+      {'I', 11, 0, 11, {2.2733101475165382e+0, 0.0, 0.0, -1.0534495466737250e+1, -2.0008720582248625e+0, -1.7958931863118799e+1, 2.7948884529419960e+1, -2.8589982771350237e+0, -8.8728569335306295e+0, 1.2360567175794303e+1, 6.4339274601576353e-1}},
+      KB(12, 1.0),
+      {'F', 12, 0, 12, {5.4293734116568762e-2, 0.0, 0.0, 0.0, 0.0, 4.4503128927524089e+0, 1.8915178993145004e+0, -5.8012039600105848e+0, 3.1116436695781989e-1, -1.5216094966251608e-1, 2.0136540080403035e-1, 4.4710615727772591e-2}}},
+     {K1_,
+      {'I', 1, 0, 1, {5.260015195876773e-2}},
+      KB(2, 0.05260015196),
+      {'I', 2, 0, 2, {1.9725056984537899e-2, 5.9175170953613698e-2}},
+      KB(3, 0.07890022794),
+      {'I', 3, 0, 3, {2.9587585476806849e-2, 0.0, 8.8762756430420548e-2}},
+      KB(4, 0.1183503419),
+      {'I', 4, 0, 4, {2.4136513415926669e-1, 0.0, -8.8454947932828610e-1, 9.2483400326179200e-1}},
+      KB(5, 0.2816496581),
+      {'I', 5, 0, 5, {3.7037037037037037e-2, 0.0, 0.0, 1.7082860872947387e-1, 1.2546768756682243e-1}},
+      KB(6, 0.3333333333),
+      {'I', 6, 0, 6, {3.7109375e-2, 0.0, 0.0, 1.7025221101954404e-1, 6.0216538980455961e-2, -1.7578125e-2}},
+      KB(7, 0.25),
+      {'I', 7, 0, 7, {3.7092000118504793e-2, 0.0, 0.0, 1.7038392571223999e-1, 1.0726203044637328e-1, -1.5319437748624402e-2, 8.2737891638140229e-3}},
+      KB(8, 0.3076923077),
+      {'I', 8, 0, 8, {6.2411095871607572e-1, 0.0, 0.0, -3.3608926294469413e+0, -8.6821934684172601e-1, 2.7592099699446708e+1, 2.0154067550477893e+1, -4.3489884181069959e+1}},
+      KB(9, 0.6512820513),
+      {'I', 9, 0, 9, {4.7766253643826437e-1, 0.0, 0.0, -2.4881146199716676e+0, -5.9029082683684300e-1, 2.1230051448181194e+1, 1.5279233632882424e+1, -3.3288210968984863e+1, -2.0331201708508626e-2}},
+      KB(10, 0.6),
+      {'I', 10, 0, 10, {-9.3714243008598733e-1, 0.0, 0.0, 5.1863724288440637e+0, 1.0914373489967296e+0, -8.1497870107469261e+0, -1.8520065659996960e+1, 2.2739487099350504e+1, 2.4936055526796524e+0, -3.0467644718982195e+0}},
+      KB(11, 0.8571428571),
+      {'I', 11, 0, 11, {2.2733101475165382e+0, 0.0, 0.0, -1.0534495466737250e+1, -2.0008720582248625e+0, -1.7958931863118799e+1, 2.7948884529419960e+1, -2.8589982771350237e+0, -8.8728569335306295e+0, 1.2360567175794303e+1, 6.4339274601576353e-1}},
+      KB(12, 1.0),
+      {'I', 12, 0, 12, {5.4293734116568762e-2, 0.0, 0.0, 0.0, 0.0, 4.4503128927524089e+0, 1.8915178993145004e+0, -5.8012039600105848e+0, 3.1116436695781989e-1, -1.5216094966251608e-1, 2.0136540080403035e-1, 4.4710615727772591e-2}},
+      {'E', 12, 0, 12, {5.4293734116568762e-2 - (4.1173689122373888e-2), 0.0, 0.0, 0.0, 0.0, 4.4503128927524089e+0 - (5.6754693391286133e+0), 1.8915178993145004e+0 - (2.3872768489717506e+0), -5.8012039600105848e+0 - (-7.4655811424655713e+0), 3.1116436695781989e-1 - (6.6149321570779360e-1), -1.5216094966251608e-1 - (-4.8634006837553356e-1), 2.0136540080403035e-1 - (1.1944219431891464e-1), 4.4710615727772591e-2 - (6.7065923591658886e-2)}}}},
 };
 #undef K1_
 #undef KB
@@ -841,7 +1020,7 @@ static int kmax_of(const orc_op* ops, int n) {
    interior cells (Solver::adaptTimeStep, solver.cu:73-74). */
 static void error_sum(const ws_t* w, int nw, const double* wts, double* err_sum, double* norm) {
     const orc_params* p = w->p;
-    int slot[8];
+    int slot[13];
     for (int j = 0; j < nw; j++) slot[j] = j;
     double es = 0.0, ns = 0.0;
     for (int g = 0; g < w->S; g++)

@@ -27,6 +27,8 @@ extern "C" {
 /* Threads of the OpenMP subgrid loops; n <= 0 only queries.  Returns the count in use. */
 int orc_set_threads(int n);
 
+#define ORC_MAX_GROUPS 8
+
 typedef struct {
     /* geometry */
     int N_c, N_r;
@@ -51,6 +53,15 @@ typedef struct {
     double imag_time_amplitude;
     /* domain lengths (set by orc_derive; the split-step iterator's wave numbers use them) */
     double L_x, L_y;
+    /* Source chains (kernel_gp_compute.cuh:68-100,146-175): when src_groups[q] > 0, source q
+       (0 pump, 1 potential, 2 pulse) is sum_g field_g * a_g(w) over its temporal groups, with
+       field_g = src_field[q][g][component] (pump, potential: N reals; pulse: N complex) and
+       a_g(w) = src_amp[q][g] (1 - w) + w src_amp_next[q][g] (complex {re, im}); the pump
+       takes the real part (a real reservoir).  The pump and potential chains replace the
+       single fields and amplitudes above; the pulse adds (1/hbar) sum_g field_g a_g(w) to k_psi. */
+    int src_groups[3];
+    const double* src_field[3][ORC_MAX_GROUPS][2];
+    double src_amp[3][ORC_MAX_GROUPS][2], src_amp_next[3][ORC_MAX_GROUPS][2];
 } orc_params;
 
 /* Fill derived constants exactly as SystemParameters::calculateAuto does.

@@ -41,7 +41,8 @@ CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I"
 
 # longest compiles first: the thread pool then finishes them side by side
 HIP_SOURCES = ["phx_rk4_f64.hip", "phx_rk4.hip", "phx_rk_generic.hip", "phx_rk_tab6.hip", "phx_rk_tab6f.hip", "phx_rk_tab4.hip",
-               "phx_rk_tab3.hip", "phx_fft.hip", "phx_solver.hip", "phx_ensemble.hip", "phx_adjoint.hip", "phx_ssfm.hip"]
+               "phx_rk_tab3.hip", "phx_fft.hip", "phx_solver.hip", "phx_ensemble.hip", "phx_adjoint.hip", "phx_ssfm.hip",
+               "phx_staged.hip"]
 CXX_SOURCES = ["phx_config.cpp"]
 
 

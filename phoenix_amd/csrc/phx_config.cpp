@@ -586,7 +586,6 @@ std::string finalize_config(SystemConfig& c) {
     /* imaginary time turns the reference's (complex-stored) reservoir complex; the reservoir here is real */
     if (c.imag_time_amplitude != 0.0 && c.use_reservoir)
         return "--imagTime with a reservoir (pump or initial reservoir) is not provided by this library";
-    if (c.use_pulses) return "--pulse is not provided by this library";
     /* calculateAuto (system.cpp:116-160) */
     if (c.h_bar_s < 0) c.h_bar_s = c.h_bar / c.e_e * 1E12;
     if (c.m_eff < 0) c.m_eff = c.m_e / c.e_e * 1E8;
@@ -620,9 +619,6 @@ std::string finalize_config(SystemConfig& c) {
     if (c.use_pumps)
         for (int g = 0; g < c.pump.groupSize(); g++)
             if (c.pump.groups[g].kind & TMP_IEXP) return "complex (iexp) temporal pump envelopes are not provided by this library";
-    if (c.use_potentials)
-        for (int g = 0; g < c.potential.groupSize(); g++)
-            if (c.potential.groups[g].kind & TMP_IEXP) return "complex (iexp) temporal potential envelopes are not provided by this library";
     return "";
 }
 

@@ -162,6 +162,62 @@ hipError_t launch_rk_tab6(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s
 template <typename real>
 hipError_t launch_rk_tab6f(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
 
+/* Staged Runge-Kutta step (phx_staged.hip): one launch per CALCULATE_K, INTERMEDIATE_SUM_K
+   and FINAL_SUM_K / ERROR_K of the reference's sequence, K's in HBM on a grid padded by H
+   cells.  For tableaux with more stages than the fused kernel holds, and for sources the
+   fused kernels do not take: pump, potential and pulse chains of several temporal groups,
+   complex temporal envelopes of the potential and the pulse. */
+constexpr int kStagedMaxGroups = 8;
+enum class StagedOp { K, SUM, FINAL };
+template <typename real>
+struct StagedArgs {
+    using v2 = typename Vec2<real>::type;
+    /* step-start state and output (full grid, row-major); x_n[0] == nullptr: no reservoir */
+    const v2* x_psi[2];
+    const real* x_n[2];
+    v2* out_psi[2];
+    real* out_n[2];
+    /* stage input and the K slots (padded grid; slot s at s * padN) */
+    v2* y_psi[2];
+    real* y_n[2];
+    v2* k_psi[2];
+    real* k_n[2];
+    /* sources: n groups of full-grid fields per component, group g at g * N */
+    const real* pump[2];
+    const real* pot[2];
+    const v2* pulse[2];
+    int npump, npot, npulse;
+    /* this stage's temporal amplitudes a_g(w) = a_g(t)(1 - w) + w a_g(t + dt) */
+    real pump_amp[kStagedMaxGroups];
+    v2 pot_amp[kStagedMaxGroups];
+    v2 pulse_amp[kStagedMaxGroups];
+    const v2* noise; /* this step's noise field (full grid) or nullptr */
+    int stochastic, tetm;
+    int Nc, Nr, H, P; /* P = Nc + 2H */
+    size_t N, padN;   /* Nc Nr, (Nr + 2H) P */
+    int periodic_x, periodic_y;
+    int stage; /* K: slot written (0-based) */
+    int ext;   /* K, SUM: padded cells within ext of the grid */
+    /* SUM / FINAL: weights w[j] of slots slot[j]; wsum = their sum (noise increment) */
+    int nw;
+    int slot[kStagedMaxStages];
+    real w[kStagedMaxStages];
+    real wsum;
+    /* FINAL: error weights e[j] of slots eslot[j] (ne = 0: no error) */
+    int ne;
+    int eslot[kStagedMaxStages];
+    real e[kStagedMaxStages];
+    real dt, dtr, dti; /* psi advances by (dtr + i dti) sum, the reservoir by dt */
+    double* partials;  /* FINAL with ne > 0: (error, norm) per block */
+    PhxPhys<real> ph;
+};
+template <typename real>
+hipError_t launch_staged(const StagedArgs<real>& a, StagedOp op, hipStream_t s);
+/* deterministic reduction of (error, norm) pairs into sums[0..1] (one workgroup, fixed order) */
+hipError_t launch_reduce_partials(const double* partials, int nwaves, double* sums, hipStream_t s);
+/* blocks of the FINAL launch (partials buffer size = 2 * this) */
+inline size_t staged_final_blocks(int Nc, int Nr) { return (size_t)((Nc + 63) / 64) * (size_t)((Nr + 3) / 4); }
+
 /* Imaginary-time normalisation (Solver::normalizeImaginaryTimePropagation): psi_k *= sqrt(amp /
    (S_k dV)) with S_k = sum over the grid of |psi_k|^2 (FP64, fixed order; S_k < 1e-10 -> 1), for
    psi_plus and, if not null, psi_minus.  partials: 2 doubles per 256-cell block, sums: 2 doubles;

@@ -161,6 +161,11 @@ hipError_t launch_rk_generic(const RkGenArgs<real>& a, RkTableau tab, hipStream_
         case RkTableau::CASHKARP_ADAPTIVE: return launch_rk_tab6<real>(a, tab, s);
         case RkTableau::FEHLBERG5:
         case RkTableau::FEHLBERG5_ADAPTIVE: return launch_rk_tab6f<real>(a, tab, s);
+        /* more stages than the fused kernel keeps in registers: staged path (launch_staged) */
+        case RkTableau::NSRK78:
+        case RkTableau::NSRK78_ADAPTIVE:
+        case RkTableau::DP85:
+        case RkTableau::DP85_ADAPTIVE: return hipErrorInvalidValue;
     }
     return hipErrorInvalidValue;
 }

@@ -93,6 +93,17 @@ struct phx_solver {
     double graph_dt = 0;
     hipStream_t graph_stream = nullptr;
     bool graph_off = false;
+    /* staged path (phx_staged.hip): padded stage input and K slots, per-group source fields */
+    bool staged = false;
+    int staged_H = 0;
+    size_t staged_padN = 0;
+    void* st_y_psi[2] = {nullptr, nullptr};
+    void* st_y_n[2] = {nullptr, nullptr};
+    void* st_k_psi[2] = {nullptr, nullptr};
+    void* st_k_n[2] = {nullptr, nullptr};
+    void* st_pulse[2] = {nullptr, nullptr};
+    int st_npump = 0, st_npot = 0, st_npulse = 0;
+    double* st_partials = nullptr;
     std::vector<void*> allocs;
 
     size_t cells() const { return (size_t)cfg.N_c * cfg.N_r; }
@@ -168,10 +179,19 @@ int initialize_t(phx_solver* s) {
     if (c.use_fft_mask && (!phx::fft_supported(c.N_c, sizeof(real)) || !phx::fft_supported(c.N_r, sizeof(real))))
         return fail(PHX_ERR_UNSUPPORTED, "--fftMask needs power-of-two grid sizes between 4 and " +
                                              std::to_string(sizeof(real) == 4 ? 8192 : 4096));
-    if (c.use_pumps && c.pump.groupSize() > 1 && !c.pump.allConstant())
-        return fail(PHX_ERR_UNSUPPORTED, "more than one temporal pump group");
-    if (c.use_potentials && c.potential.groupSize() > 1 && !c.potential.allConstant())
-        return fail(PHX_ERR_UNSUPPORTED, "more than one temporal potential group");
+    /* The staged path serves what the fused kernels do not: more than seven stages, pulses,
+       several non-constant temporal groups of a source, complex temporal potential envelopes. */
+    const auto complex_temporal = [](const phx::Envelope& e) {
+        for (int g = 0; g < e.groupSize(); g++)
+            if (e.groups[g].kind & phx::TMP_IEXP) return true;
+        return false;
+    };
+    s->staged = (s->iter && (!phx::rk_tableau_fused(s->iter->fixed) || !phx::rk_tableau_fused(s->iter->adaptive))) ||
+                c.use_pulses || (c.use_pumps && c.pump.groupSize() > 1 && !c.pump.allConstant()) ||
+                (c.use_potentials && c.potential.groupSize() > 1 && !c.potential.allConstant()) ||
+                (c.use_potentials && complex_temporal(c.potential));
+    if (s->staged && s->ssfm)
+        return fail(PHX_ERR_UNSUPPORTED, "--pulse and non-constant temporal groups are not provided with SSFM");
 
     for (int k = 0; k < ncomp; k++) {
         for (int b = 0; b < 2; b++) {
@@ -179,15 +199,44 @@ int initialize_t(phx_solver* s) {
             if (c.use_reservoir)
                 if (int r = dev_alloc(s, &s->n[k][b], N * sizeof(real))) return r;
         }
+        /* staged path: one field per temporal group (group g at g N); the fused kernels: one
+           field, all groups summed (constant envelopes) or the single group */
+        s->st_npump = (s->staged && c.use_reservoir && c.use_pumps) ? c.pump.groupSize() : 0;
+        s->st_npot = (s->staged && c.use_potentials) ? c.potential.groupSize() : 0;
+        s->st_npulse = (s->staged && c.use_pulses) ? c.pulse.groupSize() : 0;
+        if (s->st_npump > phx::kStagedMaxGroups || s->st_npot > phx::kStagedMaxGroups || s->st_npulse > phx::kStagedMaxGroups)
+            return fail(PHX_ERR_UNSUPPORTED, "more than " + std::to_string(phx::kStagedMaxGroups) + " temporal groups of one source");
         if (c.use_reservoir && c.use_pumps)
-            if (int r = dev_alloc(s, &s->pump[k], N * sizeof(real))) return r;
+            if (int r = dev_alloc(s, &s->pump[k], std::max(1, s->st_npump) * N * sizeof(real))) return r;
         if (c.use_potentials)
-            if (int r = dev_alloc(s, &s->pot[k], N * sizeof(real))) return r;
+            if (int r = dev_alloc(s, &s->pot[k], std::max(1, s->st_npot) * N * sizeof(real))) return r;
+        if (s->st_npulse)
+            if (int r = dev_alloc(s, &s->st_pulse[k], s->st_npulse * 2 * N * sizeof(real))) return r;
         if (c.use_fft_mask)
             if (int r = dev_alloc(s, &s->mask[k], N * sizeof(real))) return r;
     }
     if (c.use_stochastic)
         if (int r = dev_alloc(s, &s->noise, 2 * N * sizeof(real))) return r;
+    if (s->staged) {
+        const int H = std::max(phx::rk_tableau_stages(s->iter->fixed), phx::rk_tableau_stages(s->iter->adaptive));
+        s->staged_H = H;
+        s->staged_padN = (size_t)(c.N_c + 2 * H) * (size_t)(c.N_r + 2 * H);
+        const size_t pn = s->staged_padN;
+        for (int k = 0; k < ncomp; k++) {
+            if (int r = dev_alloc(s, &s->st_y_psi[k], 2 * pn * sizeof(real))) return r;
+            if (int r = dev_alloc(s, &s->st_k_psi[k], (size_t)H * 2 * pn * sizeof(real))) return r;
+            if (c.use_reservoir) {
+                if (int r = dev_alloc(s, &s->st_y_n[k], pn * sizeof(real))) return r;
+                if (int r = dev_alloc(s, &s->st_k_n[k], (size_t)H * pn * sizeof(real))) return r;
+            }
+        }
+        const size_t nb = phx::staged_final_blocks(c.N_c, c.N_r);
+        if (int r = dev_alloc(s, (void**)&s->st_partials, 2 * nb * sizeof(double))) return r;
+        if (!s->sums) {
+            if (int r = dev_alloc(s, (void**)&s->sums, 2 * sizeof(double))) return r;
+            PHX_HIP(hipHostMalloc((void**)&s->sums_host, 2 * sizeof(double), hipHostMallocDefault));
+        }
+    }
     if (c.imag_time_amplitude != 0.0) {
         if (int r = dev_alloc(s, (void**)&s->norm_partials, 2 * sizeof(double) * phx::imag_norm_blocks(N))) return r;
         if (int r = dev_alloc(s, (void**)&s->norm_sums, 2 * sizeof(double))) return r;
@@ -233,13 +282,36 @@ int initialize_t(phx_solver* s) {
             if (int r = upload_real<real>(s, s->n[k][0], n0[k])) return r;
         const uint32_t pol = k == 0 ? phx::POL_PLUS : phx::POL_MINUS;
         std::vector<std::complex<double>> f;
+        /* one spatial field per temporal group (solver_initialization.cu:120-150) */
+        auto group_field = [&](const phx::Envelope& e, int g) {
+            f.assign(N, {0.0, 0.0});
+            e.calculate(f.data(), g, pol, c.N_c, c.N_r, c.L_x, c.L_y, c.dx, c.dy, 0.0);
+        };
         if (s->pump[k]) {
-            env_field(s, c.pump, pol, f);
-            if (int r = upload_real<real>(s, s->pump[k], f)) return r;
+            if (s->st_npump) {
+                for (int g = 0; g < s->st_npump; g++) {
+                    group_field(c.pump, g);
+                    if (int r = upload_real<real>(s, (real*)s->pump[k] + (size_t)g * N, f)) return r;
+                }
+            } else {
+                env_field(s, c.pump, pol, f);
+                if (int r = upload_real<real>(s, s->pump[k], f)) return r;
+            }
         }
         if (s->pot[k]) {
-            env_field(s, c.potential, pol, f);
-            if (int r = upload_real<real>(s, s->pot[k], f)) return r;
+            if (s->st_npot) {
+                for (int g = 0; g < s->st_npot; g++) {
+                    group_field(c.potential, g);
+                    if (int r = upload_real<real>(s, (real*)s->pot[k] + (size_t)g * N, f)) return r;
+                }
+            } else {
+                env_field(s, c.potential, pol, f);
+                if (int r = upload_real<real>(s, s->pot[k], f)) return r;
+            }
+        }
+        for (int g = 0; g < s->st_npulse; g++) {
+            group_field(c.pulse, g);
+            if (int r = upload_complex<real>(s, (real*)s->st_pulse[k] + (size_t)g * 2 * N, f)) return r;
         }
         if (s->mask[k]) {
             f.assign(N, {0.0, 0.0});
@@ -351,6 +423,115 @@ int generic_launch(phx_solver* s, phx::RkTableau tab, double t_k, double dt) {
     a.ph = phx::make_phys<real>(c);
     PHX_HIP(phx::launch_rk_generic<real>(a, tab, s->stream));
     return PHX_OK;
+}
+
+/* One staged step attempt (phx_staged.hip) from the current buffers into the other ones:
+   the reference's CALCULATE_K / INTERMEDIATE_SUM_K sequence on shrinking extents, then the
+   final sum (and, for an adaptive tableau, the reduced error and norm in s->sums).  Each
+   temporal group g of a source enters stage s with a_g(t)(1 - c_s) + c_s a_g(t + dt)
+   (kernel_gp_compute.cuh:79,93,98). */
+template <typename real>
+int staged_attempt(phx_solver* s, phx::RkTableau tab_id, double t_k, double dt) {
+    auto& c = s->cfg;
+    using v2 = typename phx::Vec2<real>::type;
+    const phx::StagedTab tab = phx::staged_tableau(tab_id);
+    phx::StagedArgs<real> a{};
+    const int nxt = s->cur ^ 1;
+    const int ncomp = c.tetm ? 2 : 1;
+    for (int k = 0; k < ncomp; k++) {
+        a.x_psi[k] = (const v2*)s->psi[k][s->cur];
+        a.out_psi[k] = (v2*)s->psi[k][nxt];
+        a.y_psi[k] = (v2*)s->st_y_psi[k];
+        a.k_psi[k] = (v2*)s->st_k_psi[k];
+        if (c.use_reservoir) {
+            a.x_n[k] = (const real*)s->n[k][s->cur];
+            a.out_n[k] = (real*)s->n[k][nxt];
+            a.y_n[k] = (real*)s->st_y_n[k];
+            a.k_n[k] = (real*)s->st_k_n[k];
+        }
+        a.pump[k] = (const real*)s->pump[k];
+        a.pot[k] = (const real*)s->pot[k];
+        a.pulse[k] = (const v2*)s->st_pulse[k];
+    }
+    a.npump = s->st_npump;
+    a.npot = s->st_npot;
+    a.npulse = s->st_npulse;
+    a.noise = c.use_stochastic ? (const v2*)s->noise : nullptr;
+    a.stochastic = c.use_stochastic;
+    a.tetm = c.tetm;
+    a.Nc = c.N_c;
+    a.Nr = c.N_r;
+    a.H = s->staged_H;
+    a.P = c.N_c + 2 * s->staged_H;
+    a.N = s->cells();
+    a.padN = s->staged_padN;
+    a.periodic_x = c.periodic_x;
+    a.periodic_y = c.periodic_y;
+    a.dt = (real)dt;
+    const bool imag = c.imag_time_amplitude != 0.0;
+    a.dtr = imag ? real(0) : (real)dt;
+    a.dti = imag ? -(real)dt : real(0);
+    a.partials = s->st_partials;
+    a.ph = phx::make_phys<real>(c);
+    /* temporal amplitudes of every group at t and t + dt, in the build's precision */
+    std::complex<real> amp0[3][phx::kStagedMaxGroups], amp1[3][phx::kStagedMaxGroups];
+    const phx::Envelope* env[3] = {&c.pump, &c.potential, &c.pulse};
+    const int ng[3] = {a.npump, a.npot, a.npulse};
+    for (int q = 0; q < 3; q++)
+        for (int g = 0; g < ng[q]; g++) {
+            amp0[q][g] = std::complex<real>(env[q]->temporal(g, t_k));
+            amp1[q][g] = std::complex<real>(env[q]->temporal(g, s->rnd(t_k + dt)));
+        }
+    auto set_amps = [&](real w) {
+        for (int q = 0; q < 3; q++)
+            for (int g = 0; g < ng[q]; g++) {
+                const real re = amp0[q][g].real() * (real(1) - w) + w * amp1[q][g].real();
+                const real im = amp0[q][g].imag() * (real(1) - w) + w * amp1[q][g].imag();
+                if (q == 0) a.pump_amp[g] = re;
+                v2 z;
+                z.x = re;
+                z.y = im;
+                if (q == 1) a.pot_amp[g] = z;
+                if (q == 2) a.pulse_amp[g] = z;
+            }
+    };
+    /* weights of a sum: the nonzero a[s][j] (or b, e) with their slots */
+    auto set_weights = [&](const double* wts, int n, int* slot, real* w, int& count, real* wsum) {
+        count = 0;
+        real sum = 0;
+        for (int j = 0; j < n; j++) {
+            sum += (real)wts[j];
+            if (wts[j] == 0.0) continue;
+            slot[count] = j;
+            w[count] = (real)wts[j];
+            count++;
+        }
+        if (wsum) *wsum = sum;
+    };
+    const int H = s->staged_H;
+    for (int st = 0; st < tab.S; st++) {
+        set_amps((real)tab.c[st]);
+        a.stage = st;
+        a.ext = H - (st + 1);
+        PHX_HIP(phx::launch_staged<real>(a, phx::StagedOp::K, s->stream));
+        if (st + 1 < tab.S) {
+            set_weights(tab.a[st + 1], st + 1, a.slot, a.w, a.nw, &a.wsum);
+            PHX_HIP(phx::launch_staged<real>(a, phx::StagedOp::SUM, s->stream));
+        }
+    }
+    set_weights(tab.b, tab.NB, a.slot, a.w, a.nw, &a.wsum);
+    a.ne = 0;
+    if (tab.err) set_weights(tab.e, tab.S, a.eslot, a.e, a.ne, nullptr);
+    PHX_HIP(phx::launch_staged<real>(a, phx::StagedOp::FINAL, s->stream));
+    if (tab.err)
+        PHX_HIP(phx::launch_reduce_partials(s->st_partials, (int)phx::staged_final_blocks(c.N_c, c.N_r), s->sums, s->stream));
+    return PHX_OK;
+}
+
+/* one step attempt of the configured iterator: fused generic kernel or staged path */
+template <typename real>
+int rk_attempt(phx_solver* s, phx::RkTableau tab, double t_k, double dt) {
+    return s->staged ? staged_attempt<real>(s, tab, t_k, dt) : generic_launch<real>(s, tab, t_k, dt);
 }
 
 /* Solver::applyFFTFilter(true) (solver_fft.cu:67-100) on psi+ and, with -tetm, psi-. */
@@ -488,14 +669,14 @@ int iterate_t(phx_solver* s, bool force_fixed) {
     const bool imag = c.imag_time_amplitude != 0.0;
     if (s->ssfm) {
         if (int r = ssfm_step<real>(s, t_pre)) return r;
-    } else if (c.iterator == "RK4" && !c.tetm && !c.use_stochastic && !imag) {
+    } else if (c.iterator == "RK4" && !c.tetm && !c.use_stochastic && !imag && !s->staged) {
         if (!s->tuned) {
             if (int r = autotune_rk4<real>(s)) return r;
             s->tuned = true;
         }
         if (int r = rk4_launch<real>(s, t_pre, s->dt)) return r;
     } else if (!adaptive) {
-        if (int r = generic_launch<real>(s, s->iter->fixed, t_pre, s->dt)) return r;
+        if (int r = rk_attempt<real>(s, s->iter->fixed, t_pre, s->dt)) return r;
         s->cur ^= 1;
     } else {
         /* step(true) of the adaptive iterators (dormand_prince.cu:37-95, bogacki.cu:28-75,
@@ -504,7 +685,7 @@ int iterate_t(phx_solver* s, bool force_fixed) {
         for (;;) {
             /* retries see the kernel time already advanced (updateKernelTime in adaptTimeStep, solver.cu:97) */
             const double t_k = rejections == 0 ? t_pre : s->t;
-            if (int r = generic_launch<real>(s, s->iter->adaptive, t_k, s->dt)) return r;
+            if (int r = rk_attempt<real>(s, s->iter->adaptive, t_k, s->dt)) return r;
             PHX_HIP(hipMemcpyAsync(s->sums_host, s->sums, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
             PHX_HIP(hipStreamSynchronize(s->stream));
             double new_dt = 0, final_error = 0;
@@ -552,7 +733,7 @@ int graph_block_t(phx_solver* s, int64_t* taken) {
     auto& c = s->cfg;
     *taken = 0;
     const auto constant = [](const phx::Envelope& e) { return e.groupSize() != 1 || e.allConstant(); };
-    if (s->graph_off || !s->tuned || c.iterator != "RK4" || c.tetm || c.use_stochastic || c.imag_time_amplitude != 0.0 ||
+    if (s->graph_off || !s->tuned || s->staged || c.iterator != "RK4" || c.tetm || c.use_stochastic || c.imag_time_amplitude != 0.0 ||
         !constant(c.pump) || !constant(c.potential))
         return PHX_OK;
     if (const char* e = std::getenv("PHX_GRAPH"))
@@ -801,6 +982,7 @@ int phx_get_info(phx_solver* s, phx_info* info) {
     info->rejected = s->rejected;
     info->tile_rows = s->tuned ? s->tune.ty : 0;
     info->prefetch_rows = s->tuned ? s->tune.pf : 0;
+    info->staged = s->staged;
     return PHX_OK;
 }
 

@@ -51,6 +51,11 @@ enum class RkTableau {
     CASHKARP_ADAPTIVE,
     FEHLBERG5,
     FEHLBERG5_ADAPTIVE,
+    /* staged path only (more stages than the fused kernel keeps in registers) */
+    NSRK78,
+    NSRK78_ADAPTIVE,
+    DP85,
+    DP85_ADAPTIVE,
 };
 
 /* RungeKutta4::step (runge_kutta.cu:74-94) */
@@ -294,6 +299,70 @@ struct TabFehlberg5Adaptive : TabFehlberg5Base {
     static constexpr bool ERR = true;
 };
 
+/* NSRK78::step (NSRK78.cu:22-116): 13 stages, 8th-order output; the adaptive form sums the
+   output into the buffer and the error with the weights its ERROR_K call passes (b minus a
+   7th-order embedded solution), exponent 1/8.  Staged path only (phx_staged.hip). */
+struct TabNSRK78Base {
+    static constexpr double c[13] = {0.0, 0.001, 0.1111111111, 0.1666666667, 0.4166666667, 0.5, 0.8333333333, 0.1666666667, 0.6666666667, 0.76, 0.84, 1.0, 1.0};
+    static constexpr double a[13][13] = {
+        {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {1.0 / 1000.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {-491.0 / 81.0, 500.0 / 81.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {1.0 / 24.0, 0.0, 1.0 / 8.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {5.0 / 12.0, 0.0, -25.0 / 16.0, 25.0 / 16.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {1.0 / 20.0, 0.0, 0.0, 1.0 / 4.0, 1.0 / 5.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {-43.0 / 180.0, 0.0, 0.0, 127.0 / 108.0, -329.0 / 135.0, 7.0 / 3.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {27931.0 / 240300.0, 0.0, 0.0, -631.0 / 16020.0, 2459.0 / 6675.0, -3572.0 / 12015.0, 5.0 / 267.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {26114.0 / 12015.0, 0.0, 0.0, -7480.0 / 801.0, 67264.0 / 4005.0, -30640.0 / 2403.0, 1051.0 / 1335.0, 3.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {33096587331.0 / 17382812500.0, 0.0, 0.0, -1828977848.0 / 173828125.0, 62801809904.0 / 4345703125.0, -9389764774.0 / 869140625.0, 6380757669.0 / 8691406250.0, 98417891.0 / 19531250.0, -1692691.0 / 39062500.0, 0.0, 0.0, 0.0, 0.0},
+        {-1456295425347.0 / 2642187500000.0, 0.0, 0.0, 110740056.0 / 34765625.0, -21221682384.0 / 4345703125.0, 58859060169.0 / 13906250000.0, -177381525069.0 / 1529687500000.0, -28942485159.0 / 27812500000.0, -1272297.0 / 312500000.0, 5151.0 / 297616.0, 0.0, 0.0, 0.0},
+        {844300798.0 / 137013275.0, 0.0, 0.0, -9997568.0 / 206035.0, 49636624.0 / 1030175.0, -3358834871.0 / 91067470.0, -40456983.0 / 1813108.0, 495817135.0 / 16647628.0, -149375.0 / 84266.0, 7470703125.0 / 1567431866.0, 1562500000.0 / 72342361.0, 0.0, 0.0},
+        {-26225423.0 / 37371100.0, 0.0, 0.0, 807744.0 / 98345.0, -205584.0 / 37825.0, 5882202.0 / 1278485.0, -84543.0 / 432718.0, -223415.0 / 39338.0, -3625.0 / 6188.0, 292968750.0 / 374084711.0, 0.0, 0.0, 0.0}};
+    static constexpr double b[13] = {4241.0 / 88200.0, 0.0, 0.0, 0.0, 0.0, 9946.0 / 23205.0, -10449.0 / 1925.0, 415449.0 / 1573075.0, -2025.0 / 5096.0, 48828125.0 / 44900856.0, 48828125.0 / 9843561.0, 463.0 / 12600.0, 0.0};
+    static constexpr double e[13] = {4241.0 / 88200.0 - 3799.0 / 79800.0, 0.0, 0.0, 0.0, 0.0, 9946.0 / 23205.0 - 538.0 / 1365.0, -10449.0 / 1925.0 - 351.0 / 1925.0, 415449.0 / 1573075.0 - 4149.0 / 15575.0, -2025.0 / 5096.0 - (-45.0 / 392.0), 48828125.0 / 44900856.0 - 48828125.0 / 284372088.0, 48828125.0 / 9843561.0 - 0.0, 463.0 / 12600.0 - 0.0, 0.0 - 221.0 / 4200.0};
+};
+struct TabNSRK78 : TabNSRK78Base {
+    static constexpr int S = 13, NB = 13;
+    static constexpr bool ERR = false;
+};
+struct TabNSRK78Adaptive : TabNSRK78Base {
+    static constexpr int S = 13, NB = 13;
+    static constexpr bool ERR = true;
+};
+
+/* DormandPrince85::step (dormand_prince.cu:128-205): 12 stages, the coefficients of the
+   reference's sequence.  The reference swaps its two branches (its fixed step only sums into
+   the buffer, its adaptive step finalises in place and adapts on an error it never computed);
+   here the fixed step applies b, and the adaptive step estimates its error with b - b5, b5 the
+   5th-order weights its ERROR_K call passes, exponent 1/4 as passed to adaptTimeStep.  Staged
+   path only. */
+struct TabDP85Base {
+    static constexpr double c[12] = {0.0, 0.05260015196, 0.07890022794, 0.1183503419, 0.2816496581, 0.3333333333, 0.25, 0.3076923077, 0.6512820513, 0.6, 0.8571428571, 1.0};
+    static constexpr double a[12][12] = {
+        {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {5.260015195876773e-2, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {1.9725056984537899e-2, 5.9175170953613698e-2, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {2.9587585476806849e-2, 0.0, 8.8762756430420548e-2, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {2.4136513415926669e-1, 0.0, -8.8454947932828610e-1, 9.2483400326179200e-1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {3.7037037037037037e-2, 0.0, 0.0, 1.7082860872947387e-1, 1.2546768756682243e-1, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {3.7109375e-2, 0.0, 0.0, 1.7025221101954404e-1, 6.0216538980455961e-2, -1.7578125e-2, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {3.7092000118504793e-2, 0.0, 0.0, 1.7038392571223999e-1, 1.0726203044637328e-1, -1.5319437748624402e-2, 8.2737891638140229e-3, 0.0, 0.0, 0.0, 0.0, 0.0},
+        {6.2411095871607572e-1, 0.0, 0.0, -3.3608926294469413e+0, -8.6821934684172601e-1, 2.7592099699446708e+1, 2.0154067550477893e+1, -4.3489884181069959e+1, 0.0, 0.0, 0.0, 0.0},
+        {4.7766253643826437e-1, 0.0, 0.0, -2.4881146199716676e+0, -5.9029082683684300e-1, 2.1230051448181194e+1, 1.5279233632882424e+1, -3.3288210968984863e+1, -2.0331201708508626e-2, 0.0, 0.0, 0.0},
+        {-9.3714243008598733e-1, 0.0, 0.0, 5.1863724288440637e+0, 1.0914373489967296e+0, -8.1497870107469261e+0, -1.8520065659996960e+1, 2.2739487099350504e+1, 2.4936055526796524e+0, -3.0467644718982195e+0, 0.0, 0.0},
+        {2.2733101475165382e+0, 0.0, 0.0, -1.0534495466737250e+1, -2.0008720582248625e+0, -1.7958931863118799e+1, 2.7948884529419960e+1, -2.8589982771350237e+0, -8.8728569335306295e+0, 1.2360567175794303e+1, 6.4339274601576353e-1, 0.0}};
+    static constexpr double b[12] = {5.4293734116568762e-2, 0.0, 0.0, 0.0, 0.0, 4.4503128927524089e+0, 1.8915178993145004e+0, -5.8012039600105848e+0, 3.1116436695781989e-1, -1.5216094966251608e-1, 2.0136540080403035e-1, 4.4710615727772591e-2};
+    static constexpr double e[12] = {5.4293734116568762e-2 - (4.1173689122373888e-2), 0.0, 0.0, 0.0, 0.0, 4.4503128927524089e+0 - (5.6754693391286133e+0), 1.8915178993145004e+0 - (2.3872768489717506e+0), -5.8012039600105848e+0 - (-7.4655811424655713e+0), 3.1116436695781989e-1 - (6.6149321570779360e-1), -1.5216094966251608e-1 - (-4.8634006837553356e-1), 2.0136540080403035e-1 - (1.1944219431891464e-1), 4.4710615727772591e-2 - (6.7065923591658886e-2)};
+};
+struct TabDP85 : TabDP85Base {
+    static constexpr int S = 12, NB = 12;
+    static constexpr bool ERR = false;
+};
+struct TabDP85Adaptive : TabDP85Base {
+    static constexpr int S = 12, NB = 12;
+    static constexpr bool ERR = true;
+};
+
 /* Number of stages the fused kernel evaluates (= its halo in cells). */
 inline int rk_tableau_stages(RkTableau t) {
     switch (t) {
@@ -320,8 +389,72 @@ inline int rk_tableau_stages(RkTableau t) {
         case RkTableau::FEHLBERG5:
         case RkTableau::FEHLBERG5_ADAPTIVE: return 6;
         case RkTableau::DP45_ADAPTIVE: return 7;
+        case RkTableau::DP85:
+        case RkTableau::DP85_ADAPTIVE: return 12;
+        case RkTableau::NSRK78:
+        case RkTableau::NSRK78_ADAPTIVE: return 13;
     }
-    return 7;
+    return 13;
+}
+
+/* The fused kernels keep every stage in registers and serve up to seven stages. */
+inline bool rk_tableau_fused(RkTableau t) { return rk_tableau_stages(t) <= 7; }
+
+/* Runtime form of a tableau, read by the staged path at launch time. */
+constexpr int kStagedMaxStages = 13;
+struct StagedTab {
+    int S = 0, NB = 0;
+    bool err = false;
+    double c[kStagedMaxStages] = {}, b[kStagedMaxStages] = {}, e[kStagedMaxStages] = {};
+    double a[kStagedMaxStages][kStagedMaxStages] = {};
+};
+
+template <class T>
+inline StagedTab staged_from() {
+    StagedTab t;
+    t.S = T::S;
+    t.NB = T::NB;
+    t.err = T::ERR;
+    for (int s = 0; s < T::S; s++) {
+        t.c[s] = T::c[s];
+        for (int j = 0; j < T::S; j++) t.a[s][j] = T::a[s][j];
+        if (T::ERR) t.e[s] = T::e[s];
+    }
+    for (int j = 0; j < T::NB; j++) t.b[j] = T::b[j];
+    return t;
+}
+
+inline StagedTab staged_tableau(RkTableau t) {
+    switch (t) {
+        case RkTableau::RK4: return staged_from<TabRK4>();
+        case RkTableau::DP45_FIXED: return staged_from<TabDP45Fixed>();
+        case RkTableau::DP45_ADAPTIVE: return staged_from<TabDP45Adaptive>();
+        case RkTableau::NEWTON: return staged_from<TabNewton>();
+        case RkTableau::MP: return staged_from<TabMP>();
+        case RkTableau::HEUN2: return staged_from<TabHeun2>();
+        case RkTableau::RALSTON: return staged_from<TabRalston>();
+        case RkTableau::HEUN3: return staged_from<TabHeun3>();
+        case RkTableau::RALSTON3: return staged_from<TabRalston3>();
+        case RkTableau::RK3: return staged_from<TabRK3>();
+        case RkTableau::SSPRK3: return staged_from<TabSSPRK3>();
+        case RkTableau::HW: return staged_from<TabHW>();
+        case RkTableau::FEHLBERG2: return staged_from<TabFehlberg2>();
+        case RkTableau::FEHLBERG2_ADAPTIVE: return staged_from<TabFehlberg2Adaptive>();
+        case RkTableau::RALSTON4: return staged_from<TabRalston4>();
+        case RkTableau::RULE38: return staged_from<TabRule38>();
+        case RkTableau::BOSHA: return staged_from<TabBoSha>();
+        case RkTableau::BOSHA_ADAPTIVE: return staged_from<TabBoShaAdaptive>();
+        case RkTableau::NYSTROEM: return staged_from<TabNystroem>();
+        case RkTableau::CASHKARP: return staged_from<TabCashKarp>();
+        case RkTableau::CASHKARP_ADAPTIVE: return staged_from<TabCashKarpAdaptive>();
+        case RkTableau::FEHLBERG5: return staged_from<TabFehlberg5>();
+        case RkTableau::FEHLBERG5_ADAPTIVE: return staged_from<TabFehlberg5Adaptive>();
+        case RkTableau::NSRK78: return staged_from<TabNSRK78>();
+        case RkTableau::NSRK78_ADAPTIVE: return staged_from<TabNSRK78Adaptive>();
+        case RkTableau::DP85: return staged_from<TabDP85>();
+        case RkTableau::DP85_ADAPTIVE: return staged_from<TabDP85Adaptive>();
+    }
+    return StagedTab{};
 }
 
 /* Iterator registry: the reference's REGISTER_SOLVER keys (solver_factory.hpp:47-60)
@@ -355,6 +488,8 @@ inline const IteratorInfo* iterator_table(int* count) {
         {"CashKarp", RkTableau::CASHKARP, RkTableau::CASHKARP_ADAPTIVE, true, 1.0 / 4.0},
         {"Fehlberg5", RkTableau::FEHLBERG5, RkTableau::FEHLBERG5_ADAPTIVE, true, 1.0 / 4.0},
         {"DP45", RkTableau::DP45_FIXED, RkTableau::DP45_ADAPTIVE, true, 1.0 / 4.0},
+        {"DP85", RkTableau::DP85, RkTableau::DP85_ADAPTIVE, true, 1.0 / 4.0},
+        {"NSRK78", RkTableau::NSRK78, RkTableau::NSRK78_ADAPTIVE, true, 1.0 / 8.0},
     };
     *count = (int)(sizeof(table) / sizeof(table[0]));
     return table;

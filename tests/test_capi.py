@@ -86,7 +86,7 @@ def test_envelope_flags_enable_reservoir_and_pump():
     (["--N", 31, 32], -1),               # odd dimension
     (["--N", 32, 32, "--tstep", 0], -1),  # dt <= 0
     (["--N", 32, 32, "--tstep", 2, "--tmax", 1], -1),  # dt > t_max
-    (["--iterator", "DP85"], -3),         # not provided by this library
+    (["--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss", "time", "iexp", 0, 1, 1], -3),  # complex pump amplitude
     (["--iterator", "NoSuchKey"], -3),    # unknown key (solver_factory.hpp:59)
     (["--imagTime", 1.0, "--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss"], -3),  # real reservoir
     (["--N", "abc", 32], -1),
@@ -105,11 +105,13 @@ def test_adaptive_flag_only_with_adaptive_iterator():
     assert i["adaptive"] == 1 and i["halo"] == 7 and i["tolerance"] == 1e-6 and i["dt_max"] == 0.1
 
 
-# REGISTER_SOLVER keys served by the fused kernel: (stages fixed, stages adaptive or None)
+# REGISTER_SOLVER keys: (stages fixed, stages adaptive or None)
 ITERATORS = {"Newton": (1, None), "MP": (2, None), "Heun2": (2, None), "Ralston": (2, None), "Heun3": (3, None),
              "Ralston3": (3, None), "RK3": (3, None), "SSPRK3": (3, None), "HW": (3, None), "RK4": (4, None),
              "Ralston4": (4, None), "Rule38": (4, None), "Nystroem": (6, None), "Fehlberg2": (3, 3), "BoSha": (3, 4),
-             "CashKarp": (6, 6), "Fehlberg5": (6, 6), "DP45": (6, 7)}
+             "CashKarp": (6, 6), "Fehlberg5": (6, 6), "DP45": (6, 7),
+             # more stages than the fused kernel holds: staged path
+             "DP85": (12, 12), "NSRK78": (13, 13)}
 
 
 @pytest.mark.parametrize("key", list(ITERATORS))
@@ -164,7 +166,7 @@ def test_executable_rejects_invalid_input_with_exit_code_1(tmp_path):
     r = subprocess.run([exe, "--N", "64", "64", "--tstep", "-1", "--path", str(tmp_path)], capture_output=True, text=True,
                        timeout=60)
     assert r.returncode == 1
-    r = subprocess.run([exe, "--N", "64", "64", "--iterator", "DP85", "--path", str(tmp_path)], capture_output=True,
+    r = subprocess.run([exe, "--N", "64", "64", "--iterator", "DP87", "--path", str(tmp_path)], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 1
-    assert "DP85" in (r.stdout + r.stderr)
+    assert "DP87" in (r.stdout + r.stderr)
