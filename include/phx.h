@@ -98,6 +98,47 @@ int phx_step(phx_solver* s, int64_t n_steps, int64_t* n_done);
 /* Iterate until t >= t_max.  Replaces the main loop of main.cu:80-117. */
 int phx_run(phx_solver* s, int64_t* n_done);
 
+/* One output interval of the main loop: iterate until t reaches the next multiple of
+   --outEvery (the step that would pass it is shortened to land on it) or t_max; *finished
+   is set to 1 once t >= t_max.  phx_run is a loop over this call.
+   Replaces: the inner while loop of main() (main.cu:84-104). */
+int phx_run_interval(phx_solver* s, int64_t* n_done, int* finished);
+
+/* Envelope sources of a configuration. */
+enum phx_source { PHX_SRC_PUMP = 0, PHX_SRC_POTENTIAL = 1, PHX_SRC_PULSE = 2, PHX_SRC_FFT_MASK = 3 };
+
+/* Number of temporal groups of a source (0 if the source is unset).
+   Replaces: Envelope::groupSize (envelope.hpp:101). */
+int phx_source_groups(phx_solver* s, int source, int* n_groups);
+
+/* Spatial field of one temporal group of a source for component 0 (plus) or 1 (minus),
+   evaluated on the host exactly as at initialisation: full grid, row-major, complex
+   interleaved doubles (2 N_c N_r).  Replaces: Envelope::calculate for one group
+   (envelope.cpp:332-412), i.e. the matrices outputInitialMatrices writes
+   (solver_output_matrices.cu:72-150). */
+int phx_source_field(phx_solver* s, int source, int component, int group, double* out, size_t bytes);
+
+/* Temporal envelope of a group at time t and its oscillator parameters (0 for a constant
+   group, *constant = 1).  Replaces: Envelope::temporal / updateTemporal (envelope.cpp:466-499)
+   as cached by Solver::cacheValues (solver_cache.cu:13-35). */
+int phx_source_temporal(phx_solver* s, int source, int group, double t, double* re, double* im, double* t0,
+                        double* freq, double* sigma, int* constant);
+
+/* File output settings: --path, --output keys (comma separated; default "mat,scalar",
+   system.cpp:65), --historyMatrix x0 x1 y0 y1 increment, --historyTime start every,
+   --fileoutPrecision (system_initialization.cpp:53-63,160-181). */
+typedef struct {
+    char path[512];
+    char keys[256];
+    int32_t history;                  /* --historyMatrix given */
+    int32_t hx0, hx1, hy0, hy1, hinc; /* region [x0, x1) x [y0, y1), every hinc-th cell */
+    int32_t hevery;                   /* every hevery-th output interval */
+    double hstart;                    /* from this time on */
+    int32_t precision;                /* significant digits, -1: stream default */
+    double L_x, L_y;
+} phx_output_config;
+int phx_get_output_config(phx_solver* s, phx_output_config* oc);
+
 /* Apply the k-space filter now (forward FFT, mask/N, inverse FFT on psi).
    Replaces: Solver::applyFFTFilter(true) (solver_fft.cu:67-100). */
 int phx_apply_fft_filter(phx_solver* s);

@@ -32,6 +32,7 @@ EXPORTED = [
     "phx_ensemble_destroy",
     "phx_adjoint_create", "phx_adjoint_forward", "phx_adjoint_backward", "phx_adjoint_set_stream", "phx_adjoint_sync",
     "phx_adjoint_destroy",
+    "phx_run_interval", "phx_source_groups", "phx_source_field", "phx_source_temporal", "phx_get_output_config",
 ]
 PHX_ENS_RESERVOIR, PHX_ENS_PUMP, PHX_ENS_POTENTIAL = 1, 2, 4
 

@@ -470,6 +470,20 @@ std::string parse_config(SystemConfig& c, const std::vector<std::string>& argv_i
         }
         c.output_keys.push_back(s.substr(prev));
     }
+    if ((index = A.find({"fileout-precision", "fileoutPrecision"}, 0, "--")) != -1)
+        c.fileout_precision = (int)A.num("fileout_precision", ++index);
+    if ((index = A.find({"historyMatrix", "hM"}, 0, "--")) != -1) {
+        c.history_x0 = (int)A.num("history_matrix_start_x", ++index);
+        c.history_x1 = (int)A.num("history_matrix_end_x", index);
+        c.history_y0 = (int)A.num("history_matrix_start_y", index);
+        c.history_y1 = (int)A.num("history_matrix_end_y", index);
+        c.history_increment = (int)A.num("history_matrix_output_increment", index);
+        c.history = true;
+    }
+    if ((index = A.find({"historyTime", "hT"}, 0, "--")) != -1) {
+        c.history_start = A.num("history_time", ++index);
+        c.history_every = (int)A.num("history_time_every", index);
+    }
     if ((index = A.find({"N", "gridsize"}, 0, "--")) != -1) {
         c.N_c = (int)A.num("N_c", ++index);
         c.N_r = (int)A.num("N_r", index);

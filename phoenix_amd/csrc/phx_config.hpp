@@ -83,6 +83,12 @@ struct SystemConfig {
          use_stochastic = false;
     std::string path = "data/";
     std::vector<std::string> output_keys = {"mat", "scalar"};
+    /* --historyMatrix x0 x1 y0 y1 increment, --historyTime start every, --fileoutPrecision
+       (system_initialization.cpp:160-181) */
+    bool history = false;
+    int history_x0 = 0, history_x1 = -1, history_y0 = 0, history_y1 = -1, history_increment = 1, history_every = 1;
+    double history_start = 0.0;
+    int fileout_precision = -1;
     /* custom parameters (custom_rhs.h) */
 #define PHX_X(name, def) double name = def;
     PHX_CUSTOM_PARAMETERS(PHX_X)

@@ -85,6 +85,7 @@ struct phx_solver {
     /* time bookkeeping in the build's precision (system_parameters.hpp:33) */
     double t = 0, dt = 0, fft_cached_t = 0;
     int64_t iteration = 0, rejected = 0;
+    int64_t out_iter = 1; /* output intervals started (main.cu:64, out_every_iterations) */
     phx::Rk4Tune tune; /* launch shape of the fused RK4 kernel */
     bool tuned = false;
     /* captured run of kGraphSteps fused RK4 launches (replayed while nothing changes between steps) */
@@ -867,17 +868,18 @@ int phx_step(phx_solver* s, int64_t n_steps, int64_t* n_done) {
 
 /* main.cu:80-117 (render disabled): iterate in chunks of output_every, shortening the
    step that would overshoot the next output time. */
-int phx_run(phx_solver* s, int64_t* n_done) {
-    if (!s || !s->initialized) return fail(PHX_ERR_STATE, "phx_run before phx_initialize");
+int phx_run_interval(phx_solver* s, int64_t* n_done, int* finished) {
+    if (!s || !s->initialized) return fail(PHX_ERR_STATE, "phx_run_interval before phx_initialize");
     auto& c = s->cfg;
     int64_t done = 0;
-    int64_t out_iter = 1;
-    while (s->t < c.t_max) {
-        while (s->t < out_iter * c.output_every) {
+    if (s->t < c.t_max) {
+        /* main.cu:84-104: iterate up to the next output time; the step that would pass it is
+           shortened to land on it and runs with a fixed step */
+        while (s->t < s->out_iter * c.output_every) {
             const double dt = s->dt;
             bool forced = false;
-            if (s->t + s->dt > out_iter * c.output_every) {
-                const double next_dt = out_iter * c.output_every - s->t;
+            if (s->t + s->dt > s->out_iter * c.output_every) {
+                const double next_dt = s->out_iter * c.output_every - s->t;
                 if (next_dt > 0) {
                     s->dt = s->rnd(next_dt);
                     forced = true;
@@ -892,7 +894,25 @@ int phx_run(phx_solver* s, int64_t* n_done) {
             if (r == 0) break;
             done++;
         }
-        out_iter++;
+        s->out_iter++;
+    }
+    if (n_done) *n_done = done;
+    if (finished) *finished = s->t >= c.t_max ? 1 : 0;
+    return PHX_OK;
+}
+
+int phx_run(phx_solver* s, int64_t* n_done) {
+    if (!s || !s->initialized) return fail(PHX_ERR_STATE, "phx_run before phx_initialize");
+    int64_t done = 0;
+    int finished = 0;
+    while (!finished) {
+        int64_t d = 0;
+        const int r = phx_run_interval(s, &d, &finished);
+        done += d;
+        if (r != PHX_OK) {
+            if (n_done) *n_done = done;
+            return r;
+        }
     }
     if (n_done) *n_done = done;
     return PHX_OK;
@@ -948,6 +968,79 @@ int phx_set_time(phx_solver* s, double t, double dt) {
     s->dt = s->rnd(dt);
     s->cfg.t = s->t;
     s->cfg.dt = s->dt;
+    return PHX_OK;
+}
+
+static const phx::Envelope* source_envelope(phx_solver* s, int source) {
+    switch (source) {
+        case PHX_SRC_PUMP: return &s->cfg.pump;
+        case PHX_SRC_POTENTIAL: return &s->cfg.potential;
+        case PHX_SRC_PULSE: return &s->cfg.pulse;
+        case PHX_SRC_FFT_MASK: return &s->cfg.fft_mask;
+    }
+    return nullptr;
+}
+
+int phx_source_groups(phx_solver* s, int source, int* n_groups) {
+    if (!s || !n_groups) return fail(PHX_ERR_INVALID, "null argument");
+    const phx::Envelope* e = source_envelope(s, source);
+    if (!e) return fail(PHX_ERR_INVALID, "unknown source " + std::to_string(source));
+    *n_groups = e->size() > 0 ? e->groupSize() : 0;
+    return PHX_OK;
+}
+
+int phx_source_field(phx_solver* s, int source, int component, int group, double* out, size_t bytes) {
+    if (!s || !out) return fail(PHX_ERR_INVALID, "null argument");
+    const phx::Envelope* e = source_envelope(s, source);
+    if (!e) return fail(PHX_ERR_INVALID, "unknown source " + std::to_string(source));
+    const auto& c = s->cfg;
+    if (component < 0 || component > 1 || group < 0 || group >= e->groupSize())
+        return fail(PHX_ERR_INVALID, "no such component / temporal group");
+    if (bytes != 2 * s->cells() * sizeof(double)) return fail(PHX_ERR_INVALID, "buffer size mismatch");
+    std::vector<std::complex<double>> f(s->cells());
+    const std::string err = e->calculate(f.data(), group, component == 0 ? phx::POL_PLUS : phx::POL_MINUS, c.N_c, c.N_r, c.L_x,
+                                         c.L_y, c.dx, c.dy, source == PHX_SRC_FFT_MASK ? 1.0 : 0.0);
+    if (!err.empty()) return fail(PHX_ERR_INVALID, err);
+    std::memcpy(out, f.data(), bytes);
+    return PHX_OK;
+}
+
+int phx_source_temporal(phx_solver* s, int source, int group, double t, double* re, double* im, double* t0, double* freq,
+                        double* sigma, int* constant) {
+    if (!s) return fail(PHX_ERR_INVALID, "null handle");
+    const phx::Envelope* e = source_envelope(s, source);
+    if (!e) return fail(PHX_ERR_INVALID, "unknown source " + std::to_string(source));
+    if (group < 0 || group >= e->groupSize()) return fail(PHX_ERR_INVALID, "no such temporal group");
+    const auto& T = e->groups[group];
+    const std::complex<double> v = e->temporal(group, t);
+    if (re) *re = v.real();
+    if (im) *im = v.imag();
+    if (t0) *t0 = T.t0;
+    if (freq) *freq = T.freq;
+    if (sigma) *sigma = T.sigma;
+    if (constant) *constant = (T.kind & phx::TMP_CONSTANT) ? 1 : 0;
+    return PHX_OK;
+}
+
+int phx_get_output_config(phx_solver* s, phx_output_config* oc) {
+    if (!s || !oc) return fail(PHX_ERR_INVALID, "null argument");
+    const auto& c = s->cfg;
+    std::memset(oc, 0, sizeof(*oc));
+    std::snprintf(oc->path, sizeof(oc->path), "%s", c.path.c_str());
+    std::string keys;
+    for (size_t i = 0; i < c.output_keys.size(); i++) keys += (i ? "," : "") + c.output_keys[i];
+    std::snprintf(oc->keys, sizeof(oc->keys), "%s", keys.c_str());
+    oc->history = c.history;
+    oc->hx0 = c.history_x0;
+    oc->hx1 = c.history_x1 < 0 ? c.N_c : c.history_x1;
+    oc->hy0 = c.history_y0;
+    oc->hy1 = c.history_y1 < 0 ? c.N_r : c.history_y1;
+    oc->hinc = std::max(1, c.history_increment);
+    oc->hevery = std::max(1, c.history_every);
+    oc->hstart = c.history_start;
+    oc->precision = c.fileout_precision;
+    oc->L_x = c.L_x;
+    oc->L_y = c.L_y;
     return PHX_OK;
 }
 

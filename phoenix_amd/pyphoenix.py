@@ -119,9 +119,21 @@ class pyphoenix:
         env["HIP_VISIBLE_DEVICES"] = str(self.gpu_id)
         process = subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, env=env, universal_newlines=True)
         if self.doprint:
+            # the executable's progress block (printCMD): relayed block by block, as pyphoenix.py:199-210 does
+            keys = ("T =", "Progress:", "Current System:", "Runtime:", "Time per ps:")
+            block = []
             for line in process.stdout:
-                if self.debug or "Runtime:" in line or "T =" in line:
+                if self.debug:
                     sys.stdout.write(line)
+                    continue
+                if "Total Runtime" in line:
+                    sys.stdout.write(line)
+                    continue
+                if any(k in line for k in keys):
+                    block.append(line)
+                if "Time per ps:" in line:
+                    sys.stdout.write("".join(block))
+                    block = []
         self.stdout, self.stderr = process.communicate()
         if process.returncode != 0:
             print("Something went wrong. Please check the error output:")

@@ -46,6 +46,8 @@ def read_txt(path):
         head = f.readline().split()
         N_c, N_r = int(head[2]), int(head[3])
         vals = np.array([float(x) for line in f for x in line.split()])
+    if vals.size == N_c * N_r:  # a real matrix: one block
+        return vals.reshape(N_r, N_c).astype(complex)
     return (vals[:N_c * N_r] + 1j * vals[N_c * N_r:2 * N_c * N_r]).reshape(N_r, N_c)
 
 
@@ -115,3 +117,52 @@ def test_pyphoenix_run_matches_executable(tmp_path):
     e = rel_l2(g_psi, o_psi)
     say(f"  rel L2 psi {e:.3e}")
     assert e <= TOL["fp32"]
+
+
+def test_executable_writes_the_reference_outputs(tmp_path):
+    """Default --output keys ("mat,scalar"): initial matrices (one file per temporal group of a
+    source, with its oscillator header), the final matrices and scalar.txt with one row per
+    output interval; --historyMatrix snapshots under timeoutput/ (solver_cache.cu:7-72,
+    solver_output_matrices.cu:20-150)."""
+    N, L, dt = 32, 15.5, 0.04
+    exe = os.path.join(BIN, "PHOENIX_gpu_fp64")
+    out = tmp_path / "out"
+    t0, sigma, freq = 0.5, 0.3, 4.0
+    args = ["--N", N, N, "--L", L, L, "--boundary", "periodic", "periodic", "--tstep", dt, "--tmax", 1.0,
+            "--outEvery", 0.2, "--fftEvery", 1e9, "--initRandom", 0.1, 5,
+            "--pump", 5.0, "add", 8, 8, 0, 0, "both", 1, "none", "gauss+noDivide",
+            "--pump", 2.0, "add", 4, 4, 3, 0, "both", 1, "none", "gauss+noDivide", "time", "cos", t0, sigma, freq,
+            "--historyMatrix", 0, 16, 0, 16, 2, "--path", str(out)]
+    say("[cli outputs] PHOENIX_gpu_fp64 32x32, outEvery 0.2, two pump groups, history matrix")
+    r = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for line in ("T = ", "Progress:", "Current System: Scalar", "Runtime:", "Time per ps:", "Total Runtime"):
+        assert line in r.stdout, line
+    names = sorted(os.listdir(out))
+    say("  files: " + " ".join(names))
+    for f in ("initial_wavefunction_plus.txt", "initial_reservoir_plus.txt", "pump_plus.txt", "pump_plus_1.txt",
+              "wavefunction_plus.txt", "reservoir_plus.txt", "scalar.txt", "timeoutput"):
+        assert f in names, f
+    dx = L / (N - 1)
+    P0 = O.envelope(N, N, L, L, dx, dx, 5.0, 8, 8, 0, 0, 1.0, 1 | 8)
+    P1 = O.envelope(N, N, L, L, dx, dx, 2.0, 4, 4, 3, 0, 1.0, 1 | 8)
+    assert np.abs(read_txt(out / "pump_plus.txt") - P0).max() < 1e-12 * np.abs(P0).max()
+    assert np.abs(read_txt(out / "pump_plus_1.txt") - P1).max() < 1e-12 * np.abs(P1).max()
+    head = open(out / "pump_plus_1.txt").readline()
+    assert f"OSC T0 {t0:g} FREQ {freq:g} SIGMA {sigma:g}" in head, head
+    # scalar.txt: keys in std::map order, one row per output interval
+    lines = open(out / "scalar.txt").read().split("\n")
+    cols = lines[0].split()
+    assert cols == ["index", "max_plus", "min_plus", "pump_1", "t"], cols
+    rows = np.array([[float(x) for x in ln.split()] for ln in lines[1:] if ln.strip()])
+    assert rows.shape[0] == 5 and np.allclose(rows[:, 4], [0.2, 0.4, 0.6, 0.8, 1.0], atol=1e-12)
+    tt = rows[:, 4]
+    pump_1 = np.exp(-0.5 * ((tt - t0) / sigma) ** 2) * (1 + np.cos(freq * (tt - t0))) / 2
+    assert np.allclose(rows[:, 3], pump_1, rtol=1e-9, atol=1e-12)
+    psi = read_txt(out / "wavefunction_plus.txt")
+    assert abs(rows[-1, 1] - np.abs(psi).max()) < 1e-9 * np.abs(psi).max()
+    assert abs(rows[-1, 2] - np.abs(psi).min()) < 1e-9 * np.abs(psi).max()
+    hist = sorted(os.listdir(out / "timeoutput"))
+    assert len([h for h in hist if h.startswith("wavefunction_plus_")]) == 5, hist
+    h = read_txt(out / "timeoutput" / [x for x in hist if x.startswith("wavefunction_plus_1.0")][0])
+    assert h.shape == (8, 8) and np.abs(h - psi[0:16:2, 0:16:2]).max() < 1e-12 * np.abs(psi).max()
