@@ -126,7 +126,8 @@ int phx_source_temporal(phx_solver* s, int source, int group, double t, double* 
 
 /* File output settings: --path, --output keys (comma separated; default "mat,scalar",
    system.cpp:65), --historyMatrix x0 x1 y0 y1 increment, --historyTime start every,
-   --fileoutPrecision (system_initialization.cpp:53-63,160-181). */
+   --fileoutPrecision (system_initialization.cpp:53-63,160-181), --outputFileType
+   (filehandler.cpp:49-69). */
 typedef struct {
     char path[512];
     char keys[256];
@@ -136,6 +137,7 @@ typedef struct {
     double hstart;                    /* from this time on */
     int32_t precision;                /* significant digits, -1: stream default */
     double L_x, L_y;
+    char filetype[8];                 /* --outputFileType: "txt", "npy", "ash" or "mat" */
 } phx_output_config;
 int phx_get_output_config(phx_solver* s, phx_output_config* oc);
 

@@ -470,6 +470,18 @@ std::string parse_config(SystemConfig& c, const std::vector<std::string>& argv_i
         }
         c.output_keys.push_back(s.substr(prev));
     }
+    if ((index = A.find({"of", "outputFileType"}, 0, "--")) != -1) {
+        std::string t = A.str(++index);
+        if (t == "phoenix") t = "ash";
+        if (t == "numpy") t = "npy";
+        if (t == "text") t = "txt";
+        if (t == "matlab") t = "mat";
+        if (t != "ash" && t != "npy" && t != "txt" && t != "mat") {
+            std::fprintf(stderr, "[phx] Invalid output file type '%s'. Defaulting to 'txt'...\n", t.c_str());
+            t = "txt";
+        }
+        c.output_filetype = t;
+    }
     if ((index = A.find({"fileout-precision", "fileoutPrecision"}, 0, "--")) != -1)
         c.fileout_precision = (int)A.num("fileout_precision", ++index);
     if ((index = A.find({"historyMatrix", "hM"}, 0, "--")) != -1) {

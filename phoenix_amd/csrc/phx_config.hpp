@@ -89,6 +89,7 @@ struct SystemConfig {
     int history_x0 = 0, history_x1 = -1, history_y0 = 0, history_y1 = -1, history_increment = 1, history_every = 1;
     double history_start = 0.0;
     int fileout_precision = -1;
+    std::string output_filetype = "txt"; /* --outputFileType txt|npy|ash|mat (filehandler.cpp:49-69) */
     /* custom parameters (custom_rhs.h) */
 #define PHX_X(name, def) double name = def;
     PHX_CUSTOM_PARAMETERS(PHX_X)

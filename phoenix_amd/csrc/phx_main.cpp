@@ -34,7 +34,9 @@ namespace {
 
 struct Out {
     phx_output_config oc;
-    std::string path; /* --path, ending in '/' */
+    std::string path;  /* --path, ending in '/' */
+    std::string ftype; /* txt, npy, ash, mat */
+    bool fp32 = false; /* binary files hold the build's precision */
     phx_info in;
     std::vector<std::string> keys;
     std::map<std::string, std::vector<double>> scalars; /* cacheValues: ordered by key, like std::map there */
@@ -66,18 +68,87 @@ std::string header(const Out& o, int nc, int nr, double Lx, double Ly, double t,
     return h.str();
 }
 
-/* outputMatrixToFile, txt form: header, then real parts and imaginary parts of the region
-   [x0, x1) x [y0, y1), every inc-th cell (filehandler.cpp:282-307) */
+/* Binary forms (simple_binary.hpp): npy ("\x93NUMPY" magic) and ash ("\x93PHOENIX" magic) share
+   the NumPy header layout -- version 1.0, little-endian dtype ('<f4', '<f8', '<c8', '<c16'), C
+   order, shape (rows, cols), the header text padded with spaces to (10 + length) % 16 == 15 and
+   ended by a newline (simple_binary.hpp:51-84) -- followed by the raw data, complex values
+   interleaved; mat is the reference's MATLAB-v4-style layout: a 128-byte text header, five int32
+   (type 0, rows, cols, complex flag, name length), the name "mat", then the real block and (if
+   complex) the imaginary block, column-major (simple_binary.hpp:145-262). */
+template <typename real>
+void write_binary(FILE* f, const std::string& ftype, const std::vector<real>& re, const std::vector<real>& im, bool cplx,
+                  int rows, int cols) {
+    const size_t n = (size_t)rows * cols;
+    if (ftype == "mat") {
+        char head[128] = {0};
+        std::snprintf(head, sizeof head, "%s",
+                      cplx ? "MATLAB 4.0 MAT-file, Created by PHOENIX::Output (complex)" : "MATLAB 4.0 MAT-file, Created by PHOENIX::Output");
+        std::fwrite(head, 1, 128, f);
+        const int32_t ah[5] = {0, rows, cols, cplx ? 1 : 0, 4};
+        std::fwrite(ah, sizeof(int32_t), 5, f);
+        std::fwrite("mat", 1, 4, f);
+        std::vector<real> col(n);
+        for (int part = 0; part < (cplx ? 2 : 1); part++) {
+            const std::vector<real>& src = part == 0 ? re : im;
+            for (int j = 0; j < cols; j++)
+                for (int i = 0; i < rows; i++) col[(size_t)j * rows + i] = src[(size_t)i * cols + j];
+            std::fwrite(col.data(), sizeof(real), n, f);
+        }
+        return;
+    }
+    std::string descr = std::string("<") + (cplx ? (sizeof(real) == 4 ? "c8" : "c16") : (sizeof(real) == 4 ? "f4" : "f8"));
+    std::string h = "{'descr': '" + descr + "', 'fortran_order': False, 'shape': (" + std::to_string(rows) + ", " +
+                    std::to_string(cols) + "), }";
+    while ((10 + h.size()) % 16 != 15) h += ' ';
+    h += '\n';
+    const std::string magic = ftype == "npy" ? "NUMPY" : "PHOENIX";
+    std::fputc(0x93, f);
+    std::fwrite(magic.data(), 1, magic.size(), f);
+    std::fputc(0x01, f);
+    std::fputc(0x00, f);
+    std::fputc((int)(h.size() & 0xFF), f);
+    std::fputc((int)((h.size() >> 8) & 0xFF), f);
+    std::fwrite(h.data(), 1, h.size(), f);
+    if (!cplx) {
+        std::fwrite(re.data(), sizeof(real), n, f);
+        return;
+    }
+    std::vector<real> il(2 * n);
+    for (size_t i = 0; i < n; i++) {
+        il[2 * i] = re[i];
+        il[2 * i + 1] = im[i];
+    }
+    std::fwrite(il.data(), sizeof(real), il.size(), f);
+}
+
+/* outputMatrixToFile (filehandler.cpp:233-398): the region [x0, x1) x [y0, y1), every inc-th cell;
+   txt: header, then the real parts and (complex matrices) the imaginary parts (filehandler.cpp:282-307) */
 void write_matrix(const Out& o, const std::string& name, const std::vector<double>& z, int x0, int x1, int y0, int y1,
                   int inc, const std::string& head, bool cplx = true) {
-    const std::string file = o.path + name + ".txt";
-    FILE* f = std::fopen(file.c_str(), "w");
+    const std::string file = o.path + name + "." + o.ftype;
+    FILE* f = std::fopen(file.c_str(), o.ftype == "txt" ? "w" : "wb");
     if (!f) {
         std::fprintf(stderr, "Unable to write '%s'\n", file.c_str());
         return;
     }
-    std::fputs(head.c_str(), f);
     const int N_c = o.in.N_c;
+    if (o.ftype != "txt") {
+        const int cols = (x1 - x0 + inc - 1) / inc, rows = (y1 - y0 + inc - 1) / inc;
+        std::vector<double> re, im;
+        for (int r = y0; r < y1; r += inc)
+            for (int c = x0; c < x1; c += inc) {
+                re.push_back(z[2 * ((size_t)r * N_c + c)]);
+                im.push_back(z[2 * ((size_t)r * N_c + c) + 1]);
+            }
+        if (o.fp32)
+            write_binary<float>(f, o.ftype, std::vector<float>(re.begin(), re.end()), std::vector<float>(im.begin(), im.end()),
+                                cplx, rows, cols);
+        else
+            write_binary<double>(f, o.ftype, re, im, cplx, rows, cols);
+        std::fclose(f);
+        return;
+    }
+    std::fputs(head.c_str(), f);
     for (int part = 0; part < (cplx ? 2 : 1); part++) /* real matrices: the real block only (filehandler.cpp:375-391) */
         for (int r = y0; r < y1; r += inc) {
             std::string line;
@@ -266,6 +337,8 @@ int run(phx_solver* s) {
     phx_get_info(s, &o.in);
     std::stringstream ks(o.oc.keys);
     for (std::string k; std::getline(ks, k, ',');) o.keys.push_back(k);
+    o.ftype = o.oc.filetype[0] ? o.oc.filetype : "txt";
+    o.fp32 = sizeof(real) == 4;
     o.path = o.oc.path;
     if (o.path.empty()) o.path = "data/";
     if (o.path.back() != '/') o.path += "/";

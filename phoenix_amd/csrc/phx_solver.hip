@@ -1041,6 +1041,7 @@ int phx_get_output_config(phx_solver* s, phx_output_config* oc) {
     oc->precision = c.fileout_precision;
     oc->L_x = c.L_x;
     oc->L_y = c.L_y;
+    std::snprintf(oc->filetype, sizeof(oc->filetype), "%s", c.output_filetype.c_str());
     return PHX_OK;
 }
 

@@ -166,3 +166,44 @@ def test_executable_writes_the_reference_outputs(tmp_path):
     assert len([h for h in hist if h.startswith("wavefunction_plus_")]) == 5, hist
     h = read_txt(out / "timeoutput" / [x for x in hist if x.startswith("wavefunction_plus_1.0")][0])
     assert h.shape == (8, 8) and np.abs(h - psi[0:16:2, 0:16:2]).max() < 1e-12 * np.abs(psi).max()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_executable_binary_output_types(prec, tmp_path):
+    """--outputFileType npy / ash / mat (filehandler.cpp:49-69, simple_binary.hpp): the same matrices
+    as the txt output, in the build's precision."""
+    N, L, dt = 32, 15.5, 0.04
+    exe = os.path.join(BIN, f"PHOENIX_gpu_{prec}")
+    base = ["--N", N, N, "--L", L, L, "--boundary", "periodic", "zero", "--tstep", dt, "--tmax", 0.38,
+            "--outEvery", 1e9, "--fftEvery", 1e9, "--initRandom", 0.1, 9,
+            "--pump", 5.0, "add", 8, 8, 0, 0, "both", 1, "none", "gauss+noDivide"]
+    res = {}
+    for ft in ("txt", "npy", "ash", "mat"):
+        out = tmp_path / ft
+        r = subprocess.run([exe] + [str(a) for a in base + ["--outputFileType", ft, "--path", out]], capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        res[ft] = out
+    say(f"[cli binary outputs] {prec}: " + " ".join(sorted(os.listdir(res['npy']))))
+    psi = read_txt(res["txt"] / "wavefunction_plus.txt")
+    pump = read_txt(res["txt"] / "pump_plus.txt").real
+    ctype, rtype = (np.complex64, np.float32) if prec == "fp32" else (np.complex128, np.float64)
+    a = np.load(res["npy"] / "wavefunction_plus.npy")
+    assert a.dtype == ctype and a.shape == (N, N)
+    assert np.array_equal(a.astype(np.complex128), psi) if prec == "fp64" else np.abs(a - psi).max() < 1e-6 * np.abs(psi).max()
+    b = np.load(res["npy"] / "pump_plus.npy")
+    assert b.dtype == rtype and np.abs(b - pump).max() <= 1e-6 * np.abs(pump).max()
+    # ash: the npy layout with the magic "\x93PHOENIX"
+    raw = open(res["ash"] / "wavefunction_plus.ash", "rb").read()
+    assert raw[:8] == b"\x93PHOENIX" and raw[8:10] == b"\x01\x00"
+    hl = raw[10] | (raw[11] << 8)
+    data = np.frombuffer(raw[12 + hl:], dtype=ctype)
+    assert np.array_equal(data.reshape(N, N), a)
+    # mat: 128-byte text, 5 int32 (type, rows, cols, complex, name length), "mat\0", real then imaginary block,
+    # column-major
+    raw = open(res["mat"] / "wavefunction_plus.mat", "rb").read()
+    hdr = np.frombuffer(raw[128:148], dtype=np.int32)
+    assert list(hdr) == [0, N, N, 1, 4] and raw[148:152] == b"mat\x00"
+    vals = np.frombuffer(raw[152:], dtype=rtype)
+    re_, im_ = vals[:N * N].reshape(N, N).T, vals[N * N:].reshape(N, N).T
+    assert np.array_equal(re_ + 1j * im_, a)
