@@ -346,9 +346,12 @@ def cpu_baseline(N: int, L: float, seconds: float = 10.0, subgrids: tuple = (4, 
     p = O.params(N, N, L, L, periodic=(True, True), pump=True, subgrids=subgrids)
     psi, n, P = workload_inputs(N, L)
     t0 = time.perf_counter()
-    O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=10)  # probe (also pages the workspace in)
-    probe = (time.perf_counter() - t0) / 10
-    steps = int(min(2000, max(20, seconds / max(probe, 1e-6))))
+    O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=5)  # probes: the difference removes the per-call setup
+    t1 = time.perf_counter()
+    O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=15)
+    t2 = time.perf_counter()
+    probe = max((t2 - t1) - (t1 - t0), 1e-6) / 10
+    steps = int(min(5000, max(20, seconds / probe)))
     t0 = time.perf_counter()
     O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=steps)
     dt = time.perf_counter() - t0

@@ -192,18 +192,133 @@ __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<rea
     }
 }
 
+/* ------------------------------------------------------------------------------------------
+   Direct DFT for lengths that are not powers of two (any even N; the reference validates even
+   grid sizes, system_validate_input.cpp:19-56): out[k] = sum_j x[j] w^((j k) mod N) with
+   w = e^(-+2 pi i / N) from the twiddle table, one output per thread and pass, the sequence in
+   LDS.  O(N) multiply-adds per output value instead of O(log N): a general-size path next to the
+   radix-4 Stockham kernels, which stay the fast path for powers of two. */
+template <typename real, int SIGN>
+__device__ __forceinline__ Cx<real> dft_one(const Cx<real>* x, int stride, const Cx<real>* __restrict__ tw, int N, int k) {
+    real sr = 0, si = 0;
+    int idx = 0;
+    for (int j = 0; j < N; j++) {
+        Cx<real> w = tw[idx];
+        if (SIGN > 0) w.i = -w.i;
+        const Cx<real> v = x[j * stride];
+        sr = phx_fma(v.r, w.r, phx_fma(-v.i, w.i, sr));
+        si = phx_fma(v.r, w.i, phx_fma(v.i, w.r, si));
+        idx += k; /* (j k) mod N, incrementally: idx + k < 2 N */
+        if (idx >= N) idx -= N;
+    }
+    return {sr, si};
+}
+
+template <typename real, int SIGN>
+__global__ __launch_bounds__(kFftThreads) void dft_rows(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw, int N) {
+    extern __shared__ unsigned char lds_raw[];
+    Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
+    typename Vec2<real>::type* row = data + (size_t)blockIdx.x * (size_t)N;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) {
+        const auto v = row[i];
+        A[i] = {v.x, v.y};
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < N; k += blockDim.x) {
+        const Cx<real> f = dft_one<real, SIGN>(A, 1, tw, N, k);
+        typename Vec2<real>::type v;
+        v.x = f.r;
+        v.y = f.i;
+        row[k] = v;
+    }
+}
+
+/* CB adjacent columns: forward DFT along the column, the k-space factor (as fft_cols_filter), inverse
+   DFT; two LDS buffers (the transforms are out of place). */
+template <typename real, int CB, bool PROP>
+__global__ __launch_bounds__(kFftThreads) void dft_cols_filter(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw,
+                                                              const real* __restrict__ mask, const Cx<real>* __restrict__ rowf,
+                                                              const Cx<real>* __restrict__ colf, int Nc, int Nr, real Ntot) {
+    extern __shared__ unsigned char lds_raw[];
+    Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
+    Cx<real>* B = A + CB * Nr;
+    const int c0 = fft_xcd_group(blockIdx.x, gridDim.x) * CB;
+    for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
+        const int r = i / CB, j = i - r * CB;
+        const auto v = data[(size_t)r * Nc + c0 + j];
+        A[j * Nr + r] = {v.x, v.y};
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
+        const int k = i / CB, j = i - k * CB;
+        Cx<real> f = dft_one<real, -1>(A + j * Nr, 1, tw, Nr, k);
+        if constexpr (PROP) {
+            f = cmul(cmul(f, colf[c0 + j]), rowf[k]);
+        } else {
+            /* mask of the quadrant-swapped grid (fft_shift_2D, kernel_fft.cuh:15-32) */
+            int rs = k + (Nr >> 1), cs = c0 + j + (Nc >> 1);
+            if (rs >= Nr) rs -= Nr;
+            if (cs >= Nc) cs -= Nc;
+            const real m = mask[(size_t)rs * Nc + cs];
+            f.r = f.r / Ntot * m;
+            f.i = f.i / Ntot * m;
+        }
+        B[j * Nr + k] = f;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
+        const int r = i / CB, j = i - r * CB;
+        const Cx<real> f = dft_one<real, +1>(B + j * Nr, 1, tw, Nr, r);
+        typename Vec2<real>::type v;
+        v.x = f.r;
+        v.y = f.i;
+        data[(size_t)r * Nc + c0 + j] = v;
+    }
+}
+
+/* sizes the direct DFT takes: even, and two column buffers of one column in 64 KiB */
+static bool dft_supported(int N, size_t real_size) { return N >= 2 && N % 2 == 0 && (size_t)N * 4 * real_size <= 65536; }
+
+template <typename real, bool PROP>
+static hipError_t dft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, const void* rowf,
+                                const void* colf, int Nc, int Nr, hipStream_t s) {
+    const auto* twc = (const Cx<real>*)tw_c;
+    const auto* twr = (const Cx<real>*)tw_r;
+    const auto* rf = (const Cx<real>*)rowf;
+    const auto* cf = (const Cx<real>*)colf;
+    const real ntot = (real)((double)Nc * (double)Nr);
+    hipLaunchKernelGGL((dft_rows<real, -1>), dim3(Nr), dim3(kFftThreads), (size_t)Nc * sizeof(Cx<real>), s, psi, twc, Nc);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const size_t per_col = 2 * (size_t)Nr * sizeof(Cx<real>);
+    auto fits = [&](int cb) { return per_col * cb <= 65536 && Nc % cb == 0; };
+    if (fits(4))
+        hipLaunchKernelGGL((dft_cols_filter<real, 4, PROP>), dim3(Nc / 4), dim3(kFftThreads), 4 * per_col, s, psi, twr, mask, rf, cf, Nc, Nr, ntot);
+    else if (fits(2))
+        hipLaunchKernelGGL((dft_cols_filter<real, 2, PROP>), dim3(Nc / 2), dim3(kFftThreads), 2 * per_col, s, psi, twr, mask, rf, cf, Nc, Nr, ntot);
+    else
+        hipLaunchKernelGGL((dft_cols_filter<real, 1, PROP>), dim3(Nc), dim3(kFftThreads), per_col, s, psi, twr, mask, rf, cf, Nc, Nr, ntot);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((dft_rows<real, +1>), dim3(Nr), dim3(kFftThreads), (size_t)Nc * sizeof(Cx<real>), s, psi, twc, Nc);
+    return hipGetLastError();
+}
+
 static int ilog2(int n) {
     int l = 0;
     while ((1 << l) < n) l++;
     return l;
 }
 
-bool fft_supported(int N, size_t real_size) {
+/* lengths the radix-4 Stockham kernels take: powers of two with one LDS row of N complex values
+   and at most max_bfly butterflies per thread */
+static bool stockham_supported(int N, size_t real_size) {
     if (N < 4 || (N & (N - 1))) return false;
-    /* one LDS row of N complex values, and at most max_bfly butterflies per thread */
     const int mb = real_size == 4 ? max_bfly<float>() : max_bfly<double>();
     return (size_t)N * 2 * real_size <= 65536 && N / 4 <= mb * kFftThreads;
 }
+
+bool fft_supported(int N, size_t real_size) { return stockham_supported(N, real_size) || dft_supported(N, real_size); }
 
 /* smallest power of two KB with butterflies <= KB * threads */
 static int bfly_bound(int butterflies) {
@@ -278,16 +393,23 @@ static hipError_t fft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c
     return launch_rows<real, +1>(psi, tw_c, Nc, Nr, s);
 }
 
+/* powers of two on both axes: Stockham kernels; otherwise the direct DFT kernels for both axes */
 template <typename real>
 hipError_t launch_fft_filter(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, int Nc, int Nr,
                              hipStream_t s) {
-    return fft_roundtrip<real, false>(psi, tw_c, tw_r, mask, nullptr, nullptr, Nc, Nr, s);
+    if (stockham_supported(Nc, sizeof(real)) && stockham_supported(Nr, sizeof(real)))
+        return fft_roundtrip<real, false>(psi, tw_c, tw_r, mask, nullptr, nullptr, Nc, Nr, s);
+    if (!fft_supported(Nc, sizeof(real)) || !fft_supported(Nr, sizeof(real))) return hipErrorInvalidValue;
+    return dft_roundtrip<real, false>(psi, tw_c, tw_r, mask, nullptr, nullptr, Nc, Nr, s);
 }
 
 template <typename real>
 hipError_t launch_fft_propagate(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const void* rowf, const void* colf,
                                 int Nc, int Nr, hipStream_t s) {
-    return fft_roundtrip<real, true>(psi, tw_c, tw_r, nullptr, rowf, colf, Nc, Nr, s);
+    if (stockham_supported(Nc, sizeof(real)) && stockham_supported(Nr, sizeof(real)))
+        return fft_roundtrip<real, true>(psi, tw_c, tw_r, nullptr, rowf, colf, Nc, Nr, s);
+    if (!fft_supported(Nc, sizeof(real)) || !fft_supported(Nr, sizeof(real))) return hipErrorInvalidValue;
+    return dft_roundtrip<real, true>(psi, tw_c, tw_r, nullptr, rowf, colf, Nc, Nr, s);
 }
 
 template hipError_t launch_fft_propagate<float>(float2*, const void*, const void*, const void*, const void*, int, int, hipStream_t);

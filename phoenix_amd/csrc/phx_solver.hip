@@ -166,8 +166,9 @@ int initialize_t(phx_solver* s) {
     s->ssfm = c.iterator == "SSFM";
     if (!s->iter && !s->ssfm) return fail(PHX_ERR_UNSUPPORTED, "iterator '" + c.iterator + "' is not provided by this library");
     if (s->ssfm && (!phx::fft_supported(c.N_c, sizeof(real)) || !phx::fft_supported(c.N_r, sizeof(real))))
-        return fail(PHX_ERR_UNSUPPORTED, "SSFM needs power-of-two grid sizes between 4 and " +
-                                             std::to_string(sizeof(real) == 4 ? 8192 : 4096));
+        return fail(PHX_ERR_UNSUPPORTED, std::string("SSFM needs grid sizes of at most ") +
+                                             (sizeof(real) == 4 ? "8192 (powers of two) or 4096" : "4096 (powers of two) or 2048") +
+                                             " points per axis");
     if (c.adaptive && s->iter && s->iter->can_adapt) {
         const int nw = phx::rk_generic_waves(s->iter->adaptive, c.N_c, c.N_r);
         PHX_HIP(hipMalloc((void**)&s->partials, 2 * sizeof(double) * (size_t)nw));
@@ -178,8 +179,9 @@ int initialize_t(phx_solver* s) {
     }
     if (const char* e = std::getenv("PHX_MAX_REJECTIONS")) s->max_rejections = std::atoi(e);
     if (c.use_fft_mask && (!phx::fft_supported(c.N_c, sizeof(real)) || !phx::fft_supported(c.N_r, sizeof(real))))
-        return fail(PHX_ERR_UNSUPPORTED, "--fftMask needs power-of-two grid sizes between 4 and " +
-                                             std::to_string(sizeof(real) == 4 ? 8192 : 4096));
+        return fail(PHX_ERR_UNSUPPORTED, std::string("--fftMask needs grid sizes of at most ") +
+                                             (sizeof(real) == 4 ? "8192 (powers of two) or 4096" : "4096 (powers of two) or 2048") +
+                                             " points per axis");
     /* The staged path serves what the fused kernels do not: more than seven stages, pulses,
        several non-constant temporal groups of a source, complex temporal potential envelopes. */
     const auto complex_temporal = [](const phx::Envelope& e) {

@@ -62,6 +62,33 @@ def test_fft_filter_matches_oracle(prec, shape):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape", [(48, 40), (400, 400), (100, 36), (30, 70), (256, 90), (2048, 6)])
+def test_fft_filter_general_sizes_match_oracle(prec, shape):
+    """Even sizes that are not powers of two (the reference's default grid is 400 x 400) take the
+    direct-DFT kernels; a power of two on one axis only as well."""
+    Nc, Nr = shape
+    say(f"[fft filter, direct DFT] prec={prec} shape={shape}")
+    rng = np.random.default_rng(Nc * 5 + Nr)
+    psi = rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc))
+    mask = rng.random((Nr, Nc))
+    s = make_solver(prec, Nc, Nr)
+    s.set("psi_plus", psi)
+    s.set("fft_mask_plus", mask)
+    s.apply_fft_filter()
+    g = s.get("psi_plus").astype(np.complex128)
+    s.close()
+    ref = O.fft_filter(psi, mask)
+    if prec == "fp64":
+        e = np.abs(g - ref).max() / np.abs(psi).max()
+        say(f"  max abs / max|psi| = {e:.3e}")
+        assert e <= 1e-12
+    else:
+        e = rel_l2(g, ref)
+        say(f"  rel L2 = {e:.3e}")
+        assert e <= 1e-5
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_fft_filter_analytic(prec):
     """mask = 1 is the identity; a mask passing only k = 0 maps psi to its mean."""
     N = 256

@@ -28,7 +28,7 @@ def rel_l2(a, b):
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 @pytest.mark.parametrize("Nc,Nr,tetm,noise", [(64, 64, False, False), (128, 64, False, True), (64, 128, True, False),
-                                             (64, 64, True, True)])
+                                             (64, 64, True, True), (60, 48, False, False), (50, 64, True, True)])
 def test_ssfm_matches_oracle(Nc, Nr, tetm, noise, prec):
     from phoenix_amd._lib import Solver
     L, dt, steps, seed = 31.5, 0.02, 25, 31
