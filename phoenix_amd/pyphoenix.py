@@ -94,13 +94,15 @@ class pyphoenix:
         return False
 
     def _args(self, config):
+        """Command line of a config dict as pyphoenix.py:167-186 builds it: one string of
+        "--key value ..." split on whitespace, so keys and values may carry spaces (a key
+        "initialState " repeats a flag, a value "none gauss+ring" is two tokens)."""
         args = []
         for k in config:
-            args.append("--" + str(k))
-            if isinstance(config[k], list):
-                args += [str(e) for e in config[k]]
-            else:
-                args.append(str(config[k]))
+            args += ("--" + str(k)).split()
+            values = config[k] if isinstance(config[k], list) else [config[k]]
+            for e in values:
+                args += str(e).split()
         args += ["--path", self.result_path, "--path", "."]
         if not self.sfml:
             args.append("-nosfml")

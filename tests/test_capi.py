@@ -170,3 +170,24 @@ def test_executable_rejects_invalid_input_with_exit_code_1(tmp_path):
                        text=True, timeout=60)
     assert r.returncode == 1
     assert "DP87" in (r.stdout + r.stderr)
+
+
+def test_pyphoenix_config_dict_tokenises_like_the_reference(tmp_path):
+    """pyphoenix builds one '--key value ...' string and splits it on whitespace (pyphoenix.py:167-186):
+    a key with a trailing space repeats a flag, a value holding a space is two tokens.  The example-3 style
+    config of the reference's examples then parses (TE-TM, two initial-state envelopes, a ring pump)."""
+    from phoenix_amd.pyphoenix import pyphoenix
+    p = pyphoenix(precision="fp32", tetm=True, result_path=str(tmp_path), doprint=False)
+    config = {"N": [64, 64], "L": [100, 100], "tmax": 5, "outEvery": 5, "fftEvery": 5, "deltaLT": 0.000025,
+              "boundary": ["zero", "zero"], "initRandom": [1E-7, 17],
+              "initialState": [1, "add", 3, 3, 0, 0, "plus", 1, 0, "gauss+noDivide+ring"],
+              "initialState ": [1, "add", 3, 3, 0, 0, "minus", 1, -4, "gauss+noDivide+ring"],
+              "pump": [100, "add", 3, 3, 0, 0, "both", 1, "none gauss+noDivide+ring"], "threads": 2}
+    args = p._args(config)
+    assert args.count("--initialState") == 2 and "--initialState " not in args
+    i = args.index("--pump")
+    assert args[i + 9:i + 11] == ["none", "gauss+noDivide+ring"]
+    assert args[-4:] == ["--path", ".", "-nosfml", "-tetm"]
+    s = make(args, prec="fp32")
+    info = s.info
+    assert info["tetm"] == 1 and info["use_pump"] == 1 and (info["N_c"], info["periodic_x"]) == (64, 0)
