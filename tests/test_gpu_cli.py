@@ -207,3 +207,48 @@ def test_executable_binary_output_types(prec, tmp_path):
     vals = np.frombuffer(raw[152:], dtype=rtype)
     re_, im_ = vals[:N * N].reshape(N, N).T, vals[N * N:].reshape(N, N).T
     assert np.array_equal(re_ + 1j * im_, a)
+
+
+def test_reference_example_2_config_through_pyphoenix(tmp_path):
+    """The pyphoenix config of the reference's example 2 (examples/example_2/example.ipynb: 200 x 200 grid,
+    loaded pump / initial state / initial reservoir / potential, a local super-Gaussian k-space mask applied
+    every step -- a non-power-of-two grid, so the direct-DFT filter) for a few steps, against the oracle
+    stepping RK4 and filtering after every step.  Deviations from the notebook: no random initial noise
+    (its seed is random) and a short t_max."""
+    from phoenix_amd.pyphoenix import pyphoenix
+    N, x_max, m_eff, g_c, gamma_c, R, dt, steps = 200, 20, 0.5 * 5.864E-4, 6E-6, 0.16, 0.01, 0.02, 6
+    dx = x_max / (N - 1)
+    x = -x_max / 2 + dx * np.arange(N)
+    X, Y = np.meshgrid(x, x)
+    r = np.sqrt(X ** 2 + Y ** 2)
+    P = 20.0 * np.exp(-((r - 4.0) / 1.5) ** 2)
+    V = 2e-3 * (np.exp(-((X - 2) ** 2 + Y ** 2) / 4.0) - np.exp(-((X + 2) ** 2 + Y ** 2) / 4.0))
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        write_txt(tmp_path / "pump_plus.txt", P.astype(complex), x_max)
+        write_txt(tmp_path / "potential_plus.txt", V.astype(complex), x_max)
+        config = {"N": [N, N], "L": [x_max, x_max], "meff": m_eff, "tmax": (steps - 0.5) * dt, "tstep": dt,
+                  "outEvery": 1e9, "gc": g_c, "gr": 2 * g_c, "R": R, "gammaR": 1.5 * gamma_c, "gammaC": gamma_c,
+                  "boundary": ["periodic", "periodic"], "fftEvery": 0.001,
+                  "fftMask": [1.0, "add", 0.2, 0.2, 0, 0, "both", 10, "none", "gauss+local+noDivide"],
+                  "pump": ["load", "pump_plus.txt", "1", "add", "both"],
+                  "initialState": ["load", "pump_plus.txt", 1, "add", "both"],
+                  "initialReservoir": ["load", "pump_plus.txt", 1, "add", "both"],
+                  "potential": ["load", "potential_plus.txt", 1, "add", "both"]}
+        say("[reference example 2] 200x200 fp32 via pyphoenix, k-space mask every step")
+        p = pyphoenix(precision="fp32", result_path=str(tmp_path / "out"), doprint=False)
+        assert p.run(config) == 0, p.stderr
+    finally:
+        os.chdir(cwd)
+    g = read_txt(tmp_path / "out" / "wavefunction_plus.txt")
+    mask = O.envelope(N, N, x_max, x_max, dx, dx, 1.0, 0.2, 0.2, 0, 0, 10.0, 1 | 8 | 16).real
+    par = O.params(N, N, x_max, x_max, periodic=(True, True), pump=True, potential=True, m_eff=m_eff, g_c=g_c,
+                   g_r=2 * g_c, R=R, gamma_r=1.5 * gamma_c, gamma_c=gamma_c)
+    psi, n = P.astype(complex), P.copy()
+    for _ in range(steps):
+        psi, n = O.rk4_run(par, psi, n, pump=P, pot=V, dt=dt, steps=1)
+        psi = O.fft_filter(psi, mask)
+    e = rel_l2(g, psi)
+    say(f"  rel L2 psi {e:.3e}")
+    assert e <= 1e-5
