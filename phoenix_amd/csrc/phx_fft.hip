@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstdlib>
+
 #include "custom_rhs.h"
 #include "phx_kernels.h"
 
@@ -29,8 +31,9 @@ namespace phx {
 
 constexpr int kFftThreads = 256;
 
+/* aligned to its full size, so an element moves with one 64-bit (FP32) / 128-bit (FP64) LDS access */
 template <typename real>
-struct Cx {
+struct alignas(2 * sizeof(real)) Cx {
     real r, i;
 };
 
@@ -59,7 +62,7 @@ __device__ void stockham(Cx<real>* buf, const Cx<real>* __restrict__ tw, int N, 
     while (n >= 4) {
         const int m = n >> 2;
         const int stepshift = log2N - log2n; /* twiddle index p * N / n */
-        Cx<real> in[KB][4];
+        Cx<real> in[KB][4], wt[KB][3];
 #pragma unroll
         for (int k = 0; k < KB; k++) {
             const int b = threadIdx.x + k * kFftThreads;
@@ -71,6 +74,10 @@ __device__ void stockham(Cx<real>* buf, const Cx<real>* __restrict__ tw, int N, 
                 in[k][1] = x[q + s * (p + m)];
                 in[k][2] = x[q + s * (p + 2 * m)];
                 in[k][3] = x[q + s * (p + 3 * m)];
+                /* the pass's twiddles are read before the barrier, so their latency overlaps it */
+                wt[k][0] = tw[p << stepshift];
+                wt[k][1] = tw[(2 * p) << stepshift];
+                wt[k][2] = tw[(3 * p) << stepshift];
             }
         }
         __syncthreads();
@@ -82,7 +89,7 @@ __device__ void stockham(Cx<real>* buf, const Cx<real>* __restrict__ tw, int N, 
                 const int p = bb >> log2s, q = bb & (s - 1);
                 Cx<real>* y = buf + seq * N;
                 const Cx<real> a = in[k][0], bv = in[k][1], c = in[k][2], d = in[k][3];
-                Cx<real> w1 = tw[p << stepshift], w2 = tw[(2 * p) << stepshift], w3 = tw[(3 * p) << stepshift];
+                Cx<real> w1 = wt[k][0], w2 = wt[k][1], w3 = wt[k][2];
                 if (SIGN > 0) {
                     w1.i = -w1.i;
                     w2.i = -w2.i;
@@ -120,20 +127,31 @@ __device__ void stockham(Cx<real>* buf, const Cx<real>* __restrict__ tw, int N, 
 
 template <typename real, int SIGN, int KB>
 __global__ __launch_bounds__(kFftThreads) void fft_rows(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw, int N, int log2N) {
-    extern __shared__ unsigned char lds_raw[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
     Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
     typename Vec2<real>::type* row = data + (size_t)blockIdx.x * (size_t)N;
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        const auto v = row[i];
-        A[i] = {v.x, v.y};
+    /* N <= PT * threads (the launcher's KB): every load of the row is issued before the first
+       LDS write; indices past the end read the last element and are not written */
+    constexpr int PT = 4 * KB;
+    typename Vec2<real>::type v[PT];
+#pragma unroll
+    for (int k = 0; k < PT; k++) v[k] = row[min((int)threadIdx.x + k * kFftThreads, N - 1)];
+#pragma unroll
+    for (int k = 0; k < PT; k++) {
+        const int i = threadIdx.x + k * kFftThreads;
+        if (i < N) A[i] = {v[k].x, v[k].y};
     }
     __syncthreads();
     stockham<real, SIGN, KB>(A, tw, N, log2N, 1);
-    for (int i = threadIdx.x; i < N; i += blockDim.x) {
-        typename Vec2<real>::type v;
-        v.x = A[i].r;
-        v.y = A[i].i;
-        row[i] = v;
+#pragma unroll
+    for (int k = 0; k < PT; k++) {
+        const int i = threadIdx.x + k * kFftThreads;
+        if (i < N) {
+            typename Vec2<real>::type o;
+            o.x = A[i].r;
+            o.y = A[i].i;
+            row[i] = o;
+        }
     }
 }
 
@@ -153,42 +171,79 @@ __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<rea
                                                               const real* __restrict__ mask, const Cx<real>* __restrict__ rowf,
                                                               const Cx<real>* __restrict__ colf, int Nc, int Nr, int log2Nr,
                                                               real Ntot) {
-    extern __shared__ unsigned char lds_raw[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
     Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
     const int c0 = fft_xcd_group(blockIdx.x, gridDim.x) * CB;
-    for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
-        const int r = i / CB, j = i - r * CB; /* consecutive threads read consecutive columns */
-        const auto v = data[(size_t)r * Nc + c0 + j];
-        A[j * Nr + r] = {v.x, v.y};
+    /* CB * Nr <= PT * threads (the launcher's KB): all loads issued before the first LDS write;
+       consecutive threads read consecutive columns; indices past the end are clamped, not written */
+    constexpr int PT = 4 * KB;
+    const int cnt = CB * Nr;
+    {
+        typename Vec2<real>::type v[PT];
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int i = min((int)threadIdx.x + k * kFftThreads, cnt - 1);
+            const int r = i / CB, j = i - r * CB;
+            v[k] = data[(size_t)r * Nc + c0 + j];
+        }
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int i = threadIdx.x + k * kFftThreads;
+            const int r = i / CB, j = i - r * CB;
+            if (i < cnt) A[j * Nr + r] = {v[k].x, v[k].y};
+        }
     }
     __syncthreads();
     stockham<real, -1, KB>(A, tw, Nr, log2Nr, CB);
     if constexpr (PROP) {
-        for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
+        /* factor tables read before they are used (all loads in flight together) */
+        Cx<real> rf[PT];
+#pragma unroll
+        for (int k = 0; k < PT; k++) rf[k] = rowf[min((int)threadIdx.x + k * kFftThreads, cnt - 1) / CB];
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int i = threadIdx.x + k * kFftThreads;
             const int r = i / CB, j = i - r * CB;
-            Cx<real>& f = A[j * Nr + r];
-            f = cmul(cmul(f, colf[c0 + j]), rowf[r]);
+            if (i < cnt) {
+                Cx<real>& f = A[j * Nr + r];
+                f = cmul(cmul(f, colf[c0 + j]), rf[k]);
+            }
         }
     } else {
-        /* mask of the quadrant-swapped grid (fft_shift_2D, kernel_fft.cuh:15-32), applied as data / N * mask */
-        for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
-            const int r = i / CB, j = i - r * CB; /* row-major walk: consecutive threads read consecutive mask cells */
+        /* mask of the quadrant-swapped grid (fft_shift_2D, kernel_fft.cuh:15-32), applied as data / N * mask;
+           row-major walk: consecutive threads read consecutive mask cells, all loads issued first */
+        real mk[PT];
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int i = min((int)threadIdx.x + k * kFftThreads, cnt - 1);
+            const int r = i / CB, j = i - r * CB;
             const int rs = (r + (Nr >> 1)) & (Nr - 1);
             const int cs = (c0 + j + (Nc >> 1)) & (Nc - 1);
-            const real m = mask[(size_t)rs * Nc + cs];
-            Cx<real>& f = A[j * Nr + r];
-            f.r = f.r / Ntot * m;
-            f.i = f.i / Ntot * m;
+            mk[k] = mask[(size_t)rs * Nc + cs];
+        }
+#pragma unroll
+        for (int k = 0; k < PT; k++) {
+            const int i = threadIdx.x + k * kFftThreads;
+            const int r = i / CB, j = i - r * CB;
+            if (i < cnt) {
+                Cx<real>& f = A[j * Nr + r];
+                f.r = f.r / Ntot * mk[k];
+                f.i = f.i / Ntot * mk[k];
+            }
         }
     }
     __syncthreads();
     stockham<real, +1, KB>(A, tw, Nr, log2Nr, CB);
-    for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
+#pragma unroll
+    for (int k = 0; k < PT; k++) {
+        const int i = threadIdx.x + k * kFftThreads;
         const int r = i / CB, j = i - r * CB;
-        typename Vec2<real>::type v;
-        v.x = A[j * Nr + r].r;
-        v.y = A[j * Nr + r].i;
-        data[(size_t)r * Nc + c0 + j] = v;
+        if (i < cnt) {
+            typename Vec2<real>::type o;
+            o.x = A[j * Nr + r].r;
+            o.y = A[j * Nr + r].i;
+            data[(size_t)r * Nc + c0 + j] = o;
+        }
     }
 }
 
@@ -216,7 +271,7 @@ __device__ __forceinline__ Cx<real> dft_one(const Cx<real>* x, int stride, const
 
 template <typename real, int SIGN>
 __global__ __launch_bounds__(kFftThreads) void dft_rows(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw, int N) {
-    extern __shared__ unsigned char lds_raw[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
     Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
     typename Vec2<real>::type* row = data + (size_t)blockIdx.x * (size_t)N;
     for (int i = threadIdx.x; i < N; i += blockDim.x) {
@@ -239,7 +294,7 @@ template <typename real, int CB, bool PROP>
 __global__ __launch_bounds__(kFftThreads) void dft_cols_filter(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw,
                                                               const real* __restrict__ mask, const Cx<real>* __restrict__ rowf,
                                                               const Cx<real>* __restrict__ colf, int Nc, int Nr, real Ntot) {
-    extern __shared__ unsigned char lds_raw[];
+    extern __shared__ __align__(16) unsigned char lds_raw[];
     Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
     Cx<real>* B = A + CB * Nr;
     const int c0 = fft_xcd_group(blockIdx.x, gridDim.x) * CB;
@@ -381,7 +436,18 @@ static hipError_t fft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c
     auto fits = [&](int cb) {
         return per_col * cb <= 65536 && Nc % cb == 0 && cb * (Nr / 4) <= max_bfly<real>() * kFftThreads && (cb == 1 || Nc / cb >= 512);
     };
-    if (fits(8))
+    /* PHX_FFT_CB: column block width for tuning runs (taken when it fits LDS and registers) */
+    static const int cb_env = [] { const char* e = std::getenv("PHX_FFT_CB"); return e ? std::atoi(e) : 0; }();
+    auto fits_env = [&](int cb) {
+        return cb_env == cb && per_col * cb <= 65536 && Nc % cb == 0 && cb * (Nr / 4) <= max_bfly<real>() * kFftThreads;
+    };
+    if (fits_env(8))
+        e = launch_cols<real, 8, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
+    else if (fits_env(4))
+        e = launch_cols<real, 4, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
+    else if (fits_env(1))
+        e = launch_cols<real, 1, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
+    else if (fits(8))
         e = launch_cols<real, 8, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
     else if (fits(4))
         e = launch_cols<real, 4, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
