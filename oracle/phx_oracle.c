@@ -1341,12 +1341,21 @@ int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, doubl
             }
             for (int k = 0; k < nc; k++) {
                 /* result = (g_c |psi|^2, -hbar/2 gamma_c) + V a + g_r n + i hbar/2 R n (+ g_pm |psi_o|^2) (- g_c/dV) */
-                double A = p->g_c * nrm[k];
-                if (p->use_potential) A += pot[k][i] * p->pot_amp;
+                double A = p->g_c * nrm[k], B = -p->h_bar_s * 0.5 * p->gamma_c;
+                if (p->src_groups[1] > 0) {
+                    /* potential chain: result += V_g a_g with the amplitude of the step start (:276-279) */
+                    for (int g = 0; g < p->src_groups[1]; g++) {
+                        const double v = p->src_field[1][g][k][i];
+                        A += v * p->src_amp[1][g][0];
+                        B += v * p->src_amp[1][g][1];
+                    }
+                } else if (p->use_potential) {
+                    A += pot[k][i] * p->pot_amp;
+                }
                 A += p->g_r * nv[k];
                 if (p->tetm) A += p->g_pm * nrm[1 - k];
                 if (stoch) A -= p->g_c / p->dV;
-                const double B = -p->h_bar_s * 0.5 * p->gamma_c + p->h_bar_s * 0.5 * p->R * nv[k];
+                B += p->h_bar_s * 0.5 * p->R * nv[k];
                 /* psi * exp((B - i A) dt / hbar) */
                 const double er = B * p->one_over_h_bar_s * dt, ei = -A * p->one_over_h_bar_s * dt;
                 const double m = exp(er), fr = m * cos(ei), fi = m * sin(ei);
@@ -1356,7 +1365,11 @@ int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, doubl
                 if (p->use_reservoir) {
                     double res = -p->gamma_r * nv[k];
                     res -= p->R * nrm[k] * nv[k];
-                    if (p->use_pump) res += pump[k][i] * p->pump_amp;
+                    if (p->src_groups[0] > 0) {
+                        for (int g = 0; g < p->src_groups[0]; g++) res += p->src_field[0][g][k][i] * p->src_amp[0][g][0];
+                    } else if (p->use_pump) {
+                        res += pump[k][i] * p->pump_amp;
+                    }
                     if (stoch) res += p->R * nv[k] / p->dV;
                     nn[k][i] = nv[k] + res * dt;
                 }
@@ -1367,12 +1380,25 @@ int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, doubl
         /* gp_scalar_independent (kernel_gp_compute.cuh:387-440) */
         for (size_t i = 0; i < N; i++) {
             for (int k = 0; k < nc; k++) {
-                if (!stoch) continue;
-                /* scalar: the reservoir of the step start; TE-TM: the new one of the component */
-                const double nr = p->use_reservoir ? (p->tetm ? nn[k][i] : n[0][i]) : 0.0;
-                const double sg = sqrt((p->R * nr + p->gamma_c) / (4.0 * p->dV));
-                psi[k][2 * i] += xi[2 * i] * sg;
-                psi[k][2 * i + 1] += xi[2 * i + 1] * sg;
+                double rr = 0.0, ri = 0.0;
+                /* pulse chain: scalar (dt/hbar) F_g a_g (:392-396), TE-TM (-i dt/hbar) F_g a_g (:409-413) */
+                for (int g = 0; g < p->src_groups[2]; g++) {
+                    const double* F = p->src_field[2][g][k];
+                    const double ar = p->src_amp[2][g][0], ai = p->src_amp[2][g][1];
+                    const double zr = p->one_over_h_bar_s * dt * (F[2 * i] * ar - F[2 * i + 1] * ai);
+                    const double zi = p->one_over_h_bar_s * dt * (F[2 * i] * ai + F[2 * i + 1] * ar);
+                    rr += p->tetm ? zi : zr;
+                    ri += p->tetm ? -zr : zi;
+                }
+                if (stoch) {
+                    /* scalar: the reservoir of the step start; TE-TM: the new one of the component */
+                    const double nr = p->use_reservoir ? (p->tetm ? nn[k][i] : n[0][i]) : 0.0;
+                    const double sg = sqrt((p->R * nr + p->gamma_c) / (4.0 * p->dV));
+                    rr += xi[2 * i] * sg;
+                    ri += xi[2 * i + 1] * sg;
+                }
+                psi[k][2 * i] += rr;
+                psi[k][2 * i + 1] += ri;
             }
             if (p->use_reservoir && !p->tetm) n[0][i] = nn[0][i];
         }

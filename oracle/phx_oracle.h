@@ -121,8 +121,11 @@ int orc_iter_attempt(const orc_params* p, const char* key, double* psi_plus, dou
    (split_step_fourier_method.cu:25-71, gp_scalar_linear_fourier / nonlinear / independent,
    kernel_gp_compute.cuh:244-440).  Transforms: unnormalised 2D DFT of the N_r x N_c
    row-major grid (orc_fft2d).  Pump / potential amplitudes pump_amp / pot_amp (the value at
-   the step start).  With -tetm the reservoirs stay at their input values, as in the
-   reference (its independent step never installs the new TE-TM reservoir). */
+   the step start); source chains (src_groups > 0) replace the single fields, each group at
+   src_amp (the amplitude of the step start): potential V_g a_g (complex a_g), pump P_g Re a_g,
+   pulse (dt/hbar) F_g a_g in the scalar model and (-i dt/hbar) F_g a_g with -tetm.  With -tetm
+   the reservoirs stay at their input values, as in the reference (its independent step never
+   installs the new TE-TM reservoir). */
 int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
                  const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
                  double dt, int steps, int64_t step0);

@@ -260,12 +260,19 @@ struct SsfmArgs {
     typename Vec2<real>::type* psi[2];
     real* n_cur[2]; /* reservoir of the step start (nullptr without reservoir) */
     real* n_nxt[2]; /* reservoir after the nonlinear step */
-    const real* pump[2];
-    const real* pot[2];
+    /* sources as chains of temporal groups (a single field is one group), each group at its
+       temporal amplitude of the step start: pump P_g Re a_g, potential V_g a_g (complex a_g),
+       pulse F_g a_g (complex field and amplitude) */
+    int npump, npot, npulse;
+    const real* pump[2][kStagedMaxGroups];
+    const real* pot[2][kStagedMaxGroups];
+    const typename Vec2<real>::type* pulse[2][kStagedMaxGroups];
+    real pamp[kStagedMaxGroups];
+    typename Vec2<real>::type vamp[kStagedMaxGroups], lamp[kStagedMaxGroups];
     const typename Vec2<real>::type* noise; /* this step's noise field (stochastic only) */
     size_t cells;
     int tetm, stochastic;
-    real dt, pamp, vamp; /* temporal amplitudes of pump and potential at the step start */
+    real dt;
     PhxPhys<real> ph;
 };
 template <typename real>

@@ -96,6 +96,7 @@ struct phx_solver {
     bool graph_off = false;
     /* staged path (phx_staged.hip): padded stage input and K slots, per-group source fields */
     bool staged = false;
+    bool grouped = false; /* one source field per temporal group (staged path, SSFM with chains or pulses) */
     int staged_H = 0;
     size_t staged_padN = 0;
     void* st_y_psi[2] = {nullptr, nullptr};
@@ -189,12 +190,13 @@ int initialize_t(phx_solver* s) {
             if (e.groups[g].kind & phx::TMP_IEXP) return true;
         return false;
     };
-    s->staged = (s->iter && (!phx::rk_tableau_fused(s->iter->fixed) || !phx::rk_tableau_fused(s->iter->adaptive))) ||
-                c.use_pulses || (c.use_pumps && c.pump.groupSize() > 1 && !c.pump.allConstant()) ||
-                (c.use_potentials && c.potential.groupSize() > 1 && !c.potential.allConstant()) ||
-                (c.use_potentials && complex_temporal(c.potential));
-    if (s->staged && s->ssfm)
-        return fail(PHX_ERR_UNSUPPORTED, "--pulse and non-constant temporal groups are not provided with SSFM");
+    s->grouped = c.use_pulses || (c.use_pumps && c.pump.groupSize() > 1 && !c.pump.allConstant()) ||
+                 (c.use_potentials && c.potential.groupSize() > 1 && !c.potential.allConstant()) ||
+                 (c.use_potentials && complex_temporal(c.potential));
+    /* SSFM takes the group fields in its own pointwise kernels (phx_ssfm.hip) */
+    s->staged = !s->ssfm && (s->grouped || (s->iter && (!phx::rk_tableau_fused(s->iter->fixed) ||
+                                                        !phx::rk_tableau_fused(s->iter->adaptive))));
+    if (s->staged) s->grouped = true;
 
     for (int k = 0; k < ncomp; k++) {
         for (int b = 0; b < 2; b++) {
@@ -204,9 +206,9 @@ int initialize_t(phx_solver* s) {
         }
         /* staged path: one field per temporal group (group g at g N); the fused kernels: one
            field, all groups summed (constant envelopes) or the single group */
-        s->st_npump = (s->staged && c.use_reservoir && c.use_pumps) ? c.pump.groupSize() : 0;
-        s->st_npot = (s->staged && c.use_potentials) ? c.potential.groupSize() : 0;
-        s->st_npulse = (s->staged && c.use_pulses) ? c.pulse.groupSize() : 0;
+        s->st_npump = (s->grouped && c.use_reservoir && c.use_pumps) ? c.pump.groupSize() : 0;
+        s->st_npot = (s->grouped && c.use_potentials) ? c.potential.groupSize() : 0;
+        s->st_npulse = (s->grouped && c.use_pulses) ? c.pulse.groupSize() : 0;
         if (s->st_npump > phx::kStagedMaxGroups || s->st_npot > phx::kStagedMaxGroups || s->st_npulse > phx::kStagedMaxGroups)
             return fail(PHX_ERR_UNSUPPORTED, "more than " + std::to_string(phx::kStagedMaxGroups) + " temporal groups of one source");
         if (c.use_reservoir && c.use_pumps)
@@ -576,21 +578,47 @@ int ssfm_step(phx_solver* s, double t_pre) {
     }
     const int ncomp = c.tetm ? 2 : 1;
     phx::SsfmArgs<real> a{};
+    const size_t N = s->cells();
+    /* sources: the group fields (group g at g N) at each group's amplitude of the step start
+       (updateKernelTime, solver_iterator.cu:11-27), or the one summed / single-group field */
+    static const double w0[1] = {0.0};
+    a.npump = s->st_npump ? s->st_npump : (s->pump[0] ? 1 : 0);
+    a.npot = s->st_npot ? s->st_npot : (s->pot[0] ? 1 : 0);
+    a.npulse = s->st_npulse;
+    for (int g = 0; g < a.npump; g++) {
+        if (s->st_npump)
+            a.pamp[g] = (real)c.pump.temporal(g, t_pre).real();
+        else
+            stage_amps<real>(c.pump, t_pre, s->dt, w0, 1, &a.pamp[g]);
+    }
+    for (int g = 0; g < a.npot; g++) {
+        if (s->st_npot) {
+            const std::complex<real> z(c.potential.temporal(g, t_pre));
+            a.vamp[g].x = z.real();
+            a.vamp[g].y = z.imag();
+        } else {
+            stage_amps<real>(c.potential, t_pre, s->dt, w0, 1, &a.vamp[g].x);
+            a.vamp[g].y = 0;
+        }
+    }
+    for (int g = 0; g < a.npulse; g++) {
+        const std::complex<real> z(c.pulse.temporal(g, t_pre));
+        a.lamp[g].x = z.real();
+        a.lamp[g].y = z.imag();
+    }
     for (int k = 0; k < ncomp; k++) {
         a.psi[k] = (v2*)s->psi[k][s->cur];
         a.n_cur[k] = c.use_reservoir ? (real*)s->n[k][s->cur] : nullptr;
         a.n_nxt[k] = c.use_reservoir ? (real*)s->n[k][s->cur ^ 1] : nullptr;
-        a.pump[k] = (const real*)s->pump[k];
-        a.pot[k] = (const real*)s->pot[k];
+        for (int g = 0; g < a.npump; g++) a.pump[k][g] = (const real*)s->pump[k] + (size_t)g * N;
+        for (int g = 0; g < a.npot; g++) a.pot[k][g] = (const real*)s->pot[k] + (size_t)g * N;
+        for (int g = 0; g < a.npulse; g++) a.pulse[k][g] = (const v2*)s->st_pulse[k] + (size_t)g * N;
     }
     a.noise = (const v2*)s->noise;
-    a.cells = s->cells();
+    a.cells = N;
     a.tetm = c.tetm;
     a.stochastic = c.use_stochastic;
     a.dt = (real)s->dt;
-    static const double w0[1] = {0.0};
-    stage_amps<real>(c.pump, t_pre, s->dt, w0, 1, &a.pamp);
-    stage_amps<real>(c.potential, t_pre, s->dt, w0, 1, &a.vamp);
     a.ph = phx::make_phys<real>(c);
     for (int k = 0; k < ncomp; k++)
         PHX_HIP(phx::launch_fft_propagate<real>(a.psi[k], s->tw_c, s->tw_r, s->ssfm_rowf, s->ssfm_colf, c.N_c, c.N_r, s->stream));

@@ -4,16 +4,20 @@
  * (launch_fft_propagate, phx_fft.hip).
  *
  *   ssfm_nonlinear    : gp_scalar_nonlinear (kernel_gp_compute.cuh:264-382): per cell
- *                       psi <- psi exp(dt (R n - gamma_c) / 2) exp(-i dt A / hbar),
- *                       A = g_c |psi|^2 + V a_V + g_r n (+ g_pm |psi_other|^2 with TE-TM)
- *                       (- g_c / dV with noise), and the explicit Euler reservoir step
- *                       n' = n + dt (-(gamma_r + R |psi|^2) n + P a_P (+ R n / dV with noise));
- *                       the TE-TM splitting delta_LT is not part of it (kernel_gp_compute.cuh:320,357).
- *   ssfm_independent  : gp_scalar_independent (kernel_gp_compute.cuh:387-440): psi += xi
- *                       sqrt((R n + gamma_c) / (4 dV)); the scalar model takes the reservoir of
- *                       the step start and then installs the new one; the TE-TM model takes
- *                       the new reservoir of each component and keeps the old one (the reference
- *                       never copies it back, so its TE-TM reservoirs do not evolve under SSFM).
+ *                       psi <- psi exp(dt (R n - gamma_c) / 2 + dt B / hbar) exp(-i dt A / hbar),
+ *                       A = g_c |psi|^2 + sum_g V_g Re a_g + g_r n (+ g_pm |psi_other|^2 with TE-TM)
+ *                       (- g_c / dV with noise), B = sum_g V_g Im a_g, and the explicit Euler
+ *                       reservoir step n' = n + dt (-(gamma_r + R |psi|^2) n + sum_g P_g Re a_g
+ *                       (+ R n / dV with noise)); the TE-TM splitting delta_LT is not part of it
+ *                       (kernel_gp_compute.cuh:320,357).
+ *   ssfm_independent  : gp_scalar_independent (kernel_gp_compute.cuh:387-440): psi += the pulse
+ *                       sum_g F_g a_g times dt / hbar in the scalar model and times -i dt / hbar
+ *                       in the TE-TM model (the reference writes the two differently, :392-396
+ *                       and :409-413), plus the noise xi sqrt((R n + gamma_c) / (4 dV)); the scalar
+ *                       model takes the reservoir of the step start and then installs the new one;
+ *                       the TE-TM model takes the new reservoir of each component and keeps the old
+ *                       one (the reference never copies it back, so its TE-TM reservoirs do not
+ *                       evolve under SSFM).
  * One thread per cell, coalesced; the fields are the solver's full-grid arrays.
  */
 #include <hip/hip_runtime.h>
@@ -48,12 +52,16 @@ __global__ __launch_bounds__(256) void ssfm_nonlinear(SsfmArgs<real> a) {
     }
 #pragma unroll
     for (int k = 0; k < NC; k++) {
-        real A = ph.g_c * nrm[k];
-        if (a.pot[k]) A += a.pot[k][i] * a.vamp;
+        real A = ph.g_c * nrm[k], B = 0;
+        for (int g = 0; g < a.npot; g++) {
+            const real v = a.pot[k][g][i];
+            A += v * a.vamp[g].x;
+            B += v * a.vamp[g].y;
+        }
         A += ph.g_r * n[k];
         if (TETM) A += ph.g_pm * nrm[1 - k];
         if (a.stochastic) A -= ph.g_c / ph.dV;
-        const real er = a.dt * (real(0.5) * (ph.R * n[k] - ph.gamma_c));
+        const real er = phx_fma(a.dt * ph.one_over_h_bar_s, B, a.dt * (real(0.5) * (ph.R * n[k] - ph.gamma_c)));
         const real ei = -(A * ph.one_over_h_bar_s * a.dt);
         real sn, cs;
         phx_sincos_dev(ei, sn, cs);
@@ -66,7 +74,7 @@ __global__ __launch_bounds__(256) void ssfm_nonlinear(SsfmArgs<real> a) {
         if (a.n_nxt[k]) {
             real rn = -(ph.gamma_r * n[k]);
             rn -= ph.R * nrm[k] * n[k];
-            if (a.pump[k]) rn += a.pump[k][i] * a.pamp;
+            for (int g = 0; g < a.npump; g++) rn += a.pump[k][g][i] * a.pamp[g];
             if (a.stochastic) rn += ph.R * n[k] / ph.dV;
             a.n_nxt[k][i] = phx_fma(rn, a.dt, n[k]);
         }
@@ -82,15 +90,33 @@ __global__ __launch_bounds__(256) void ssfm_independent(SsfmArgs<real> a) {
     const real n_old = a.n_cur[0] ? a.n_cur[0][i] : real(0);
 #pragma unroll
     for (int k = 0; k < NC; k++) {
+        if (a.npulse == 0 && !a.stochastic) continue;
+        real dr = 0, di = 0;
+        const real f = a.dt * ph.one_over_h_bar_s;
+        for (int g = 0; g < a.npulse; g++) {
+            const auto F = a.pulse[k][g][i];
+            const auto am = a.lamp[g];
+            /* (dt / hbar) F a_g */
+            const real zr = f * phx_fma(F.x, am.x, -(F.y * am.y)), zi = f * phx_fma(F.x, am.y, F.y * am.x);
+            if (TETM) { /* -i (dt / hbar) F a_g */
+                dr += zi;
+                di -= zr;
+            } else {
+                dr += zr;
+                di += zi;
+            }
+        }
         if (a.stochastic) {
             const real nr = TETM ? (a.n_nxt[k] ? a.n_nxt[k][i] : real(0)) : n_old;
             const real sg = phx_sqrt(phx_fma(ph.R, nr, ph.gamma_c) / (real(4) * ph.dV));
             const auto xi = a.noise[i];
-            auto p = a.psi[k][i];
-            p.x = phx_fma(xi.x, sg, p.x);
-            p.y = phx_fma(xi.y, sg, p.y);
-            a.psi[k][i] = p;
+            dr = phx_fma(xi.x, sg, dr);
+            di = phx_fma(xi.y, sg, di);
         }
+        auto p = a.psi[k][i];
+        p.x += dr;
+        p.y += di;
+        a.psi[k][i] = p;
     }
     /* scalar model: the new reservoir becomes current (TE-TM keeps the old one) */
     if (!TETM && a.n_nxt[0]) a.n_cur[0][i] = a.n_nxt[0][i];

@@ -72,3 +72,85 @@ def test_ssfm_matches_oracle(Nc, Nr, tetm, noise, prec):
     errs = {k: rel_l2(out[k], ref[k]) for k in out}
     say("  " + " ".join(f"{k} {e:.2e}" for k, e in errs.items()))
     assert max(errs.values()) <= TOL[prec]
+
+
+def _env(Nc, Nr, L, amp, w, x, y, charge=0.0):
+    dx = L / (Nc - 1)
+    Ly = L * (Nr - 1) / (Nc - 1)
+    return O.envelope(Nc, Nr, L, Ly, dx, dx, amp, w, w, x, y, 1.0, 1 | 8, charge=charge)  # gauss + noDivide
+
+
+def _t_cos(t, t0, sigma, freq):
+    q = (t - t0) / sigma
+    return np.exp(-0.5 * q * q) * (1.0 + np.cos(freq * (t - t0))) / 2.0
+
+
+def _t_iexp(t, t0, sigma, freq):
+    return np.exp(-complex((t - t0) ** 2 / (2.0 * sigma * sigma), freq * (t - t0)))
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("tetm", [False, True])
+def test_ssfm_source_chains_and_pulse_match_oracle(prec, tetm):
+    """SSFM with two pump groups (constant + cos envelope), two potential groups (constant + complex
+    iexp envelope) and a pulse of charge 1 with an iexp envelope: each group at its amplitude of the
+    step start; the pulse enters the independent step as (dt/hbar) F a (scalar) or -i (dt/hbar) F a
+    (TE-TM)."""
+    from phoenix_amd._lib import Solver
+    Nc, Nr, L, dt, steps = 64, 64, 31.5, 0.02, 20
+    tc, ti, tp = (0.1, 0.3, 9.0), (0.2, 0.4, 5.0), (0.15, 0.3, 7.0)
+    say(f"[ssfm chains] tetm={tetm} prec={prec}")
+    rng = np.random.default_rng(61 + tetm)
+    P0 = 1.5 * PTH * np.real(_env(Nc, Nr, L, 1.0, 6.0, 0.0, 0.0))
+    V0 = 2e-3 * np.real(_env(Nc, Nr, L, 1.0, 4.0, 3.0, -2.0))
+    P1 = np.real(_env(Nc, Nr, L, 3.0, 4.0, -4.0, 3.0))
+    V1 = np.real(_env(Nc, Nr, L, 1e-3, 5.0, 1.0, 1.0))
+    F = _env(Nc, Nr, L, 1e-5, 5.0, 2.0, -1.0, charge=1.0)
+    init = {"psi_plus": 0.05 * (rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc))),
+            "n_plus": 0.1 * rng.random((Nr, Nc)), "pump_plus": P0, "potential_plus": V0}
+    if tetm:
+        init.update(psi_minus=0.05 * (rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc))),
+                    n_minus=0.1 * rng.random((Nr, Nc)), pump_minus=0.5 * P0, potential_minus=V0[::-1].copy())
+    args = ["--N", Nc, Nr, "--L", L, L, "--boundary", "periodic", "periodic", "--outEvery", 1e9, "--fftEvery", 1e9,
+            "--iterator", "SSFM", "--tstep", dt, "--tmax", (steps - 0.5) * dt,
+            "--pump", 1.0, "add", 10, 10, 0, 0, "both", 1, "none", "gauss+noDivide",
+            "--pump", 3.0, "add", 4.0, 4.0, -4.0, 3.0, "both", 1, "none", "gauss+noDivide", "time", "cos", *tc,
+            "--potential", 1.0, "add", 10, 10, 0, 0, "both", 1, "none", "gauss+noDivide",
+            "--potential", 1e-3, "add", 5.0, 5.0, 1.0, 1.0, "both", 1, "none", "gauss+noDivide", "time", "iexp", *ti,
+            "--pulse", 1e-5, "add", 5.0, 5.0, 2.0, -1.0, "both", 1, 1, "gauss+noDivide", "time", "iexp", *tp]
+    if tetm:
+        args += ["-tetm"]
+    s = Solver([str(a) for a in args], prec)
+    for k, v in init.items():
+        s.set(k, v)
+    done = s.step(steps + 5)
+    out = {k: s.get(k).astype(np.complex128 if k.startswith("psi") else np.float64) for k in init
+           if k.startswith(("psi", "n_"))}
+    info = s.info
+    s.close()
+    assert done == steps and info["iterator"] == "SSFM" and info["staged"] == 0
+    psi, n = init["psi_plus"], init["n_plus"]
+    pm, nm = init.get("psi_minus"), init.get("n_minus")
+    pump_m = 0.5 * P0 if tetm else None
+    pot_m = init.get("potential_minus")
+    t = 0.0
+    for st in range(steps):
+        p = O.params(Nc, Nr, L, L, tetm=tetm, pump=True, potential=True)
+        O.set_chain(p, "pump", [(P0, pump_m, 1.0, 1.0), (P1, P1 if tetm else None, _t_cos(t, *tc), 0.0)])
+        O.set_chain(p, "potential", [(V0, pot_m, 1.0, 1.0), (V1, V1 if tetm else None, _t_iexp(t, *ti), 0.0)])
+        O.set_chain(p, "pulse", [(F, F if tetm else None, _t_iexp(t, *tp), 0.0)])
+        r = O.ssfm_run(p, psi, n, dt=dt, steps=1, psi_minus=pm, n_minus=nm, step0=st)
+        if tetm:
+            psi, n, pm, nm = r
+        else:
+            psi, n = r
+        t = t + dt
+    ref = {"psi_plus": psi, "n_plus": n, "psi_minus": pm, "n_minus": nm}
+    errs = {k: rel_l2(out[k], ref[k]) for k in out}
+    say("  " + " ".join(f"{k} {e:.2e}" for k, e in errs.items()))
+    assert max(errs.values()) <= TOL[prec]
+    # the pulse and the complex potential amplitude matter at this size
+    p = O.params(Nc, Nr, L, L, tetm=tetm, pump=True, potential=True)
+    r0 = O.ssfm_run(p, init["psi_plus"], init["n_plus"], pump=P0, pot=V0, dt=dt, steps=steps,
+                    psi_minus=init.get("psi_minus"), n_minus=init.get("n_minus"), pump_minus=pump_m, pot_minus=pot_m)
+    assert rel_l2(r0[0], psi) > 1e-4

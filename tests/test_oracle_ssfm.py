@@ -104,3 +104,45 @@ def test_tetm_reservoir_is_not_installed():
     ps = O.params(N, N, L, L, pump=True)
     _, n2 = O.ssfm_run(ps, psi, n, pump=P, dt=dt, steps=3)
     assert np.abs(n2 - n).max() > 1e-4
+
+
+def test_uniform_field_source_chains_and_pulse_closed_form():
+    """Chains of temporal groups at the amplitude of the step start (gp_scalar_nonlinear /
+    gp_scalar_independent, kernel_gp_compute.cuh:264-440): the potential enters as V_g a_g with complex
+    a_g (its imaginary part is a gain dt Im(a) V / hbar), the pump as P_g Re a_g, the pulse as
+    (dt / hbar) F_g a_g in the scalar model."""
+    N, L, dt, steps = 16, 7.5, 0.03, 3
+    p = O.params(N, N, L, L, pump=True, potential=True)
+    psi0, n0 = 0.4 - 0.2j, 0.8
+    Vg, av = [2e-3, -1e-3], [0.7 + 0.2j, 1.1 - 0.4j]
+    Pg, ap = [3.0, 1.5], [1.3, 0.6]
+    Fg, al = [0.02 + 0.01j, -0.03j], [0.5 - 0.5j, 0.9 + 0.1j]
+    full = lambda v: np.full((N, N), v)
+    O.set_chain(p, "potential", [(full(v), None, a, a) for v, a in zip(Vg, av)])
+    O.set_chain(p, "pump", [(full(v), None, a, a) for v, a in zip(Pg, ap)])
+    O.set_chain(p, "pulse", [(full(v), None, a, a) for v, a in zip(Fg, al)])
+    out, n = O.ssfm_run(p, full(psi0), full(n0), dt=dt, steps=steps)
+    x, y = psi0, n0
+    for _ in range(steps):
+        rho = abs(x) ** 2
+        A = p.g_c * rho + sum(v * a.real for v, a in zip(Vg, av)) + p.g_r * y
+        B = sum(v * a.imag for v, a in zip(Vg, av))
+        x_new = x * np.exp(dt * (p.R * y - p.gamma_c) / 2 + dt * B / p.h_bar_s) * np.exp(-1j * dt * A / p.h_bar_s)
+        x_new += dt / p.h_bar_s * sum(f * a for f, a in zip(Fg, al))
+        y = y + dt * (-(p.gamma_r + p.R * rho) * y + sum(v * a for v, a in zip(Pg, ap)))
+        x = x_new
+    assert np.abs(out - x).max() < 1e-13 * abs(x)
+    assert np.abs(n - y).max() < 1e-13 * abs(y)
+
+
+def test_tetm_pulse_enters_times_minus_i():
+    """The TE-TM independent step adds -i (dt / hbar) F_g a_g to each component (kernel_gp_compute.cuh:409-413)."""
+    N, L, dt = 16, 7.5, 0.03
+    p = O.params(N, N, L, L, tetm=True, reservoir=False, g_c=0.0, gamma_c=0.0)
+    p.g_pm = 0.0
+    Fp, Fm, a = 0.02 + 0.01j, -0.03j, 0.5 - 0.5j
+    full = lambda v: np.full((N, N), v)
+    O.set_chain(p, "pulse", [(full(Fp), full(Fm), a, a)])
+    out_p, _, out_m, _ = O.ssfm_run(p, full(0.1 + 0j), psi_minus=full(0.2j), dt=dt)
+    assert np.abs(out_p - (0.1 - 1j * dt / p.h_bar_s * Fp * a)).max() < 1e-15
+    assert np.abs(out_m - (0.2j - 1j * dt / p.h_bar_s * Fm * a)).max() < 1e-15
