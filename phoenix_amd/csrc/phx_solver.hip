@@ -652,30 +652,36 @@ int autotune_rk4(phx_solver* s) {
     const int cands[] = {8, 10, 12, 16, 20, 24};
     /* deeper row prefetch exists for the pumped reservoir model */
     const int npf = (c.use_reservoir && c.use_pumps && !c.use_potentials) ? 3 : 1;
-    float best = 1e30f;
-    phx::Rk4Tune best_t = s->tune;
-    for (int cand = 0; cand < 6 * npf; cand++) {
-        const int ty = cands[cand % 6];
-        phx::Rk4Tune t = s->tune;
-        t.ty = ty;
-        t.pf = 2 + cand / 6;
-        const phx::Rk4Tune keep = s->tune;
-        s->tune = t;
-        int rc = PHX_OK;
-        for (int w = 0; w < 2 && !rc; w++) rc = rk4_launch<real>(s, s->t, s->dt), s->cur ^= 1;
-        PHX_HIP(hipEventRecord(e0, s->stream));
-        for (int r = 0; r < 6 && !rc; r++) rc = rk4_launch<real>(s, s->t, s->dt), s->cur ^= 1;
-        PHX_HIP(hipEventRecord(e1, s->stream));
-        PHX_HIP(hipEventSynchronize(e1));
-        s->tune = keep;
-        if (rc) return rc;
-        float ms = 0;
-        PHX_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best) {
-            best = ms;
-            best_t = t;
+    /* three round-robin passes over the candidates, each candidate's fastest pass counts: a
+       single pass in a fixed order would time the first candidates while the clocks ramp up */
+    std::vector<float> cand_ms(6 * npf, 1e30f);
+    for (int pass = 0; pass < 3; pass++) {
+        for (int cand = 0; cand < 6 * npf; cand++) {
+            phx::Rk4Tune t = s->tune;
+            t.ty = cands[cand % 6];
+            t.pf = 2 + cand / 6;
+            const phx::Rk4Tune keep = s->tune;
+            s->tune = t;
+            int rc = PHX_OK;
+            /* each launch reads the current buffer and writes the other; flipping cur back keeps the state */
+            for (int w = 0; w < 2 && !rc; w++) rc = rk4_launch<real>(s, s->t, s->dt), s->cur ^= 1;
+            PHX_HIP(hipEventRecord(e0, s->stream));
+            for (int r = 0; r < 6 && !rc; r++) rc = rk4_launch<real>(s, s->t, s->dt), s->cur ^= 1;
+            PHX_HIP(hipEventRecord(e1, s->stream));
+            PHX_HIP(hipEventSynchronize(e1));
+            s->tune = keep;
+            if (rc) return rc;
+            float ms = 0;
+            PHX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            cand_ms[cand] = std::min(cand_ms[cand], ms);
         }
     }
+    int best = 0;
+    for (int cand = 1; cand < 6 * npf; cand++)
+        if (cand_ms[cand] < cand_ms[best]) best = cand;
+    phx::Rk4Tune best_t = s->tune;
+    best_t.ty = cands[best % 6];
+    best_t.pf = 2 + best / 6;
     PHX_HIP(hipEventDestroy(e0));
     PHX_HIP(hipEventDestroy(e1));
     s->tune = best_t;
