@@ -17,7 +17,7 @@
  *   - A handle is not thread safe.  Each handle owns one HIP stream (or uses the
  *     one given by phx_set_stream).
  *   - Host field layout: full grid, row-major N_r x N_c, index row*N_c + col.
- *     Complex fields (psi, pulse) are interleaved {re, im}; real fields (n, P, V,
+ *     Complex fields (psi, pulse, k-space spectra) are interleaved {re, im}; real fields (n, P, V,
  *     FFT mask) are plain.  Element type is float for PHX_FP32, double for PHX_FP64.
  */
 #ifndef PHX_H
@@ -54,7 +54,13 @@ enum phx_field {
     PHX_POTENTIAL_PLUS = 6,
     PHX_POTENTIAL_MINUS = 7,
     PHX_FFT_MASK_PLUS = 8,
-    PHX_FFT_MASK_MINUS = 9
+    PHX_FFT_MASK_MINUS = 9,
+    /* k-space spectra (complex, read-only; matrix.fft_plus / fft_minus, solver_fft.cu:67-100): after a
+       filter application DFT(psi) / (N_c N_r) * fftshift(mask), without a mask the unnormalised DFT(psi)
+       at the last FFT time; kept when fft_every < t_max and the output keys ask for fft output
+       (solver_output_matrices.cu:41,62), row-major with k = 0 first, zero before the first FFT time */
+    PHX_FFT_PLUS = 10,
+    PHX_FFT_MINUS = 11
 };
 
 /* Summary of a configured solver (read-only view of SystemParameters,

@@ -19,8 +19,9 @@ PHX_OK, PHX_ERR_INVALID, PHX_ERR_HIP, PHX_ERR_UNSUPPORTED, PHX_ERR_IO, PHX_ERR_S
 FIELDS = {
     "psi_plus": 0, "psi_minus": 1, "n_plus": 2, "n_minus": 3, "pump_plus": 4, "pump_minus": 5,
     "potential_plus": 6, "potential_minus": 7, "fft_mask_plus": 8, "fft_mask_minus": 9,
+    "fft_plus": 10, "fft_minus": 11,
 }
-COMPLEX_FIELDS = {0, 1}
+COMPLEX_FIELDS = {0, 1, 10, 11}
 
 # Every entry point declared in include/phx.h (checked by tests/test_capi.py).
 EXPORTED = [

@@ -163,14 +163,20 @@ __device__ __forceinline__ int fft_xcd_group(int b, int nblocks) {
     return x * q + (x < r ? x : r) + (b >> 3);
 }
 
+/* What the column pass does in k-space. */
+enum { kColFilter = 0, kColPropagate = 1, kColForward = 2 };
+
 /* CB adjacent columns: forward FFT along the column, a pointwise factor in k-space, inverse FFT.
-   PROP = false: F <- F / (N_c N_r) * mask[shifted] (the k-space filter);
-   PROP = true:  F <- F * colf[c] * rowf[r] (the split-step propagator; the tables carry 1 / (N_c N_r)). */
-template <typename real, int CB, int KB, bool PROP>
+   MODE kColFilter:    F <- F / (N_c N_r) * mask[shifted] (the k-space filter); with spec != nullptr
+                       the filtered spectrum is also stored there (row-major, k = 0 first), as the
+                       reference keeps it in fft_plus / fft_minus (solver_fft.cu:67-100);
+   MODE kColPropagate: F <- F * colf[c] * rowf[r] (the split-step propagator; the tables carry 1 / (N_c N_r));
+   MODE kColForward:   the forward transform only (data becomes the unnormalised spectrum). */
+template <typename real, int CB, int KB, int MODE>
 __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw,
                                                               const real* __restrict__ mask, const Cx<real>* __restrict__ rowf,
                                                               const Cx<real>* __restrict__ colf, int Nc, int Nr, int log2Nr,
-                                                              real Ntot) {
+                                                              real Ntot, typename Vec2<real>::type* spec) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
     const int c0 = fft_xcd_group(blockIdx.x, gridDim.x) * CB;
@@ -195,7 +201,8 @@ __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<rea
     }
     __syncthreads();
     stockham<real, -1, KB>(A, tw, Nr, log2Nr, CB);
-    if constexpr (PROP) {
+    if constexpr (MODE == kColForward) {
+    } else if constexpr (MODE == kColPropagate) {
         /* factor tables read before they are used (all loads in flight together) */
         Cx<real> rf[PT];
 #pragma unroll
@@ -229,11 +236,17 @@ __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<rea
                 Cx<real>& f = A[j * Nr + r];
                 f.r = f.r / Ntot * mk[k];
                 f.i = f.i / Ntot * mk[k];
+                if (spec) {
+                    typename Vec2<real>::type o;
+                    o.x = f.r;
+                    o.y = f.i;
+                    spec[(size_t)r * Nc + c0 + j] = o;
+                }
             }
         }
     }
     __syncthreads();
-    stockham<real, +1, KB>(A, tw, Nr, log2Nr, CB);
+    if constexpr (MODE != kColForward) stockham<real, +1, KB>(A, tw, Nr, log2Nr, CB);
 #pragma unroll
     for (int k = 0; k < PT; k++) {
         const int i = threadIdx.x + k * kFftThreads;
@@ -288,12 +301,13 @@ __global__ __launch_bounds__(kFftThreads) void dft_rows(typename Vec2<real>::typ
     }
 }
 
-/* CB adjacent columns: forward DFT along the column, the k-space factor (as fft_cols_filter), inverse
-   DFT; two LDS buffers (the transforms are out of place). */
-template <typename real, int CB, bool PROP>
+/* CB adjacent columns: forward DFT along the column, the k-space factor (MODE as fft_cols_filter, with
+   the same optional spectrum output), inverse DFT; two LDS buffers (the transforms are out of place). */
+template <typename real, int CB, int MODE>
 __global__ __launch_bounds__(kFftThreads) void dft_cols_filter(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw,
                                                               const real* __restrict__ mask, const Cx<real>* __restrict__ rowf,
-                                                              const Cx<real>* __restrict__ colf, int Nc, int Nr, real Ntot) {
+                                                              const Cx<real>* __restrict__ colf, int Nc, int Nr, real Ntot,
+                                                              typename Vec2<real>::type* spec) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
     Cx<real>* B = A + CB * Nr;
@@ -307,7 +321,13 @@ __global__ __launch_bounds__(kFftThreads) void dft_cols_filter(typename Vec2<rea
     for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
         const int k = i / CB, j = i - k * CB;
         Cx<real> f = dft_one<real, -1>(A + j * Nr, 1, tw, Nr, k);
-        if constexpr (PROP) {
+        if constexpr (MODE == kColForward) {
+            typename Vec2<real>::type o;
+            o.x = f.r;
+            o.y = f.i;
+            data[(size_t)k * Nc + c0 + j] = o; /* A holds the columns: the in-place store is safe */
+            continue;
+        } else if constexpr (MODE == kColPropagate) {
             f = cmul(cmul(f, colf[c0 + j]), rowf[k]);
         } else {
             /* mask of the quadrant-swapped grid (fft_shift_2D, kernel_fft.cuh:15-32) */
@@ -317,9 +337,16 @@ __global__ __launch_bounds__(kFftThreads) void dft_cols_filter(typename Vec2<rea
             const real m = mask[(size_t)rs * Nc + cs];
             f.r = f.r / Ntot * m;
             f.i = f.i / Ntot * m;
+            if (spec) {
+                typename Vec2<real>::type o;
+                o.x = f.r;
+                o.y = f.i;
+                spec[(size_t)k * Nc + c0 + j] = o;
+            }
         }
         B[j * Nr + k] = f;
     }
+    if constexpr (MODE == kColForward) return;
     __syncthreads();
     for (int i = threadIdx.x; i < CB * Nr; i += blockDim.x) {
         const int r = i / CB, j = i - r * CB;
@@ -334,9 +361,9 @@ __global__ __launch_bounds__(kFftThreads) void dft_cols_filter(typename Vec2<rea
 /* sizes the direct DFT takes: even, and two column buffers of one column in 64 KiB */
 static bool dft_supported(int N, size_t real_size) { return N >= 2 && N % 2 == 0 && (size_t)N * 4 * real_size <= 65536; }
 
-template <typename real, bool PROP>
+template <typename real, int MODE>
 static hipError_t dft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, const void* rowf,
-                                const void* colf, int Nc, int Nr, hipStream_t s) {
+                                const void* colf, int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec = nullptr) {
     const auto* twc = (const Cx<real>*)tw_c;
     const auto* twr = (const Cx<real>*)tw_r;
     const auto* rf = (const Cx<real>*)rowf;
@@ -348,13 +375,13 @@ static hipError_t dft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c
     const size_t per_col = 2 * (size_t)Nr * sizeof(Cx<real>);
     auto fits = [&](int cb) { return per_col * cb <= 65536 && Nc % cb == 0; };
     if (fits(4))
-        hipLaunchKernelGGL((dft_cols_filter<real, 4, PROP>), dim3(Nc / 4), dim3(kFftThreads), 4 * per_col, s, psi, twr, mask, rf, cf, Nc, Nr, ntot);
+        hipLaunchKernelGGL((dft_cols_filter<real, 4, MODE>), dim3(Nc / 4), dim3(kFftThreads), 4 * per_col, s, psi, twr, mask, rf, cf, Nc, Nr, ntot, spec);
     else if (fits(2))
-        hipLaunchKernelGGL((dft_cols_filter<real, 2, PROP>), dim3(Nc / 2), dim3(kFftThreads), 2 * per_col, s, psi, twr, mask, rf, cf, Nc, Nr, ntot);
+        hipLaunchKernelGGL((dft_cols_filter<real, 2, MODE>), dim3(Nc / 2), dim3(kFftThreads), 2 * per_col, s, psi, twr, mask, rf, cf, Nc, Nr, ntot, spec);
     else
-        hipLaunchKernelGGL((dft_cols_filter<real, 1, PROP>), dim3(Nc), dim3(kFftThreads), per_col, s, psi, twr, mask, rf, cf, Nc, Nr, ntot);
+        hipLaunchKernelGGL((dft_cols_filter<real, 1, MODE>), dim3(Nc), dim3(kFftThreads), per_col, s, psi, twr, mask, rf, cf, Nc, Nr, ntot, spec);
     e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    if (e != hipSuccess || MODE == kColForward) return e;
     hipLaunchKernelGGL((dft_rows<real, +1>), dim3(Nr), dim3(kFftThreads), (size_t)Nc * sizeof(Cx<real>), s, psi, twc, Nc);
     return hipGetLastError();
 }
@@ -401,9 +428,9 @@ static hipError_t launch_rows(typename Vec2<real>::type* psi, const void* tw_c, 
     return hipGetLastError();
 }
 
-template <typename real, int CB, bool PROP>
+template <typename real, int CB, int MODE>
 static hipError_t launch_cols(typename Vec2<real>::type* psi, const void* tw_r, const real* mask, const void* rowf, const void* colf,
-                              int Nc, int Nr, hipStream_t s) {
+                              int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec) {
     const size_t lds = (size_t)CB * Nr * sizeof(Cx<real>);
     const int kb = bfly_bound(CB * (Nr / 4));
     const auto* tw = (const Cx<real>*)tw_r;
@@ -412,22 +439,23 @@ static hipError_t launch_cols(typename Vec2<real>::type* psi, const void* tw_r, 
     const int l = ilog2(Nr);
     const real ntot = (real)((double)Nc * (double)Nr);
     if (kb == 1)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 1, PROP>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot);
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 1, MODE>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
     else if (kb == 2)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 2, PROP>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot);
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 2, MODE>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
     else if (kb == 4)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 4, PROP>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot);
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 4, MODE>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
     else if constexpr (max_bfly<real>() >= 8)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 8, PROP>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot);
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 8, MODE>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
-/* rows forward -> columns (forward, k-space factor, inverse) -> rows inverse */
-template <typename real, bool PROP>
+/* rows forward -> columns (forward, k-space factor, inverse) -> rows inverse; kColForward stops
+   after the forward column transform */
+template <typename real, int MODE>
 static hipError_t fft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, const void* rowf,
-                                const void* colf, int Nc, int Nr, hipStream_t s) {
+                                const void* colf, int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec = nullptr) {
     hipError_t e = launch_rows<real, -1>(psi, tw_c, Nc, Nr, s);
     if (e != hipSuccess) return e;
     /* column block width: as wide as LDS and registers allow, but no wider than keeps
@@ -442,45 +470,56 @@ static hipError_t fft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c
         return cb_env == cb && per_col * cb <= 65536 && Nc % cb == 0 && cb * (Nr / 4) <= max_bfly<real>() * kFftThreads;
     };
     if (fits_env(8))
-        e = launch_cols<real, 8, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
+        e = launch_cols<real, 8, MODE>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
     else if (fits_env(4))
-        e = launch_cols<real, 4, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
+        e = launch_cols<real, 4, MODE>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
     else if (fits_env(1))
-        e = launch_cols<real, 1, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
+        e = launch_cols<real, 1, MODE>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
     else if (fits(8))
-        e = launch_cols<real, 8, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
+        e = launch_cols<real, 8, MODE>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
     else if (fits(4))
-        e = launch_cols<real, 4, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
+        e = launch_cols<real, 4, MODE>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
     else if (fits(2))
-        e = launch_cols<real, 2, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
+        e = launch_cols<real, 2, MODE>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
     else
-        e = launch_cols<real, 1, PROP>(psi, tw_r, mask, rowf, colf, Nc, Nr, s);
-    if (e != hipSuccess) return e;
+        e = launch_cols<real, 1, MODE>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
+    if (e != hipSuccess || MODE == kColForward) return e;
     return launch_rows<real, +1>(psi, tw_c, Nc, Nr, s);
 }
 
 /* powers of two on both axes: Stockham kernels; otherwise the direct DFT kernels for both axes */
 template <typename real>
 hipError_t launch_fft_filter(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, int Nc, int Nr,
-                             hipStream_t s) {
+                             hipStream_t s, typename Vec2<real>::type* spec) {
     if (stockham_supported(Nc, sizeof(real)) && stockham_supported(Nr, sizeof(real)))
-        return fft_roundtrip<real, false>(psi, tw_c, tw_r, mask, nullptr, nullptr, Nc, Nr, s);
+        return fft_roundtrip<real, kColFilter>(psi, tw_c, tw_r, mask, nullptr, nullptr, Nc, Nr, s, spec);
     if (!fft_supported(Nc, sizeof(real)) || !fft_supported(Nr, sizeof(real))) return hipErrorInvalidValue;
-    return dft_roundtrip<real, false>(psi, tw_c, tw_r, mask, nullptr, nullptr, Nc, Nr, s);
+    return dft_roundtrip<real, kColFilter>(psi, tw_c, tw_r, mask, nullptr, nullptr, Nc, Nr, s, spec);
+}
+
+/* the unnormalised forward 2D transform of the grid, in place */
+template <typename real>
+hipError_t launch_fft_forward(typename Vec2<real>::type* data, const void* tw_c, const void* tw_r, int Nc, int Nr, hipStream_t s) {
+    if (stockham_supported(Nc, sizeof(real)) && stockham_supported(Nr, sizeof(real)))
+        return fft_roundtrip<real, kColForward>(data, tw_c, tw_r, nullptr, nullptr, nullptr, Nc, Nr, s);
+    if (!fft_supported(Nc, sizeof(real)) || !fft_supported(Nr, sizeof(real))) return hipErrorInvalidValue;
+    return dft_roundtrip<real, kColForward>(data, tw_c, tw_r, nullptr, nullptr, nullptr, Nc, Nr, s);
 }
 
 template <typename real>
 hipError_t launch_fft_propagate(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const void* rowf, const void* colf,
                                 int Nc, int Nr, hipStream_t s) {
     if (stockham_supported(Nc, sizeof(real)) && stockham_supported(Nr, sizeof(real)))
-        return fft_roundtrip<real, true>(psi, tw_c, tw_r, nullptr, rowf, colf, Nc, Nr, s);
+        return fft_roundtrip<real, kColPropagate>(psi, tw_c, tw_r, nullptr, rowf, colf, Nc, Nr, s);
     if (!fft_supported(Nc, sizeof(real)) || !fft_supported(Nr, sizeof(real))) return hipErrorInvalidValue;
-    return dft_roundtrip<real, true>(psi, tw_c, tw_r, nullptr, rowf, colf, Nc, Nr, s);
+    return dft_roundtrip<real, kColPropagate>(psi, tw_c, tw_r, nullptr, rowf, colf, Nc, Nr, s);
 }
 
 template hipError_t launch_fft_propagate<float>(float2*, const void*, const void*, const void*, const void*, int, int, hipStream_t);
 template hipError_t launch_fft_propagate<double>(double2*, const void*, const void*, const void*, const void*, int, int, hipStream_t);
-template hipError_t launch_fft_filter<float>(float2*, const void*, const void*, const float*, int, int, hipStream_t);
-template hipError_t launch_fft_filter<double>(double2*, const void*, const void*, const double*, int, int, hipStream_t);
+template hipError_t launch_fft_filter<float>(float2*, const void*, const void*, const float*, int, int, hipStream_t, float2*);
+template hipError_t launch_fft_filter<double>(double2*, const void*, const void*, const double*, int, int, hipStream_t, double2*);
+template hipError_t launch_fft_forward<float>(float2*, const void*, const void*, int, int, hipStream_t);
+template hipError_t launch_fft_forward<double>(double2*, const void*, const void*, int, int, hipStream_t);
 
 } // namespace phx

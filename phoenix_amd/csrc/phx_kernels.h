@@ -246,7 +246,11 @@ inline int rk_generic_waves(RkTableau tab, int Nc, int Nr) {
    precision) of the row length N_c and column length N_r. */
 template <typename real>
 hipError_t launch_fft_filter(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, int Nc, int Nr,
-                             hipStream_t s);
+                             hipStream_t s, typename Vec2<real>::type* spec = nullptr);
+/* The filter above also stores the filtered spectrum DFT(psi) / (N_c N_r) * fftshift(mask) in spec
+   (row-major, k = 0 first) when spec != nullptr.  launch_fft_forward: DFT(data) in place. */
+template <typename real>
+hipError_t launch_fft_forward(typename Vec2<real>::type* data, const void* tw_c, const void* tw_r, int Nc, int Nr, hipStream_t s);
 
 /* Split-step linear propagator in place: psi <- IDFT(DFT(psi) * colf[c] * rowf[r]), with the
    separable complex factors rowf (N_r), colf (N_c) computed on the host (they carry 1/(N_c N_r)). */

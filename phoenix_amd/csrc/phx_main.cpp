@@ -261,7 +261,15 @@ void cache_values(phx_solver* s, Out& o) {
     }
 }
 
-/* Solver::outputMatrices (solver_output_matrices.cu:20-70) for wavefunction and reservoir */
+/* header of the k-space matrices (solver_output_matrices.cu:23): LX, LY = -(extent / N), DX, DY = 2 / N */
+std::string fft_header(int nc, int nr, double lx, double ly, double dx, double dy, double t) {
+    std::ostringstream h;
+    h << "# SIZE " << nc << " " << nr << " LX " << lx << " LY " << ly << " DX " << dx << " DY " << dy << " TIME " << t
+      << " :: PHOENIX_ MATRIX\n";
+    return h.str();
+}
+
+/* Solver::outputMatrices (solver_output_matrices.cu:20-70) for wavefunction, reservoir and the k-space spectra */
 template <typename real>
 void output_matrices(phx_solver* s, const Out& o, int x0, int x1, int y0, int y1, int inc, const std::string& suffix,
                      const std::string& prefix) {
@@ -281,6 +289,12 @@ void output_matrices(phx_solver* s, const Out& o, int x0, int x1, int y0, int y1
                     "rv", "all"}) &&
             field<real>(s, k == 0 ? PHX_N_PLUS : PHX_N_MINUS, false, z))
             write_matrix(o, prefix + "reservoir_" + sfx + suffix, z, x0, x1, y0, y1, inc, head);
+        /* the spectrum of the last FFT time (matrix.fft_plus / fft_minus) */
+        if (o.in.fft_every < o.in.t_max && o.want({"fft_mask", "fft", k == 0 ? "fft_plus" : "fft_minus", "plus", "mat", "all"}) &&
+            field<real>(s, k == 0 ? PHX_FFT_PLUS : PHX_FFT_MINUS, true, z))
+            write_matrix(o, prefix + "fft_" + sfx + suffix, z, x0, x1, y0, y1, inc,
+                         fft_header(nc, nr, -1.0 * (x1 - x0) / o.in.N_c, -1.0 * (y1 - y0) / o.in.N_r, 2.0 / o.in.N_c,
+                                    2.0 / o.in.N_r, o.in.t));
     }
 }
 

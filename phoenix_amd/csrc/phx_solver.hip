@@ -97,6 +97,11 @@ struct phx_solver {
     /* staged path (phx_staged.hip): padded stage input and K slots, per-group source fields */
     bool staged = false;
     bool grouped = false; /* one source field per temporal group (staged path, SSFM with chains or pulses) */
+    /* k-space spectra kept for output (the reference's fft_plus / fft_minus): the filtered spectrum
+       of the last filter application, or, without a mask, a copy of psi at the last FFT time whose
+       forward transform is taken when the spectrum is read (spec_raw) */
+    void* spec[2] = {nullptr, nullptr};
+    bool spec_raw = false;
     int staged_H = 0;
     size_t staged_padN = 0;
     void* st_y_psi[2] = {nullptr, nullptr};
@@ -222,6 +227,16 @@ int initialize_t(phx_solver* s) {
     }
     if (c.use_stochastic)
         if (int r = dev_alloc(s, &s->noise, 2 * N * sizeof(real))) return r;
+    /* fft_plus / fft_minus are written when fft_every < t_max and the output keys ask for them
+       (solver_output_matrices.cu:41,62) */
+    const bool fft_sizes = phx::fft_supported(c.N_c, sizeof(real)) && phx::fft_supported(c.N_r, sizeof(real));
+    for (int k = 0; k < ncomp && c.fft_every < c.t_max && fft_sizes; k++) {
+        const char* keys[] = {"fft_mask", "fft", k == 0 ? "fft_plus" : "fft_minus", "plus", "mat", "all"};
+        bool want = false;
+        for (const char* key : keys) want = want || std::find(c.output_keys.begin(), c.output_keys.end(), key) != c.output_keys.end();
+        if (want)
+            if (int r = dev_alloc(s, &s->spec[k], 2 * N * sizeof(real))) return r;
+    }
     if (s->staged) {
         const int H = std::max(phx::rk_tableau_stages(s->iter->fixed), phx::rk_tableau_stages(s->iter->adaptive));
         s->staged_H = H;
@@ -250,7 +265,7 @@ int initialize_t(phx_solver* s) {
         if (int r = dev_alloc(s, &s->ssfm_rowf, 2 * (size_t)c.N_r * sizeof(real))) return r;
         if (int r = dev_alloc(s, &s->ssfm_colf, 2 * (size_t)c.N_c * sizeof(real))) return r;
     }
-    if (c.use_fft_mask || s->ssfm) {
+    if (c.use_fft_mask || s->ssfm || s->spec[0]) {
         for (int d = 0; d < 2; d++) {
             const int L = d == 0 ? c.N_c : c.N_r;
             std::vector<std::complex<double>> w(L);
@@ -547,7 +562,29 @@ int fft_filter_t(phx_solver* s) {
     using v2 = typename phx::Vec2<real>::type;
     for (int k = 0; k < (c.tetm ? 2 : 1); k++)
         PHX_HIP(phx::launch_fft_filter<real>((v2*)s->psi[k][s->cur], s->tw_c, s->tw_r, (const real*)s->mask[k], c.N_c, c.N_r,
-                                             s->stream));
+                                             s->stream, (v2*)s->spec[k]));
+    s->spec_raw = false;
+    return PHX_OK;
+}
+
+/* applyFFTFilter(false) (solver_fft.cu:67-79): without a mask the reference only transforms psi for
+   output; here psi is copied and the transform is taken when the spectrum is read (spectrum_ready) */
+template <typename real>
+int fft_snapshot_t(phx_solver* s) {
+    const size_t bytes = 2 * s->cells() * sizeof(real);
+    for (int k = 0; k < 2; k++)
+        if (s->spec[k]) PHX_HIP(hipMemcpyAsync(s->spec[k], s->psi[k][s->cur], bytes, hipMemcpyDeviceToDevice, s->stream));
+    s->spec_raw = true;
+    return PHX_OK;
+}
+
+template <typename real>
+int spectrum_ready_t(phx_solver* s) {
+    if (!s->spec_raw) return PHX_OK;
+    using v2 = typename phx::Vec2<real>::type;
+    for (int k = 0; k < 2; k++)
+        if (s->spec[k]) PHX_HIP(phx::launch_fft_forward<real>((v2*)s->spec[k], s->tw_c, s->tw_r, s->cfg.N_c, s->cfg.N_r, s->stream));
+    s->spec_raw = false;
     return PHX_OK;
 }
 
@@ -749,8 +786,11 @@ int iterate_t(phx_solver* s, bool force_fixed) {
     if (s->rnd(s->t - s->fft_cached_t) < c.fft_every) return 1; /* FFT guard in the build's real (solver.cu:62-63) */
     s->fft_cached_t = s->t;
     /* without a mask the reference only transforms for output (solver_fft.cu:78-79): no state change */
-    if (c.use_fft_mask)
+    if (c.use_fft_mask) {
         if (int r = fft_filter_t<real>(s)) return r;
+    } else if (s->spec[0]) {
+        if (int r = fft_snapshot_t<real>(s)) return r;
+    }
     return 1;
 }
 
@@ -780,7 +820,7 @@ int graph_block_t(phx_solver* s, int64_t* taken) {
         if (t >= c.t_max) return PHX_OK;
         t = s->rnd(t + s->dt);
         if (!(s->rnd(t - fc) < c.fft_every)) {
-            if (c.use_fft_mask) return PHX_OK;
+            if (c.use_fft_mask || s->spec[0]) return PHX_OK;
             fc = t;
         }
     }
@@ -831,6 +871,8 @@ int field_info(phx_solver* s, int field, void*** slot, size_t* elems) {
         case PHX_POTENTIAL_MINUS: *slot = &s->pot[k]; *elems = N; break;
         case PHX_FFT_MASK_PLUS:
         case PHX_FFT_MASK_MINUS: *slot = &s->mask[k]; *elems = N; break;
+        case PHX_FFT_PLUS:
+        case PHX_FFT_MINUS: *slot = &s->spec[k]; *elems = 2 * N; break;
         default: *slot = &none; return fail(PHX_ERR_INVALID, "unknown field id " + std::to_string(field));
     }
     if (**slot == nullptr) return fail(PHX_ERR_INVALID, "field " + std::to_string(field) + " is not allocated in this configuration");
@@ -965,6 +1007,8 @@ int phx_get_field(phx_solver* s, int field, void* host, size_t bytes) {
     size_t elems;
     if (int r = field_info(s, field, &slot, &elems)) return r;
     if (bytes != elems * s->rsize) return fail(PHX_ERR_INVALID, "byte count mismatch: expected " + std::to_string(elems * s->rsize));
+    if (field == PHX_FFT_PLUS || field == PHX_FFT_MINUS)
+        if (int r = s->precision == PHX_FP32 ? spectrum_ready_t<float>(s) : spectrum_ready_t<double>(s)) return r;
     PHX_HIP(hipMemcpyAsync(host, *slot, bytes, hipMemcpyDeviceToHost, s->stream));
     PHX_HIP(hipStreamSynchronize(s->stream));
     return PHX_OK;
@@ -972,6 +1016,7 @@ int phx_get_field(phx_solver* s, int field, void* host, size_t bytes) {
 
 int phx_set_field(phx_solver* s, int field, const void* host, size_t bytes) {
     if (!s || !s->initialized) return fail(PHX_ERR_STATE, "not initialized");
+    if (field == PHX_FFT_PLUS || field == PHX_FFT_MINUS) return fail(PHX_ERR_INVALID, "the k-space spectra are read-only");
     void** slot;
     size_t elems;
     if (int r = field_info(s, field, &slot, &elems)) return r;
@@ -986,6 +1031,8 @@ int phx_device_ptr(phx_solver* s, int field, void** dptr) {
     void** slot;
     size_t elems;
     if (int r = field_info(s, field, &slot, &elems)) return r;
+    if (field == PHX_FFT_PLUS || field == PHX_FFT_MINUS)
+        if (int r = s->precision == PHX_FP32 ? spectrum_ready_t<float>(s) : spectrum_ready_t<double>(s)) return r;
     *dptr = *slot;
     return PHX_OK;
 }

@@ -252,3 +252,25 @@ def test_reference_example_2_config_through_pyphoenix(tmp_path):
     e = rel_l2(g, psi)
     say(f"  rel L2 psi {e:.3e}")
     assert e <= 1e-5
+
+
+def test_executable_writes_the_kspace_spectrum(tmp_path):
+    """With --fftEvery below --tmax the default keys write fft_plus (solver_output_matrices.cu:41): the
+    spectrum of the last FFT time, complex, with the k-space header LX = LY = -1, DX = 2/N_c, DY = 2/N_r."""
+    N, L, dt = 32, 15.5, 0.04
+    exe = os.path.join(BIN, "PHOENIX_gpu_fp64")
+    out = tmp_path / "out"
+    args = ["--N", N, N, "--L", L, L, "--boundary", "periodic", "periodic", "--tstep", dt, "--tmax", 0.5,
+            "--outEvery", 1e9, "--fftEvery", 0.2, "--initRandom", 0.1, 5,
+            "--pump", 5.0, "add", 8, 8, 0, 0, "both", 1, "none", "gauss+noDivide",
+            "--fftMask", 1.0, "add", 0.5, 0.5, 0, 0, "both", 6, "none", "gauss+noDivide+local", "--path", str(out)]
+    say("[cli k-space output] PHOENIX_gpu_fp64 32x32, fftEvery 0.2")
+    r = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    head = open(out / "fft_plus.txt").readline()
+    assert head.startswith(f"# SIZE {N} {N} LX -1 LY -1 DX {2 / N:g} DY {2 / N:g} TIME "), head
+    F = read_txt(out / "fft_plus.txt")
+    mask = np.fft.fftshift(read_txt(out / "fft_mask_plus.txt").real)
+    assert F.shape == (N, N) and np.isfinite(F).all() and np.abs(F).max() > 0
+    # the filtered spectrum vanishes where the (shifted) mask does
+    assert np.abs(F[mask < 1e-12]).max(initial=0.0) <= 1e-12 * np.abs(F).max()

@@ -156,3 +156,66 @@ def test_rk4_with_periodic_filter_matches_oracle(prec):
     e, en = rel_l2(g, o), rel_l2(gn, on)
     say(f"  rel L2 psi {e:.3e} n {en:.3e}")
     assert e <= tol and en <= tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("shape,tetm", [((64, 64), False), ((60, 48), True)])
+def test_filtered_spectrum_is_kept_for_output(prec, shape, tetm):
+    """After a filter application the solver keeps DFT(psi) / (N_c N_r) * fftshift(mask), as the
+    reference's fft_plus / fft_minus hold it (solver_fft.cu:67-100); the 60x48 grid takes the direct-DFT
+    kernels.  Reference: numpy.fft.fft2 of the same psi."""
+    Nc, Nr = shape
+    say(f"[fft spectrum, mask] prec={prec} shape={shape} tetm={tetm}")
+    rng = np.random.default_rng(Nc + 3 * Nr)
+    comps = ["plus", "minus"] if tetm else ["plus"]
+    psi = {k: rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc)) for k in comps}
+    mask = {k: rng.random((Nr, Nc)) for k in comps}
+    s = make_solver(prec, Nc, Nr, steps=10, fft_every=0.3, extra=["-tetm"] if tetm else [])
+    for k in comps:
+        s.set(f"psi_{k}", psi[k])
+        s.set(f"fft_mask_{k}", mask[k])
+    s.apply_fft_filter()
+    got = {k: s.get(f"fft_{k}").astype(np.complex128) for k in comps}
+    s.close()
+    for k in comps:
+        ref = np.fft.fft2(psi[k]) / (Nc * Nr) * np.fft.fftshift(mask[k])
+        e = rel_l2(got[k], ref)
+        say(f"  {k}: rel L2 {e:.3e}")
+        assert e <= (1e-12 if prec == "fp64" else 1e-5)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_unfiltered_spectrum_of_the_last_fft_time(prec):
+    """Without a mask the reference transforms psi at every FFT time for output only (solver_fft.cu:78-79):
+    with --fftEvery 1.5 dt the spectrum is DFT(psi) after step 2 (t - t_fft >= fftEvery), not after the
+    final step 3, and psi itself is not changed."""
+    from phoenix_amd._lib import Solver
+    Nc, Nr, dt = 64, 32, 0.04
+    say(f"[fft spectrum, no mask] prec={prec}")
+    rng = np.random.default_rng(5)
+    psi0 = 0.1 * (rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc)))
+
+    def run(steps, fft_every):
+        L = 0.5 * (Nc - 1)
+        args = ["--N", Nc, Nr, "--L", L, 0.5 * (Nr - 1), "--boundary", "periodic", "periodic", "--tstep", dt,
+                "--tmax", (steps - 0.5) * dt, "--outEvery", 1e9, "--fftEvery", fft_every,
+                "--pump", 1.0, "add", 10, 10, 0, 0, "both", 1, "none", "gauss+noDivide"]
+        s = Solver([str(a) for a in args], prec)
+        s.set("psi_plus", psi0)
+        assert s.step(steps + 5) == steps
+        return s
+
+    s3 = run(3, 1.5 * dt)
+    spec = s3.get("fft_plus").astype(np.complex128)
+    psi3 = s3.get("psi_plus").astype(np.complex128)
+    s3.close()
+    s2 = run(2, 1e9)
+    psi2 = s2.get("psi_plus").astype(np.complex128)
+    s2.close()
+    s3b = run(3, 1e9)  # no FFT times at all: psi must be the same
+    psi3b = s3b.get("psi_plus").astype(np.complex128)
+    s3b.close()
+    e = rel_l2(spec, np.fft.fft2(psi2))
+    say(f"  rel L2 spectrum {e:.3e}")
+    assert e <= (1e-12 if prec == "fp64" else 1e-5)
+    assert np.array_equal(psi3, psi3b)
