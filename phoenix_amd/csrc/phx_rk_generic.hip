@@ -140,7 +140,15 @@ hipError_t launch_rk_generic(const RkGenArgs<real>& a, RkTableau tab, hipStream_
     switch (tab) {
         case RkTableau::RK4: return launch_tab<real, TabRK4>(a, s);
         case RkTableau::DP45_FIXED: return launch_tab<real, TabDP45Fixed>(a, s);
-        case RkTableau::DP45_ADAPTIVE: return launch_tab<real, TabDP45Adaptive>(a, s);
+        case RkTableau::DP45_ADAPTIVE: {
+            /* TE-TM in FP32 runs 32-row tiles: the 7-stage halo is recomputed over 32 rows instead of 16
+               (one wave per SIMD either way, no scratch); 307 instead of 366 us per attempt at 2048^2.
+               PHX_GEN_TY=16 or 32 fixes the height (tuning runs); tile height does not change results. */
+            static const int env_ty = [] { const char* e = std::getenv("PHX_GEN_TY"); return e ? std::atoi(e) : 0; }();
+            const int ty = env_ty ? env_ty : (a.tetm && sizeof(real) == 4 ? 32 : 16);
+            if (ty == 32) return dispatch_flags<real, TabDP45Adaptive, 32>(a, s);
+            return launch_tab<real, TabDP45Adaptive>(a, s);
+        }
         case RkTableau::NEWTON:
         case RkTableau::MP:
         case RkTableau::HEUN2:
