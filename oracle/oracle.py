@@ -388,3 +388,41 @@ def gaussian_field(N_c, N_r, amplitude, seed):
     out = np.zeros((N_r, N_c), np.complex128)
     lib().orc_gaussian_field(_dp(out.view(np.float64)), N_c * N_r, amplitude, seed)
     return out
+
+
+def mt19937_words(seed: int, count: int) -> np.ndarray:
+    """The first `count` 32-bit outputs of std::mt19937{seed} (Matsumoto & Nishimura 1998, seeded by
+    init_genrand, as the C++ standard specifies), from numpy's MT19937 with the same legacy seeding."""
+    rs = np.random.RandomState(seed & 0xFFFFFFFF)
+    return rs._bit_generator.random_raw(count).astype(np.uint64)
+
+
+def init_random(N_c, N_r, amplitude, seed, fp32=False, components=1):
+    """--initRandom amplitude seed (solver_initialization.cu:71-79): std::mt19937{seed} and
+    std::uniform_real_distribution<real>(-a, a); each cell of psi+ (then psi-) in row-major order
+    gets z += (u, u') with the real part drawn first.  The draws follow libstdc++'s
+    generate_canonical: for float one 32-bit word w, u01 = float(w) / 2^32; for double two
+    words, u01 = (w1 + w2 2^32) / 2^64 in double; u01 >= 1 becomes the largest value below 1;
+    then u = u01 (b - a) + a in the real type.  Returns a list of complex128 (N_r, N_c) fields."""
+    n = N_c * N_r
+    draws = 2 * n * components
+    per = 1 if fp32 else 2
+    w = mt19937_words(seed, draws * per)
+    if fp32:
+        f = np.float32
+        u01 = w.astype(np.float32) / f(4294967296.0)
+        u01 = np.where(u01 >= f(1), np.nextafter(f(1), f(0)), u01).astype(np.float32)
+        a, b = f(-amplitude), f(amplitude)
+        u = (u01 * (b - a) + a).astype(np.float32).astype(np.float64)
+    else:
+        lo, hi = w[0::2].astype(np.float64), w[1::2].astype(np.float64)
+        s = lo + hi * 4294967296.0
+        u01 = s / 18446744073709551616.0
+        u01 = np.where(u01 >= 1.0, np.nextafter(1.0, 0.0), u01)
+        a, b = -float(amplitude), float(amplitude)
+        u = u01 * (b - a) + a
+    out = []
+    for k in range(components):
+        seg = u[2 * n * k:2 * n * (k + 1)]
+        out.append((seg[0::2] + 1j * seg[1::2]).reshape(N_r, N_c))
+    return out
