@@ -110,6 +110,19 @@ int phx_run(phx_solver* s, int64_t* n_done);
    Replaces: the inner while loop of main() (main.cu:84-104). */
 int phx_run_interval(phx_solver* s, int64_t* n_done, int* finished);
 
+/* Exactly one step of the configured iterator, without iterate()'s bookkeeping: the noise of
+   the step (drawn with sqrt(*dt)), then the stages at kernel time t (the pre-step time that
+   updateKernelTime uploads before iterate() advances t) and step *dt; with
+   variable_time_step != 0 and an adaptive iterator, rejected attempts are retried at kernel
+   time t + *dt with the step size adaptTimeStep chose, until one is accepted.  On return *dt
+   holds the step size for the next step (unchanged for fixed steps) and *rejected the
+   rejected attempts.  The handle's t, FFT guard and imaginary-time normalisation are not
+   touched (the caller's iterate() does those); its step counter advances by one.
+   Replaces: the virtual Solver::step(bool variable_time_step) (solver.hpp:141), i.e.
+   RungeKutta4::step (runge_kutta.cu:74-94), DormandPrince45::step (dormand_prince.cu:37-95)
+   and the other REGISTER_SOLVER iterators, for a Solver subclass that forwards to this library. */
+int phx_step_at(phx_solver* s, double t, double* dt, int variable_time_step, int64_t* rejected);
+
 /* Envelope sources of a configuration. */
 enum phx_source { PHX_SRC_PUMP = 0, PHX_SRC_POTENTIAL = 1, PHX_SRC_PULSE = 2, PHX_SRC_FFT_MASK = 3 };
 

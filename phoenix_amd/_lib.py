@@ -34,6 +34,7 @@ EXPORTED = [
     "phx_adjoint_create", "phx_adjoint_forward", "phx_adjoint_backward", "phx_adjoint_set_stream", "phx_adjoint_sync",
     "phx_adjoint_destroy",
     "phx_run_interval", "phx_source_groups", "phx_source_field", "phx_source_temporal", "phx_get_output_config",
+    "phx_step_at",
 ]
 PHX_ENS_RESERVOIR, PHX_ENS_PUMP, PHX_ENS_POTENTIAL = 1, 2, 4
 
@@ -88,6 +89,8 @@ def lib():
         L.phx_initialize.argtypes = [vp]
         L.phx_step.argtypes = [vp, i64, C.POINTER(i64)]
         L.phx_run.argtypes = [vp, C.POINTER(i64)]
+        L.phx_run_interval.argtypes = [vp, C.POINTER(i64), C.POINTER(C.c_int)]
+        L.phx_step_at.argtypes = [vp, C.c_double, C.POINTER(C.c_double), C.c_int, C.POINTER(i64)]
         L.phx_apply_fft_filter.argtypes = [vp]
         L.phx_get_field.argtypes = [vp, C.c_int, vp, C.c_size_t]
         L.phx_set_field.argtypes = [vp, C.c_int, vp, C.c_size_t]
@@ -171,6 +174,19 @@ class Solver:
         done = C.c_int64()
         check(lib().phx_run(self._h, C.byref(done)))
         return done.value
+
+    def run_interval(self):
+        """One output interval of the main loop (phx_run_interval): (steps taken, finished)."""
+        done, fin = C.c_int64(), C.c_int()
+        check(lib().phx_run_interval(self._h, C.byref(done), C.byref(fin)))
+        return done.value, bool(fin.value)
+
+    def step_at(self, t: float, dt: float, variable_time_step: bool = False):
+        """One Solver::step at kernel time t and step dt (phx_step_at): (dt for the next step,
+        rejected attempts).  The handle's t is not advanced."""
+        d, rej = C.c_double(dt), C.c_int64()
+        check(lib().phx_step_at(self._h, t, C.byref(d), int(variable_time_step), C.byref(rej)))
+        return d.value, rej.value
 
     def apply_fft_filter(self):
         check(lib().phx_apply_fft_filter(self._h))

@@ -726,19 +726,20 @@ int autotune_rk4(phx_solver* s) {
     return PHX_OK;
 }
 
-/* Solver::iterate (solver.cu:22-70).  Returns 1 if a step was taken, 0 at t_max, <0 on error. */
+/* The step itself: the noise of the step, then Solver::step(variable_time_step) of the
+   configured iterator (runge_kutta.cu:74-94, dormand_prince.cu:37-95, ...) at kernel time
+   t_pre with s->dt; rejected adaptive attempts are retried at kernel time t_adv (the time
+   iterate() has advanced to, which adaptTimeStep's updateKernelTime uploads, solver.cu:97),
+   and s->dt leaves as adaptTimeStep sets it.  No t, iteration or FFT-guard bookkeeping. */
 template <typename real>
-int iterate_t(phx_solver* s, bool force_fixed) {
+int step_body_t(phx_solver* s, double t_pre, double t_adv, bool force_fixed) {
     auto& c = s->cfg;
-    if (s->t >= c.t_max) return 0;
-    const double t_pre = s->t;
     if (c.use_stochastic) {
         /* the noise of this step is drawn once, with sqrt(dt), before any attempt (solver.cu:36-45) */
         using v2 = typename phx::Vec2<real>::type;
         PHX_HIP(phx::launch_noise_field<real>((v2*)s->noise, c.N_c, c.N_r, c.random_seed, (uint64_t)s->iteration,
                                               (real)(c.stochastic_amplitude * std::sqrt(s->dt)), s->stream));
     }
-    s->t = s->rnd(s->t + s->dt);   /* t += dt before the step (solver.cu:50) */
     const bool adaptive = c.adaptive && s->iter && s->iter->can_adapt && !force_fixed;
     const bool imag = c.imag_time_amplitude != 0.0;
     if (s->ssfm) {
@@ -758,7 +759,7 @@ int iterate_t(phx_solver* s, bool force_fixed) {
         int rejections = 0;
         for (;;) {
             /* retries see the kernel time already advanced (updateKernelTime in adaptTimeStep, solver.cu:97) */
-            const double t_k = rejections == 0 ? t_pre : s->t;
+            const double t_k = rejections == 0 ? t_pre : t_adv;
             if (int r = rk_attempt<real>(s, s->iter->adaptive, t_k, s->dt)) return r;
             PHX_HIP(hipMemcpyAsync(s->sums_host, s->sums, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
             PHX_HIP(hipStreamSynchronize(s->stream));
@@ -775,6 +776,18 @@ int iterate_t(phx_solver* s, bool force_fixed) {
         }
         s->cur ^= 1; /* swapBuffers (solver.cu:107-114) */
     }
+    return PHX_OK;
+}
+
+/* Solver::iterate (solver.cu:22-70).  Returns 1 if a step was taken, 0 at t_max, <0 on error. */
+template <typename real>
+int iterate_t(phx_solver* s, bool force_fixed) {
+    auto& c = s->cfg;
+    if (s->t >= c.t_max) return 0;
+    const double t_pre = s->t;
+    s->t = s->rnd(s->t + s->dt);   /* t += dt before the step (solver.cu:50) */
+    if (int r = step_body_t<real>(s, t_pre, s->t, force_fixed)) return r;
+    const bool imag = c.imag_time_amplitude != 0.0;
     if (imag) {
         /* normalizeImaginaryTimePropagation after the step (solver.cu:55-56) */
         using v2 = typename phx::Vec2<real>::type;
@@ -944,6 +957,24 @@ int phx_step(phx_solver* s, int64_t n_steps, int64_t* n_done) {
     return PHX_OK;
 }
 
+int phx_step_at(phx_solver* s, double t, double* dt, int variable_time_step, int64_t* rejected) {
+    if (!s || !s->initialized) return fail(PHX_ERR_STATE, "phx_step_at before phx_initialize");
+    if (!dt || !(*dt > 0.0)) return fail(PHX_ERR_INVALID, "phx_step_at needs dt > 0");
+    const int64_t rej0 = s->rejected;
+    const double t_keep = s->t;
+    s->dt = s->rnd(*dt);
+    const double t_pre = s->rnd(t);
+    const double t_adv = s->rnd(t_pre + s->dt);
+    const int r = s->precision == PHX_FP32 ? step_body_t<float>(s, t_pre, t_adv, variable_time_step == 0)
+                                           : step_body_t<double>(s, t_pre, t_adv, variable_time_step == 0);
+    s->t = t_keep;
+    if (rejected) *rejected = s->rejected - rej0;
+    if (r) return r;
+    s->iteration++; /* the noise stream's step index */
+    *dt = s->dt;
+    return PHX_OK;
+}
+
 /* main.cu:80-117 (render disabled): iterate in chunks of output_every, shortening the
    step that would overshoot the next output time. */
 int phx_run_interval(phx_solver* s, int64_t* n_done, int* finished) {
@@ -952,12 +983,15 @@ int phx_run_interval(phx_solver* s, int64_t* n_done, int* finished) {
     int64_t done = 0;
     if (s->t < c.t_max) {
         /* main.cu:84-104: iterate up to the next output time; the step that would pass it is
-           shortened to land on it and runs with a fixed step */
-        while (s->t < s->out_iter * c.output_every) {
+           shortened to land on it and runs with a fixed step.  The reference evaluates the
+           output time, t + dt and the shortened dt in its real type; out_iter * output_every
+           is exact in double, so rounding it once gives the real-type product. */
+        const double target = s->rnd((double)s->out_iter * c.output_every);
+        while (s->t < target) {
             const double dt = s->dt;
             bool forced = false;
-            if (s->t + s->dt > s->out_iter * c.output_every) {
-                const double next_dt = s->out_iter * c.output_every - s->t;
+            if (s->rnd(s->t + s->dt) > target) {
+                const double next_dt = target - s->t;
                 if (next_dt > 0) {
                     s->dt = s->rnd(next_dt);
                     forced = true;
