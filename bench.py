@@ -186,6 +186,22 @@ def time_spinor(N: int = 2048, steps: int = 40, prec: str = "fp32"):
             "wall_ms": round(1e3 * wall, 2), "finite": finite}
 
 
+def time_large(sizes, steps: int = 100):
+    """The same workload on grids whose state no longer fits the 256 MB Infinity Cache (4096^2: 470 MB per
+    FP32 step, 8192^2: 1.88 GB): µs per RK4 step and the fraction of the HBM roofline, FP32 and FP64."""
+    out = {}
+    for N in sizes:
+        L = 0.5 * (N - 1)
+        for prec in ("fp32", "fp64"):
+            r = time_rk4(prec, N, L, steps, 10, kernel_samples=0)
+            b = (28 if prec == "fp32" else 56) * N * N
+            out[f"{N}_{prec}"] = {"us_per_step": round(r["us_per_step"], 3),
+                                  "hbm_frac": round(b / (r["us_per_step"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                                  "gcell_steps_per_s": round(N * N / (r["us_per_step"] * 1e-6) / 1e9, 2),
+                                  "tile_rows": r["tile_rows"], "prefetch_rows": r["prefetch_rows"], "finite": r["finite"]}
+    return out
+
+
 def ensemble_inputs(i: int, N: int, amp: float, seed: int):
     """Instance i of the ensemble workload: dx = 0.5 um, Gaussian pump of width 10 um at amp * P_th,
     psi0 = 0.01 complex Gaussian noise from the instance's own seed, n0 = 0."""
@@ -313,7 +329,7 @@ def time_adjoint(N: int = 512, steps: int = 2000, K: int = 50, batch: int = 16, 
     finite = bool(torch.isfinite(gP).all() and torch.isfinite(gV).all())
     # plain forward of the same batch: one uniform-batch RK4 launch per step, no checkpoints
     ens = Lb.Ensemble("fp32", ph, [(N, N)] * batch, [(0.5, 0.5)] * batch, reservoir=True, pump=True, potential=True)
-    stream = torch.cuda.current_stream()
+    stream = torch.cuda.Stream()
     ens.set_stream(stream.cuda_stream)
     for b in range(batch):
         ens.set(b, "psi_plus", psi0[b])
@@ -322,9 +338,9 @@ def time_adjoint(N: int = 512, steps: int = 2000, K: int = 50, batch: int = 16, 
         ens.set(b, "potential_plus", V.astype(np.float32))
     ens.step(dt, 100, graph_steps=100)
     torch.cuda.synchronize()
-    ev[0].record()
+    ev[0].record(stream)
     ens.step(dt, steps, graph_steps=100)
-    ev[1].record()
+    ev[1].record(stream)
     torch.cuda.synchronize()
     ms_plain = ev[0].elapsed_time(ev[1])
     ens.close()
@@ -335,29 +351,32 @@ def time_adjoint(N: int = 512, steps: int = 2000, K: int = 50, batch: int = 16, 
             "gradient_over_forward": round((ms_f + ms_b) / ms_plain, 3), "finite": finite}
 
 
-def cpu_baseline(N: int, L: float, seconds: float = 10.0, subgrids: tuple = (4, 4)):
-    """The FP64 oracle (C restatement of the reference CPU path: padded subgrids with halos, OpenMP loop
-    over subgrids as in the reference CPU build) on the host cores, on a bounded sample: a probe of 10
-    steps sets the step count for about `seconds` of work (20 to 2000 steps)."""
-    from oracle import oracle as O
+def cpu_baseline(N: int, L: float, steps: int = 200, warmup: int = 20):
+    """The reference's CPU RK4 step restated in C (oracle/phx_cpu_rk4.c: padded subgrids with 4-cell
+    halos, one halo sync per step, one OpenMP loop over subgrids running K1..K4 with `omp simd` rows;
+    its FP64 build reproduces the FP64 oracle bit for bit), built -O3 -funroll-loops for the host's
+    x86-64 level, on the host cores with OMP_PROC_BIND=true OMP_PLACES=cores.  Per precision the median
+    of `steps` individually timed steps after `warmup`, best of the 4x4, 8x8, 16x16 and 32x32 subgrid
+    splits of the 1024^2 workload.  Runs as a child process so that the OpenMP binding applies."""
+    import subprocess
+    env = dict(os.environ)
     cores = len(os.sched_getaffinity(0))
-    omp = os.environ.get("OMP_NUM_THREADS", "")
-    threads = O.set_threads(min(cores, int(omp)) if omp.isdigit() and int(omp) > 0 else cores)
-    p = O.params(N, N, L, L, periodic=(True, True), pump=True, subgrids=subgrids)
-    psi, n, P = workload_inputs(N, L)
-    t0 = time.perf_counter()
-    O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=5)  # probes: the difference removes the per-call setup
-    t1 = time.perf_counter()
-    O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=15)
-    t2 = time.perf_counter()
-    probe = max((t2 - t1) - (t1 - t0), 1e-6) / 10
-    steps = int(min(5000, max(20, seconds / probe)))
-    t0 = time.perf_counter()
-    O.rk4_run(p, psi, n, pump=P, dt=0.04, steps=steps)
-    dt = time.perf_counter() - t0
-    return dict(value=round(1e6 * dt / steps, 1), unit="µs/step", cores=threads, kind="port",
-                sample=f"FP64 oracle (C, OpenMP over {subgrids[0]}x{subgrids[1]} subgrids with 4-cell halos, "
-                       f"{threads} threads on {cores} host cores), {N}x{N}, {steps} RK4 steps, {dt:.1f} s")
+    env.setdefault("OMP_NUM_THREADS", str(cores))
+    env["OMP_PROC_BIND"] = "true"
+    env["OMP_PLACES"] = "cores"
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_bench.py"), "--size", str(N), "--steps", str(steps),
+                        "--warmup", str(warmup)], env=env, capture_output=True, text=True, timeout=600)
+    if r.returncode != 0:
+        return {"error": r.stderr[-500:]}
+    c = json.loads(r.stdout.strip().splitlines()[-1])
+    return dict(value=c["fp32"]["best_us"], unit="µs/step", cores=c["threads"], kind="port",
+                fp64_value=c["fp64"]["best_us"], per_split={"fp32": c["fp32"]["median_us_per_step"],
+                                                            "fp64": c["fp64"]["median_us_per_step"]},
+                fp64_256x256_1000_steps_ms=c["fp64_256_1000_steps_ms"],
+                sample=f"the reference's CPU RK4 step restated in C (bitwise the FP64 oracle), {N}x{N} workload, "
+                       f"FP32 (value) and FP64, median of {steps} steps after {warmup} warm-up, best of 4x4/8x8/16x16/32x32 "
+                       f"subgrids (FP32 {c['fp32']['best_split']}, FP64 {c['fp64']['best_split']}); {c['threads']} OpenMP "
+                       f"threads, OMP_PROC_BIND=true OMP_PLACES=cores, {c['cpu']}, {c['isa']}, -O3 -funroll-loops")
 
 
 def main():
@@ -366,7 +385,6 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--size", type=int, default=1024)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="wall time of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None, help="time one precision only (profiling)")
     ap.add_argument("--kernel-samples", type=int, default=200, help="single-step launches timed with event pairs")
@@ -375,6 +393,10 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--only-legs", action="store_true",
                     help="run only the batched-ensemble and adjoint legs (profiling); prints their objects")
+    ap.add_argument("--leg", choices=["batched", "adjoint", "spinor", "fft", "large"], default=None,
+                    help="run one leg only (profiling) and print its object")
+    ap.add_argument("--large-sizes", default="4096,8192", help="grid sides of the large-grid leg (comma separated)")
+    ap.add_argument("--no-large", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-spinor", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=2000)
@@ -398,6 +420,15 @@ def main():
     if a.only_legs:
         print(json.dumps({"batched": time_batched(a.batch_instances, a.batch_steps),
                           "adjoint": time_adjoint(steps=a.adjoint_steps)}), flush=True)
+        return
+    large_sizes = [int(x) for x in a.large_sizes.split(",") if x]
+    if a.leg is not None:
+        legs = {"batched": lambda: time_batched(a.batch_instances, a.batch_steps),
+                "adjoint": lambda: time_adjoint(steps=a.adjoint_steps),
+                "spinor": lambda: time_spinor(steps=a.steps),
+                "fft": lambda: {p: time_fft_filter(p, a.size, 0.5 * (a.size - 1)) for p in ("fp32", "fp64")},
+                "large": lambda: time_large(large_sizes, a.steps)}
+        print(json.dumps({a.leg: legs[a.leg]()}), flush=True)
         return
     N = a.size
     L = 0.5 * (N - 1)
@@ -450,8 +481,10 @@ def main():
         out["adjoint"] = time_adjoint(steps=a.adjoint_steps)
     if not a.no_spinor and a.only is None:
         out["spinor_dp45"] = time_spinor()
+    if not a.no_large and a.only is None:
+        out["large_grids"] = time_large(large_sizes, 100)
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(N, L, a.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(N, L)
     print(json.dumps(out), flush=True)
 
 

@@ -183,7 +183,9 @@ int phx_set_time(phx_solver* s, double t, double dt);
 int phx_get_info(phx_solver* s, phx_info* info);
 
 /* Use an external HIP stream (hipStream_t passed as void*) for all work of this
-   handle; NULL restores the handle's own stream. */
+   handle; NULL restores the handle's own stream (so the null stream itself cannot be set: a
+   caller whose work runs on it orders a stream of its own with events).  The same holds for
+   phx_ensemble_set_stream and phx_adjoint_set_stream. */
 int phx_set_stream(phx_solver* s, void* stream);
 
 /* Wait for all device work of the handle. */
@@ -260,7 +262,8 @@ int phx_ensemble_set_tile_rows(phx_ensemble* e, int ty);
    launches step by step.  Asynchronous on the ensemble's stream. */
 int phx_ensemble_step(phx_ensemble* e, double dt, int64_t n_steps, int graph_steps);
 
-/* Run on a caller-owned hipStream_t (NULL: the ensemble's own stream). */
+/* Run on a caller-owned hipStream_t (NULL: the ensemble's own stream).  On a stream that
+   cannot be captured the steps are launched one by one instead of replayed from a hipGraph. */
 int phx_ensemble_set_stream(phx_ensemble* e, void* stream);
 int phx_ensemble_sync(phx_ensemble* e);
 void phx_ensemble_destroy(phx_ensemble* e);
@@ -295,6 +298,8 @@ int phx_adjoint_forward(phx_adjoint* a, const void* psi0, const void* n0, const 
 int phx_adjoint_backward(phx_adjoint* a, const void* g_psi_T, const void* g_n_T, void* g_pump, void* g_pot, void* g_psi0,
                          void* g_n0);
 
+/* Run on a caller-owned hipStream_t (NULL: the handle's own stream).  The backward's recompute
+   stream waits for this stream's earlier work. */
 int phx_adjoint_set_stream(phx_adjoint* a, void* stream);
 int phx_adjoint_sync(phx_adjoint* a);
 void phx_adjoint_destroy(phx_adjoint* a);

@@ -37,6 +37,17 @@ EXPORTED = [
     "phx_step_at",
 ]
 PHX_ENS_RESERVOIR, PHX_ENS_PUMP, PHX_ENS_POTENTIAL = 1, 2, 4
+def stream_arg(stream_ptr: Optional[int]):
+    """hipStream_t for the set_stream entry points, whose NULL selects the handle's own stream.
+    None -> the handle's own stream.  A handle of 0 (torch's default stream) is refused: it would
+    silently select the handle's own stream, unordered with the caller's work on the default stream;
+    such callers run the handle on a stream of their own (torch.cuda.Stream) ordered by events."""
+    if stream_ptr is None:
+        return None
+    if stream_ptr == 0:
+        raise ValueError("stream handle 0 (the default stream) cannot be set on a handle: pass None for the "
+                         "handle's own stream, or a torch.cuda.Stream ordered with the default stream")
+    return stream_ptr
 
 
 class PhxError(RuntimeError):
@@ -222,7 +233,7 @@ class Solver:
         check(lib().phx_set_time(self._h, t, dt))
 
     def set_stream(self, stream_ptr: Optional[int]):
-        check(lib().phx_set_stream(self._h, stream_ptr))
+        check(lib().phx_set_stream(self._h, stream_arg(stream_ptr)))
 
     def sync(self):
         check(lib().phx_sync(self._h))
@@ -314,7 +325,7 @@ class Ensemble:
         check(lib().phx_ensemble_step(self._h, dt, n, graph_steps))
 
     def set_stream(self, stream_ptr: Optional[int]):
-        check(lib().phx_ensemble_set_stream(self._h, stream_ptr))
+        check(lib().phx_ensemble_set_stream(self._h, stream_arg(stream_ptr)))
 
     def sync(self):
         check(lib().phx_ensemble_sync(self._h))
@@ -354,7 +365,7 @@ class Adjoint:
         check(lib().phx_adjoint_backward(self._h, g_psi_T, g_n_T, g_pump, g_pot, g_psi0, g_n0))
 
     def set_stream(self, stream_ptr: Optional[int]):
-        check(lib().phx_adjoint_set_stream(self._h, stream_ptr))
+        check(lib().phx_adjoint_set_stream(self._h, stream_arg(stream_ptr)))
 
     def sync(self):
         check(lib().phx_adjoint_sync(self._h))

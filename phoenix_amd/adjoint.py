@@ -43,6 +43,39 @@ def _physics_key(ph: _lib.PhxPhysics):
     return tuple(getattr(ph, k) for k, _ in _lib.PhxPhysics._fields_)
 
 
+class _EngineStream:
+    """Run the engine on torch's current stream, or, when that is the default stream (handle 0, which
+    the handles' set_stream cannot take), on a side stream ordered after the current stream's work and
+    before its later work by events; tensors the engine touches are recorded on the side stream so the
+    caching allocator does not hand their memory out while the engine may still use it."""
+
+    _side = None
+
+    def __init__(self, engine, *tensors):
+        self.engine, self.tensors = engine, tensors
+
+    def __enter__(self):
+        self.cur = torch.cuda.current_stream()
+        if self.cur.cuda_stream != 0:
+            self.engine.set_stream(self.cur.cuda_stream)
+            self.side = None
+            return self
+        if _EngineStream._side is None:
+            _EngineStream._side = torch.cuda.Stream()
+        self.side = _EngineStream._side
+        self.side.wait_stream(self.cur)
+        self.engine.set_stream(self.side.cuda_stream)
+        return self
+
+    def __exit__(self, *exc):
+        if self.side is not None:
+            for t in self.tensors:
+                if t is not None:
+                    t.record_stream(self.side)
+            self.cur.wait_stream(self.side)
+        return False
+
+
 class RK4Adjoint(torch.autograd.Function):
     """(P, V, psi0, n0) -> (psi_T, n_T) with a hand-written adjoint backward."""
 
@@ -53,8 +86,8 @@ class RK4Adjoint(torch.autograd.Function):
                 raise ValueError("rk4_evolve expects contiguous tensors on the HIP device")
         psi_T = torch.empty_like(psi0)
         n_T = torch.empty_like(n0)
-        engine.set_stream(torch.cuda.current_stream().cuda_stream)
-        engine.forward(psi0.data_ptr(), n0.data_ptr(), pump.data_ptr(), pot.data_ptr(), psi_T.data_ptr(), n_T.data_ptr())
+        with _EngineStream(engine, pump, pot, psi0, n0, psi_T, n_T):
+            engine.forward(psi0.data_ptr(), n0.data_ptr(), pump.data_ptr(), pot.data_ptr(), psi_T.data_ptr(), n_T.data_ptr())
         engine.generation = getattr(engine, "generation", 0) + 1
         ctx.engine = engine
         ctx.generation = engine.generation
@@ -76,16 +109,17 @@ class RK4Adjoint(torch.autograd.Function):
         gpsi = torch.empty_like(psi0) if ctx.needs_input_grad[2] else None
         gn = torch.empty_like(n0) if ctx.needs_input_grad[3] else None
         e = ctx.engine
-        e.set_stream(torch.cuda.current_stream().cuda_stream)
-        if e.generation != ctx.generation:
-            # another forward pass reused the checkpoint store since: rebuild this pass's checkpoints
-            e.forward(psi0.data_ptr(), n0.data_ptr(), pump.data_ptr(), pot.data_ptr(), torch.empty_like(psi0).data_ptr(),
-                      torch.empty_like(n0).data_ptr())
-            e.generation += 1
-            ctx.generation = e.generation
-        e.backward(g_psi_T.data_ptr(), g_n_ptr, gP.data_ptr() if gP is not None else None,
-                   gV.data_ptr() if gV is not None else None, gpsi.data_ptr() if gpsi is not None else None,
-                   gn.data_ptr() if gn is not None else None)
+        scratch_psi, scratch_n = torch.empty_like(psi0), torch.empty_like(n0)
+        with _EngineStream(e, pump, pot, psi0, n0, g_psi_T, g_n_T, gP, gV, gpsi, gn, scratch_psi, scratch_n):
+            if e.generation != ctx.generation:
+                # another forward pass reused the checkpoint store since: rebuild this pass's checkpoints
+                e.forward(psi0.data_ptr(), n0.data_ptr(), pump.data_ptr(), pot.data_ptr(), scratch_psi.data_ptr(),
+                          scratch_n.data_ptr())
+                e.generation += 1
+                ctx.generation = e.generation
+            e.backward(g_psi_T.data_ptr(), g_n_ptr, gP.data_ptr() if gP is not None else None,
+                       gV.data_ptr() if gV is not None else None, gpsi.data_ptr() if gpsi is not None else None,
+                       gn.data_ptr() if gn is not None else None)
         return gP, gV, gpsi, gn, None, None
 
 

@@ -36,11 +36,11 @@ HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unuse
 # unrolled completely; the 7-stage spinor kernels exceed LLVM's default pragma threshold.
 _UNROLL = ["-mllvm", "-pragma-unroll-threshold=2000000"]
 HIP_FLAGS_FILE = {f: _UNROLL for f in ("phx_rk_generic.hip", "phx_rk_tab3.hip", "phx_rk_tab4.hip", "phx_rk_tab6.hip",
-                                     "phx_rk_tab6f.hip")}
+                                     "phx_rk_tab6f.hip", "phx_rk_dp45t.hip")}
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include")]
 
 # longest compiles first: the thread pool then finishes them side by side
-HIP_SOURCES = ["phx_rk4_f64.hip", "phx_rk4.hip", "phx_rk_generic.hip", "phx_rk_tab6.hip", "phx_rk_tab6f.hip", "phx_rk_tab4.hip",
+HIP_SOURCES = ["phx_rk_generic.hip", "phx_rk_dp45t.hip", "phx_rk4_f64.hip", "phx_rk4.hip", "phx_rk_tab6.hip", "phx_rk_tab6f.hip", "phx_rk_tab4.hip",
                "phx_rk_tab3.hip", "phx_fft.hip", "phx_solver.hip", "phx_ensemble.hip", "phx_adjoint.hip", "phx_ssfm.hip",
                "phx_staged.hip"]
 CXX_SOURCES = ["phx_config.cpp"]
@@ -53,13 +53,28 @@ def _run(cmd):
     return r
 
 
+def _includes(path: str, seen: set) -> set:
+    """Files reached from `path` through quoted #include lines (the library's own headers)."""
+    import re
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for m in re.finditer(r'^\s*#\s*include\s+"([^"]+)"', f.read(), re.M):
+            for base in (os.path.dirname(path), os.path.join(ROOT, "include")):
+                cand = os.path.normpath(os.path.join(base, m.group(1)))
+                if os.path.exists(cand):
+                    _includes(cand, seen)
+                    break
+    return seen
+
+
 def _stale(obj: str, src: str) -> bool:
-    """Object missing or older than its source or any header of the library."""
+    """Object missing or older than its source, this script or a header the source includes."""
     if not os.path.exists(obj):
         return True
     t = os.path.getmtime(obj)
-    deps = [src, __file__] + [os.path.join(d, f) for d in (CSRC, os.path.join(ROOT, "include"))
-                              for f in os.listdir(d) if f.endswith((".h", ".hpp"))]
+    deps = _includes(src, set()) | {__file__}
     return any(os.path.getmtime(d) > t for d in deps)
 
 

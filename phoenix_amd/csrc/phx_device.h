@@ -151,8 +151,15 @@ __device__ __forceinline__ void store_real_s(rsrc_t r, uint32_t off, uint32_t so
 __device__ __forceinline__ void store_cplx_s(rsrc_t r, uint32_t off, uint32_t soff, float2 v) {
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(phx_v2i, v), r, off, soff, 0);
 }
+/* A buffer store of more than 8 bytes reads its data registers after it issues.  The
+   compiler separates it from a following VALU write of those registers only when its
+   soffset is not an SGPR; with an SGPR soffset the 16-byte store has written the
+   overwriting value on gfx950 (the real part of a complex FP64 cell replaced by the next
+   FMA's result, intermittently).  So the row offset joins the per-lane offset and soffset
+   stays 0.  Out-of-range lanes stay out of range: kOutOfRange + soff < 2^32 for every
+   soff < kMaxArrayBytes, and a descriptor of size 0 drops any offset. */
 __device__ __forceinline__ void store_cplx_s(rsrc_t r, uint32_t off, uint32_t soff, double2 v) {
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(phx_v4i, v), r, off, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(phx_v4i, v), r, off + soff, 0, 0);
 }
 
 /* Largest array the whole-array descriptors address: offsets are 32-bit and the

@@ -123,6 +123,8 @@ int build_tables(phx_ensemble* e, int ty) {
             ent[par].push_back(en);
         }
     }
+    /* launches still queued on the stream read the old tables: let them finish first */
+    if (e->stream) ENS_HIP(hipStreamSynchronize(e->stream));
     for (int par = 0; par < 2; par++) {
         if (e->entries[par]) ENS_HIP(hipFree(e->entries[par]));
         ENS_HIP(hipMalloc(&e->entries[par], ent[par].size() * sizeof(ent[par][0])));
@@ -199,20 +201,23 @@ int step_t(phx_ensemble* e, double dt, int64_t n_steps, int graph_steps) {
         if (int r = build_tables<real>(e, e->ty)) return r;
     int64_t done = 0;
     graph_steps &= ~1; /* even: the buffers return to the same parity */
-    if (graph_steps >= 2 && n_steps >= graph_steps) { /* even: the buffers return to the same parity */
-        /* the captured launches hard-wire the buffer parity they start from */
-        if (!e->graph || e->graph_steps != graph_steps || e->graph_dt != dt || e->graph_ty != e->ty || e->graph_parity != e->cur) {
-            if (e->graph) ENS_HIP(hipGraphExecDestroy(e->graph));
-            e->graph = nullptr;
-            hipGraph_t g;
-            ENS_HIP(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    bool replay = graph_steps >= 2 && n_steps >= graph_steps;
+    /* the captured launches hard-wire the buffer parity they start from */
+    if (replay && (!e->graph || e->graph_steps != graph_steps || e->graph_dt != dt || e->graph_ty != e->ty ||
+                   e->graph_parity != e->cur)) {
+        if (e->graph) ENS_HIP(hipGraphExecDestroy(e->graph));
+        e->graph = nullptr;
+        if (hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            /* a stream that cannot be captured (the legacy default stream): launch by launch */
+            (void)hipGetLastError();
+            replay = false;
+        } else {
             int rc = PHX_OK;
             for (int k = 0; k < graph_steps && !rc; k++) rc = launch_step<real>(e, dt, (e->cur + k) & 1);
-            hipGraph_t gg;
-            const hipError_t ce = hipStreamEndCapture(e->stream, &gg);
+            hipGraph_t g = nullptr;
+            const hipError_t ce = hipStreamEndCapture(e->stream, &g);
             if (rc) return rc;
             ENS_HIP(ce);
-            g = gg;
             ENS_HIP(hipGraphInstantiate(&e->graph, g, nullptr, nullptr, 0));
             ENS_HIP(hipGraphDestroy(g));
             e->graph_steps = graph_steps;
@@ -220,6 +225,8 @@ int step_t(phx_ensemble* e, double dt, int64_t n_steps, int graph_steps) {
             e->graph_ty = e->ty;
             e->graph_parity = e->cur;
         }
+    }
+    if (replay) {
         while (n_steps - done >= graph_steps) {
             ENS_HIP(hipGraphLaunch(e->graph, e->stream));
             done += graph_steps;
@@ -301,9 +308,12 @@ int phx_ensemble_create(phx_ensemble** out, int precision, const phx_physics* ph
         e->off_pot.push_back(total);
         if (e->pot) total = align256(total + N * e->rsize);
     }
-    if (hipMalloc(&e->pool, total) != hipSuccess || hipMemset(e->pool, 0, total) != hipSuccess ||
-        hipStreamCreate(&e->own_stream) != hipSuccess) {
+    /* the pool is zeroed on the ensemble's own stream and waited for, so no later upload or step
+       (on this or any other stream) can run before the fill has finished */
+    if (hipMalloc(&e->pool, total) != hipSuccess || hipStreamCreate(&e->own_stream) != hipSuccess ||
+        hipMemsetAsync(e->pool, 0, total, e->own_stream) != hipSuccess || hipStreamSynchronize(e->own_stream) != hipSuccess) {
         if (e->pool) (void)hipFree(e->pool);
+        if (e->own_stream) (void)hipStreamDestroy(e->own_stream);
         delete e;
         return efail(PHX_ERR_HIP, "allocation of " + std::to_string(total) + " bytes failed");
     }

@@ -42,6 +42,14 @@ __device__ __forceinline__ Cx<real> cmul(Cx<real> a, Cx<real> w) {
     return {phx_fma(a.r, w.r, -(a.i * w.i)), phx_fma(a.r, w.i, a.i * w.r)};
 }
 
+/* LDS slot of element i: one spare slot after every 32 elements (PAD), so that the strided
+   Stockham writes of the early passes spread over all banks instead of a few. */
+template <bool PAD>
+__device__ __forceinline__ int lslot(int i) { return PAD ? i + (i >> 5) : i; }
+/* LDS elements that hold n elements */
+template <bool PAD>
+__host__ __device__ constexpr size_t lds_elems(size_t n) { return PAD ? n + n / 32 + 1 : n; }
+
 /* Butterflies a thread may own in one pass: the launchers pick KB, the smallest
    power of two with count * N / 4 <= KB * threads, up to 8 in FP32 and 4 in FP64,
    so the staged inputs stay in registers. */
@@ -54,7 +62,7 @@ __host__ __device__ constexpr int max_bfly() { return sizeof(real) == 4 ? 8 : 4;
    synchronises, and then the outputs are written to their Stockham positions in the
    same buffer.  One buffer instead of two doubles the sequences a workgroup holds.
    sign = -1 forward, +1 inverse (conjugated twiddles). */
-template <typename real, int SIGN, int KB>
+template <typename real, int SIGN, int KB, bool PAD>
 __device__ void stockham(Cx<real>* buf, const Cx<real>* __restrict__ tw, int N, int log2N, int count) {
     const int nb = N >> 2; /* radix-4 butterflies per sequence */
     const int total = nb * count;
@@ -69,11 +77,11 @@ __device__ void stockham(Cx<real>* buf, const Cx<real>* __restrict__ tw, int N, 
             if (b < total) {
                 const int seq = b / nb, bb = b - seq * nb;
                 const int p = bb >> log2s, q = bb & (s - 1);
-                const Cx<real>* x = buf + seq * N;
-                in[k][0] = x[q + s * p];
-                in[k][1] = x[q + s * (p + m)];
-                in[k][2] = x[q + s * (p + 2 * m)];
-                in[k][3] = x[q + s * (p + 3 * m)];
+                const int x = seq * N;
+                in[k][0] = buf[lslot<PAD>(x + q + s * p)];
+                in[k][1] = buf[lslot<PAD>(x + q + s * (p + m))];
+                in[k][2] = buf[lslot<PAD>(x + q + s * (p + 2 * m))];
+                in[k][3] = buf[lslot<PAD>(x + q + s * (p + 3 * m))];
                 /* the pass's twiddles are read before the barrier, so their latency overlaps it */
                 wt[k][0] = tw[p << stepshift];
                 wt[k][1] = tw[(2 * p) << stepshift];
@@ -87,7 +95,7 @@ __device__ void stockham(Cx<real>* buf, const Cx<real>* __restrict__ tw, int N, 
             if (b < total) {
                 const int seq = b / nb, bb = b - seq * nb;
                 const int p = bb >> log2s, q = bb & (s - 1);
-                Cx<real>* y = buf + seq * N;
+                const int y = seq * N;
                 const Cx<real> a = in[k][0], bv = in[k][1], c = in[k][2], d = in[k][3];
                 Cx<real> w1 = wt[k][0], w2 = wt[k][1], w3 = wt[k][2];
                 if (SIGN > 0) {
@@ -99,10 +107,10 @@ __device__ void stockham(Cx<real>* buf, const Cx<real>* __restrict__ tw, int N, 
                 const Cx<real> bpd = {bv.r + d.r, bv.i + d.i};
                 /* j(b - d) with j = -i (forward) or +i (inverse) */
                 const Cx<real> jbmd = SIGN < 0 ? Cx<real>{bv.i - d.i, -(bv.r - d.r)} : Cx<real>{-(bv.i - d.i), bv.r - d.r};
-                y[q + s * (4 * p)] = {apc.r + bpd.r, apc.i + bpd.i};
-                y[q + s * (4 * p + 1)] = cmul(Cx<real>{amc.r + jbmd.r, amc.i + jbmd.i}, w1);
-                y[q + s * (4 * p + 2)] = cmul(Cx<real>{apc.r - bpd.r, apc.i - bpd.i}, w2);
-                y[q + s * (4 * p + 3)] = cmul(Cx<real>{amc.r - jbmd.r, amc.i - jbmd.i}, w3);
+                buf[lslot<PAD>(y + q + s * (4 * p))] = {apc.r + bpd.r, apc.i + bpd.i};
+                buf[lslot<PAD>(y + q + s * (4 * p + 1))] = cmul(Cx<real>{amc.r + jbmd.r, amc.i + jbmd.i}, w1);
+                buf[lslot<PAD>(y + q + s * (4 * p + 2))] = cmul(Cx<real>{apc.r - bpd.r, apc.i - bpd.i}, w2);
+                buf[lslot<PAD>(y + q + s * (4 * p + 3))] = cmul(Cx<real>{amc.r - jbmd.r, amc.i - jbmd.i}, w3);
             }
         }
         __syncthreads();
@@ -116,16 +124,16 @@ __device__ void stockham(Cx<real>* buf, const Cx<real>* __restrict__ tw, int N, 
         for (int b = threadIdx.x; b < s * count; b += blockDim.x) {
             const int seq = b >> log2s;
             const int q = b & (s - 1);
-            Cx<real>* x = buf + seq * N;
-            const Cx<real> a = x[q], bv = x[q + s];
-            x[q] = {a.r + bv.r, a.i + bv.i};
-            x[q + s] = {a.r - bv.r, a.i - bv.i};
+            const int x = seq * N;
+            const Cx<real> a = buf[lslot<PAD>(x + q)], bv = buf[lslot<PAD>(x + q + s)];
+            buf[lslot<PAD>(x + q)] = {a.r + bv.r, a.i + bv.i};
+            buf[lslot<PAD>(x + q + s)] = {a.r - bv.r, a.i - bv.i};
         }
         __syncthreads();
     }
 }
 
-template <typename real, int SIGN, int KB>
+template <typename real, int SIGN, int KB, bool PAD>
 __global__ __launch_bounds__(kFftThreads) void fft_rows(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw, int N, int log2N) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     Cx<real>* A = reinterpret_cast<Cx<real>*>(lds_raw);
@@ -139,17 +147,18 @@ __global__ __launch_bounds__(kFftThreads) void fft_rows(typename Vec2<real>::typ
 #pragma unroll
     for (int k = 0; k < PT; k++) {
         const int i = threadIdx.x + k * kFftThreads;
-        if (i < N) A[i] = {v[k].x, v[k].y};
+        if (i < N) A[lslot<PAD>(i)] = {v[k].x, v[k].y};
     }
     __syncthreads();
-    stockham<real, SIGN, KB>(A, tw, N, log2N, 1);
+    stockham<real, SIGN, KB, PAD>(A, tw, N, log2N, 1);
 #pragma unroll
     for (int k = 0; k < PT; k++) {
         const int i = threadIdx.x + k * kFftThreads;
         if (i < N) {
             typename Vec2<real>::type o;
-            o.x = A[i].r;
-            o.y = A[i].i;
+            const Cx<real> f = A[lslot<PAD>(i)];
+            o.x = f.r;
+            o.y = f.i;
             row[i] = o;
         }
     }
@@ -172,7 +181,7 @@ enum { kColFilter = 0, kColPropagate = 1, kColForward = 2 };
                        reference keeps it in fft_plus / fft_minus (solver_fft.cu:67-100);
    MODE kColPropagate: F <- F * colf[c] * rowf[r] (the split-step propagator; the tables carry 1 / (N_c N_r));
    MODE kColForward:   the forward transform only (data becomes the unnormalised spectrum). */
-template <typename real, int CB, int KB, int MODE>
+template <typename real, int CB, int KB, int MODE, bool PAD>
 __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw,
                                                               const real* __restrict__ mask, const Cx<real>* __restrict__ rowf,
                                                               const Cx<real>* __restrict__ colf, int Nc, int Nr, int log2Nr,
@@ -196,11 +205,11 @@ __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<rea
         for (int k = 0; k < PT; k++) {
             const int i = threadIdx.x + k * kFftThreads;
             const int r = i / CB, j = i - r * CB;
-            if (i < cnt) A[j * Nr + r] = {v[k].x, v[k].y};
+            if (i < cnt) A[lslot<PAD>(j * Nr + r)] = {v[k].x, v[k].y};
         }
     }
     __syncthreads();
-    stockham<real, -1, KB>(A, tw, Nr, log2Nr, CB);
+    stockham<real, -1, KB, PAD>(A, tw, Nr, log2Nr, CB);
     if constexpr (MODE == kColForward) {
     } else if constexpr (MODE == kColPropagate) {
         /* factor tables read before they are used (all loads in flight together) */
@@ -212,7 +221,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<rea
             const int i = threadIdx.x + k * kFftThreads;
             const int r = i / CB, j = i - r * CB;
             if (i < cnt) {
-                Cx<real>& f = A[j * Nr + r];
+                Cx<real>& f = A[lslot<PAD>(j * Nr + r)];
                 f = cmul(cmul(f, colf[c0 + j]), rf[k]);
             }
         }
@@ -233,7 +242,7 @@ __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<rea
             const int i = threadIdx.x + k * kFftThreads;
             const int r = i / CB, j = i - r * CB;
             if (i < cnt) {
-                Cx<real>& f = A[j * Nr + r];
+                Cx<real>& f = A[lslot<PAD>(j * Nr + r)];
                 f.r = f.r / Ntot * mk[k];
                 f.i = f.i / Ntot * mk[k];
                 if (spec) {
@@ -246,15 +255,16 @@ __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<rea
         }
     }
     __syncthreads();
-    if constexpr (MODE != kColForward) stockham<real, +1, KB>(A, tw, Nr, log2Nr, CB);
+    if constexpr (MODE != kColForward) stockham<real, +1, KB, PAD>(A, tw, Nr, log2Nr, CB);
 #pragma unroll
     for (int k = 0; k < PT; k++) {
         const int i = threadIdx.x + k * kFftThreads;
         const int r = i / CB, j = i - r * CB;
         if (i < cnt) {
             typename Vec2<real>::type o;
-            o.x = A[j * Nr + r].r;
-            o.y = A[j * Nr + r].i;
+            const Cx<real> f = A[lslot<PAD>(j * Nr + r)];
+            o.x = f.r;
+            o.y = f.i;
             data[(size_t)r * Nc + c0 + j] = o;
         }
     }
@@ -409,20 +419,56 @@ static int bfly_bound(int butterflies) {
     return kb;
 }
 
-template <typename real, int SIGN>
-static hipError_t launch_rows(typename Vec2<real>::type* psi, const void* tw_c, int Nc, int Nr, hipStream_t s) {
-    const size_t lds = (size_t)Nc * sizeof(Cx<real>);
+/* PHX_FFT_PAD=0 keeps the unpadded LDS layout (tuning runs); padding is used where the padded
+   buffer fits the 64 KiB a workgroup may hold */
+static bool pad_lds(size_t elems, size_t elem_size) {
+    static const bool env_off = [] { const char* e = std::getenv("PHX_FFT_PAD"); return e && std::atoi(e) == 0; }();
+    return !env_off && lds_elems<true>(elems) * elem_size <= 65536;
+}
+
+template <typename real, int SIGN, bool PAD>
+static hipError_t launch_rows_p(typename Vec2<real>::type* psi, const void* tw_c, int Nc, int Nr, hipStream_t s) {
+    const size_t lds = lds_elems<PAD>((size_t)Nc) * sizeof(Cx<real>);
     const int kb = bfly_bound(Nc / 4);
     const auto* tw = (const Cx<real>*)tw_c;
     const int l = ilog2(Nc);
     if (kb == 1)
-        hipLaunchKernelGGL((fft_rows<real, SIGN, 1>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
+        hipLaunchKernelGGL((fft_rows<real, SIGN, 1, PAD>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
     else if (kb == 2)
-        hipLaunchKernelGGL((fft_rows<real, SIGN, 2>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
+        hipLaunchKernelGGL((fft_rows<real, SIGN, 2, PAD>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
     else if (kb == 4)
-        hipLaunchKernelGGL((fft_rows<real, SIGN, 4>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
+        hipLaunchKernelGGL((fft_rows<real, SIGN, 4, PAD>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
     else if constexpr (max_bfly<real>() >= 8)
-        hipLaunchKernelGGL((fft_rows<real, SIGN, 8>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
+        hipLaunchKernelGGL((fft_rows<real, SIGN, 8, PAD>), dim3(Nr), dim3(kFftThreads), lds, s, psi, tw, Nc, l);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <typename real, int SIGN>
+static hipError_t launch_rows(typename Vec2<real>::type* psi, const void* tw_c, int Nc, int Nr, hipStream_t s) {
+    if (pad_lds((size_t)Nc, sizeof(Cx<real>))) return launch_rows_p<real, SIGN, true>(psi, tw_c, Nc, Nr, s);
+    return launch_rows_p<real, SIGN, false>(psi, tw_c, Nc, Nr, s);
+}
+
+template <typename real, int CB, int MODE, bool PAD>
+static hipError_t launch_cols_p(typename Vec2<real>::type* psi, const void* tw_r, const real* mask, const void* rowf, const void* colf,
+                                int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec) {
+    const size_t lds = lds_elems<PAD>((size_t)CB * Nr) * sizeof(Cx<real>);
+    const int kb = bfly_bound(CB * (Nr / 4));
+    const auto* tw = (const Cx<real>*)tw_r;
+    const auto* rf = (const Cx<real>*)rowf;
+    const auto* cf = (const Cx<real>*)colf;
+    const int l = ilog2(Nr);
+    const real ntot = (real)((double)Nc * (double)Nr);
+    if (kb == 1)
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 1, MODE, PAD>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
+    else if (kb == 2)
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 2, MODE, PAD>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
+    else if (kb == 4)
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 4, MODE, PAD>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
+    else if constexpr (max_bfly<real>() >= 8)
+        hipLaunchKernelGGL((fft_cols_filter<real, CB, 8, MODE, PAD>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -431,24 +477,9 @@ static hipError_t launch_rows(typename Vec2<real>::type* psi, const void* tw_c, 
 template <typename real, int CB, int MODE>
 static hipError_t launch_cols(typename Vec2<real>::type* psi, const void* tw_r, const real* mask, const void* rowf, const void* colf,
                               int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec) {
-    const size_t lds = (size_t)CB * Nr * sizeof(Cx<real>);
-    const int kb = bfly_bound(CB * (Nr / 4));
-    const auto* tw = (const Cx<real>*)tw_r;
-    const auto* rf = (const Cx<real>*)rowf;
-    const auto* cf = (const Cx<real>*)colf;
-    const int l = ilog2(Nr);
-    const real ntot = (real)((double)Nc * (double)Nr);
-    if (kb == 1)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 1, MODE>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
-    else if (kb == 2)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 2, MODE>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
-    else if (kb == 4)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 4, MODE>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
-    else if constexpr (max_bfly<real>() >= 8)
-        hipLaunchKernelGGL((fft_cols_filter<real, CB, 8, MODE>), dim3(Nc / CB), dim3(kFftThreads), lds, s, psi, tw, mask, rf, cf, Nc, Nr, l, ntot, spec);
-    else
-        return hipErrorInvalidValue;
-    return hipGetLastError();
+    if (pad_lds((size_t)CB * Nr, sizeof(Cx<real>)))
+        return launch_cols_p<real, CB, MODE, true>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
+    return launch_cols_p<real, CB, MODE, false>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
 }
 
 /* rows forward -> columns (forward, k-space factor, inverse) -> rows inverse; kColForward stops

@@ -161,6 +161,12 @@ template <typename real>
 hipError_t launch_rk_tab6(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
 template <typename real>
 hipError_t launch_rk_tab6f(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
+/* adaptive DP45, TE-TM, FP32, tile height 32 or 42 (phx_rk_dp45t.hip) */
+hipError_t launch_dp45_tetm_tall(const RkGenArgs<float>& a, int ty, hipStream_t s);
+/* tile height of the adaptive DP45 kernel for a grid: the height among the compiled ones that
+   minimises (wavefront tiles per SIMD, rounded up) x (row evaluations per tile), a VALU-bound
+   model; PHX_GEN_TY fixes it (16, 32 or 42) */
+int dp45_adaptive_tile_rows(bool tetm, bool fp32, int Nc, int Nr);
 
 /* Staged Runge-Kutta step (phx_staged.hip): one launch per CALCULATE_K, INTERMEDIATE_SUM_K
    and FINAL_SUM_K / ERROR_K of the reference's sequence, K's in HBM on a grid padded by H
@@ -234,10 +240,12 @@ template <typename real>
 hipError_t launch_noise_field(typename Vec2<real>::type* xi, int Nc, int Nr, uint64_t seed, uint64_t step, real amp,
                               hipStream_t s);
 
-/* wavefront tiles of the generic kernel for a grid (partials buffer size = 2 * this) */
-inline int rk_generic_waves(RkTableau tab, int Nc, int Nr) {
+/* wavefront tiles of the generic kernel for a grid at tile height ty; the partials buffer of an
+   adaptive tableau holds 2 doubles per tile at the smallest height the launcher uses (16) */
+constexpr int kRkGenMinTileRows = 16;
+inline int rk_generic_waves(RkTableau tab, int Nc, int Nr, int ty = kRkGenMinTileRows) {
     const int S = rk_tableau_stages(tab);
-    const int outw = 64 - 2 * S, ty = 16;
+    const int outw = 64 - 2 * S;
     return ((Nc + outw - 1) / outw) * ((Nr + ty - 1) / ty);
 }
 

@@ -283,14 +283,16 @@ inline hipError_t launch_tys(const Rk4Args<real>& a, const Rk4Tune& t, hipStream
     }
 }
 
-/* Rows loaded ahead: 2, or 3 / 4 for the pumped reservoir model with dx == dy
+/* Rows loaded ahead: 2, or 3, 4, 6 or 8 for the pumped reservoir model with dx == dy
    (the benchmark configuration; chosen through Rk4Tune::pf). */
 template <typename real, bool RES, bool PUMP, bool POT>
 inline hipError_t launch_flags(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s) {
     const bool iso = a.ph.kin_yx == real(1);
     if constexpr (RES && PUMP && !POT) {
         if (iso && t.pf == 3) return launch_tys<real, true, 3, RES, PUMP, POT>(a, t, s);
-        if (iso && t.pf >= 4) return launch_tys<real, true, 4, RES, PUMP, POT>(a, t, s);
+        if (iso && (t.pf == 4 || t.pf == 5)) return launch_tys<real, true, 4, RES, PUMP, POT>(a, t, s);
+        if (iso && (t.pf == 6 || t.pf == 7)) return launch_tys<real, true, 6, RES, PUMP, POT>(a, t, s);
+        if (iso && t.pf >= 8) return launch_tys<real, true, 8, RES, PUMP, POT>(a, t, s);
     }
     return iso ? launch_tys<real, true, 2, RES, PUMP, POT>(a, t, s) : launch_tys<real, false, 2, RES, PUMP, POT>(a, t, s);
 }

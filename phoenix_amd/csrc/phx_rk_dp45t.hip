@@ -1,0 +1,26 @@
+/*
+ * phx_rk_dp45t.hip -- the TE-TM instantiations of the adaptive Dormand-Prince 5(4) kernel in FP32
+ * with tall tiles (32 and 42 output rows), in a translation unit of their own so that they compile
+ * next to the others.  The tile height is chosen by launch_rk_generic (phx_rk_generic.hip).
+ */
+#include "phx_rk_generic.h"
+
+namespace phx {
+
+template <int TY>
+static hipError_t launch_tetm(const RkGenArgs<float>& a, hipStream_t s) {
+    const bool res = a.n_in[0] != nullptr, st = a.stochastic;
+    if (res && st) return launch_gen<float, TabDP45Adaptive, TY, true, true, true>(a, s);
+    if (res) return launch_gen<float, TabDP45Adaptive, TY, true, true, false>(a, s);
+    if (st) return launch_gen<float, TabDP45Adaptive, TY, true, false, true>(a, s);
+    return launch_gen<float, TabDP45Adaptive, TY, true, false, false>(a, s);
+}
+
+hipError_t launch_dp45_tetm_tall(const RkGenArgs<float>& a, int ty, hipStream_t s) {
+    if (!a.tetm) return hipErrorInvalidValue;
+    if (ty == 32) return launch_tetm<32>(a, s);
+    if (ty == 42) return launch_tetm<42>(a, s);
+    return hipErrorInvalidValue;
+}
+
+} // namespace phx

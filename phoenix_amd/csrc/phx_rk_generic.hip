@@ -6,6 +6,9 @@
  * imaginary-time normalisation, and the dispatch over all tableaux (the other groups are instantiated in phx_rk_tab3.hip,
  * phx_rk_tab4.hip, phx_rk_tab6.hip and phx_rk_tab6f.hip).
  */
+#include <algorithm>
+#include <cstdlib>
+
 #include "phx_rk_generic.h"
 
 namespace phx {
@@ -135,18 +138,39 @@ hipError_t launch_noise_field(typename Vec2<real>::type* xi, int Nc, int Nr, uin
 template hipError_t launch_noise_field<float>(float2*, int, int, uint64_t, uint64_t, float, hipStream_t);
 template hipError_t launch_noise_field<double>(double2*, int, int, uint64_t, uint64_t, double, hipStream_t);
 
+int dp45_adaptive_tile_rows(bool tetm, bool fp32, int Nc, int Nr) {
+    if (!(tetm && fp32)) return kRkGenMinTileRows; /* the tall tiles exist for the FP32 spinor kernel */
+    const char* e = std::getenv("PHX_GEN_TY"); /* read at every launch: tests switch it within a process */
+    const int env_ty = e ? std::atoi(e) : 0;
+    if (env_ty == 16 || env_ty == 32 || env_ty == 42) return env_ty;
+    static const int simds = [] {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+            cus = 256;
+        return 4 * std::max(cus, 1);
+    }();
+    /* the kernel is bound by VALU issue (two waves per SIMD at its register count): a SIMD's time is
+       its number of tiles times the row evaluations of one tile, S (TY + 2 S) - S (S + 1) for S = 7 */
+    constexpr int S = 7;
+    int best = kRkGenMinTileRows;
+    long best_cost = -1;
+    for (int ty : {16, 32, 42}) {
+        const long waves = (long)((Nc + 63 - 2 * S) / (64 - 2 * S)) * ((Nr + ty - 1) / ty);
+        const long cost = ((waves + simds - 1) / simds) * (long)(S * (ty + 2 * S) - S * (S + 1));
+        if (best_cost < 0 || cost < best_cost) best = ty, best_cost = cost;
+    }
+    return best;
+}
+
 template <typename real>
 hipError_t launch_rk_generic(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s) {
     switch (tab) {
         case RkTableau::RK4: return launch_tab<real, TabRK4>(a, s);
         case RkTableau::DP45_FIXED: return launch_tab<real, TabDP45Fixed>(a, s);
         case RkTableau::DP45_ADAPTIVE: {
-            /* TE-TM in FP32 runs 32-row tiles: the 7-stage halo is recomputed over 32 rows instead of 16
-               (one wave per SIMD either way, no scratch); 307 instead of 366 us per attempt at 2048^2.
-               PHX_GEN_TY=16 or 32 fixes the height (tuning runs); tile height does not change results. */
-            static const int env_ty = [] { const char* e = std::getenv("PHX_GEN_TY"); return e ? std::atoi(e) : 0; }();
-            const int ty = env_ty ? env_ty : (a.tetm && sizeof(real) == 4 ? 32 : 16);
-            if (ty == 32) return dispatch_flags<real, TabDP45Adaptive, 32>(a, s);
+            const int ty = dp45_adaptive_tile_rows(a.tetm, sizeof(real) == 4, a.Nc, a.Nr);
+            if constexpr (sizeof(real) == 4)
+                if (ty != kRkGenMinTileRows && a.tetm) return launch_dp45_tetm_tall(a, ty, s);
             return launch_tab<real, TabDP45Adaptive>(a, s);
         }
         case RkTableau::NEWTON:

@@ -688,7 +688,8 @@ int autotune_rk4(phx_solver* s) {
     PHX_HIP(hipEventCreate(&e1));
     const int cands[] = {8, 10, 12, 16, 20, 24};
     /* deeper row prefetch exists for the pumped reservoir model */
-    const int npf = (c.use_reservoir && c.use_pumps && !c.use_potentials) ? 3 : 1;
+    const int pfs[] = {2, 3, 4, 6, 8};
+    const int npf = (c.use_reservoir && c.use_pumps && !c.use_potentials) ? 5 : 1;
     /* three round-robin passes over the candidates, each candidate's fastest pass counts: a
        single pass in a fixed order would time the first candidates while the clocks ramp up */
     std::vector<float> cand_ms(6 * npf, 1e30f);
@@ -696,7 +697,7 @@ int autotune_rk4(phx_solver* s) {
         for (int cand = 0; cand < 6 * npf; cand++) {
             phx::Rk4Tune t = s->tune;
             t.ty = cands[cand % 6];
-            t.pf = 2 + cand / 6;
+            t.pf = pfs[cand / 6];
             const phx::Rk4Tune keep = s->tune;
             s->tune = t;
             int rc = PHX_OK;
@@ -718,7 +719,7 @@ int autotune_rk4(phx_solver* s) {
         if (cand_ms[cand] < cand_ms[best]) best = cand;
     phx::Rk4Tune best_t = s->tune;
     best_t.ty = cands[best % 6];
-    best_t.pf = 2 + best / 6;
+    best_t.pf = pfs[best / 6];
     PHX_HIP(hipEventDestroy(e0));
     PHX_HIP(hipEventDestroy(e1));
     s->tune = best_t;

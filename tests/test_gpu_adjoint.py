@@ -249,12 +249,15 @@ def test_adjoint_without_reservoir_gradient_of_potential():
     tpsi = torch.tensor(psi0, dtype=torch.complex128, device=dev)
     psiT = torch.empty_like(tpsi)
     eng = _lib.Adjoint("fp64", ph, N, N, 1, dt, steps, 7, (True, True), reservoir=False, pump=False, potential=True)
-    s = torch.cuda.current_stream().cuda_stream
-    eng.set_stream(s)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())  # the engine's stream after the tensors' initialisation
+    eng.set_stream(side.cuda_stream)
     eng.forward(tpsi.data_ptr(), None, None, tV.data_ptr(), psiT.data_ptr(), None)
-    g = torch.tensor(2 * mask, dtype=torch.float64, device=dev) * psiT
+    torch.cuda.current_stream().wait_stream(side)
+    g = (torch.tensor(2 * mask, dtype=torch.float64, device=dev) * psiT).contiguous()
     gV = torch.empty_like(tV)
-    eng.backward(g.contiguous().data_ptr(), None, None, gV.data_ptr(), None, None)
+    side.wait_stream(torch.cuda.current_stream())
+    eng.backward(g.data_ptr(), None, None, gV.data_ptr(), None, None)
     torch.cuda.synchronize()
     p = O.params(N, N, L, L, periodic=(True, True), reservoir=False, pump=False, potential=True)
     oT, _ = O.rk4_run(p, psi0[0], None, pot=V, dt=dt, steps=steps)
@@ -264,3 +267,37 @@ def test_adjoint_without_reservoir_gradient_of_potential():
     say(f"  rel L2 dV {e:.2e}")
     assert e <= 1e-10
     eng.close()
+
+
+def test_default_stream_ordering():
+    """rk4_evolve on torch's default stream (handle 0): the engine's work is ordered with torch's own
+    kernels on that stream (the asynchronous zero fill of n0 before, the objective's reduction after),
+    so the objective equals the one computed from the plain entry point's result."""
+    import torch
+    from phoenix_amd import _lib
+    from phoenix_amd.adjoint import release_engines, rk4_evolve
+    N, L, B, steps, dt = 256, 127.5, 8, 400, 0.04
+    say("[adjoint default stream] 8 x 256^2, 400 steps")
+    P, V, psi0, _, mask = problem(N, N, L, B, seed=3)
+    dev = torch.device("cuda:0")
+    assert torch.cuda.current_stream().cuda_stream == 0
+    ph, _ = _lib.physics_defaults(N, N, L, L)
+    tm = torch.tensor(mask, dtype=torch.float64, device=dev)
+    tP = torch.tensor(P, dtype=torch.float64, device=dev, requires_grad=True)
+    tV = torch.tensor(V, dtype=torch.float64, device=dev)
+    tpsi = torch.tensor(psi0, dtype=torch.complex128, device=dev)
+    tn = torch.zeros(psi0.shape, dtype=torch.float64, device=dev)  # asynchronous fill on the default stream
+    psiT, _ = rk4_evolve(tP, tV, tpsi, tn, physics=ph, dt=dt, steps=steps, checkpoint_every=50)
+    J = float((psiT.abs() ** 2 * tm).sum())  # reduction queued right behind the engine's last launch
+    release_engines()
+    ref = 0.0
+    for b in range(B):
+        x = torch.tensor(psi0[b], dtype=torch.complex128, device=dev)
+        n = torch.zeros((N, N), dtype=torch.float64, device=dev)
+        x2, n2 = torch.empty_like(x), torch.empty_like(n)
+        _lib.rk4_scalar_device(_lib.PHX_FP64, ph, N, N, (True, True), x.data_ptr(), n.data_ptr(), x2.data_ptr(),
+                               n2.data_ptr(), tP.detach().data_ptr(), tV.data_ptr(), dt, steps, None)
+        torch.cuda.synchronize()
+        ref += float((np.abs(x.cpu().numpy()) ** 2 * mask).sum())
+    say(f"  J {J:.15e} reference {ref:.15e}")
+    assert abs(J - ref) <= 1e-12 * abs(ref)

@@ -159,3 +159,33 @@ def test_spinor_adaptive_stochastic(prec):
     e = max(rel_l2(out["psi_plus"], r["psi"]), rel_l2(out["psi_minus"], r["psi_minus"]))
     say(f"  rel L2 psi (max of components) {e:.3e}")
     assert e <= (1e-11 if prec == "fp64" else 1e-4)
+
+
+def test_spinor_tile_heights_agree():
+    """The FP32 TE-TM adaptive kernel runs 16-, 32- or 42-row tiles (PHX_GEN_TY, read at every launch):
+    the same accepted and rejected counts, results equal to rounding (rel L2 <= 1e-6; the FP64 error sum
+    is reduced over a different grouping of per-wave partials, so not bitwise)."""
+    import os
+    N, L, dt0, tmax, seed = 150, 74.5, 0.04, 0.6, 9
+    psi, n, Pp, psim, nm, Pm = inputs(N, L, 41, tetm=True)
+    fields = {"psi_plus": psi, "n_plus": n, "pump_plus": Pp, "psi_minus": psim, "n_minus": nm, "pump_minus": Pm}
+    args = ["-tetm", "--iterator", "DP45", "-adaptive", "--tol", 1e-6, "--rkvdt", 1e-4, 0.1, "--tstep", dt0,
+            "--tmax", tmax, "--dw", 0.05, "--initRandom", 0.0, seed]
+    res = {}
+    old = os.environ.get("PHX_GEN_TY")
+    try:
+        for ty in (16, 32, 42):
+            os.environ["PHX_GEN_TY"] = str(ty)
+            say(f"[spinor tile height] TY={ty}")
+            out, done, info = gpu("fp32", N, L, args, fields, 100000)
+            res[ty] = (out, done, info["rejected"])
+    finally:
+        if old is None:
+            os.environ.pop("PHX_GEN_TY", None)
+        else:
+            os.environ["PHX_GEN_TY"] = old
+    for ty in (32, 42):
+        assert res[ty][1] == res[16][1] and res[ty][2] == res[16][2]
+        e = max(rel_l2(res[ty][0]["psi_plus"], res[16][0]["psi_plus"]), rel_l2(res[ty][0]["psi_minus"], res[16][0]["psi_minus"]))
+        say(f"  TY={ty}: {res[ty][1]} steps {res[ty][2]} rejected, rel L2 vs TY=16 {e:.2e}")
+        assert e <= 1e-6
