@@ -225,6 +225,24 @@ int phx_rk4_scalar_device(int precision, const phx_physics* ph, int N_c, int N_r
                           void* psi_a, void* n_a, void* psi_b, void* n_b, const void* pump, const void* pot,
                           double dt, int64_t n_steps, void* stream);
 
+/* Scratch held by phx_rk4_scalar_device_noalloc on each device (bytes). */
+#define PHX_NOALLOC_SCRATCH_BYTES (32u << 20)
+
+/* phx_rk4_scalar_device without the caller's second buffer and without any device allocation:
+   psi / n hold the state and receive the result; the double buffer is a fixed array of
+   PHX_NOALLOC_SCRATCH_BYTES in the library's device image (each device that runs it holds it
+   until the process ends).  Grids needing more than that (2 N_c N_r reals for psi, rounded up
+   to 256 bytes, plus N_c N_r for n) are rejected with PHX_ERR_INVALID: up to 2.79 M cells in
+   FP32 with a reservoir (e.g. 1024 x 1024, 1600 x 1600), 1.39 M in FP64 (1024 x 1024).  Calls no
+   hipMalloc / hipFree; an odd step count copies the result back with hipMemcpyAsync on `stream`.
+   All calls on one device share that device's array: calls that overlap in time on one device
+   (other streams, other threads) race.  Results are bitwise those of phx_rk4_scalar_device.
+   No counterpart in the reference: for callers that cannot allocate device memory (code under
+   hipGraph capture, tight serving loops, harnesses that forbid hipMalloc). */
+int phx_rk4_scalar_device_noalloc(int precision, const phx_physics* ph, int N_c, int N_r, int periodic_x, int periodic_y,
+                                  void* psi, void* n, const void* pump, const void* pot, double dt, int64_t n_steps,
+                                  void* stream);
+
 /* ---------------------------------------------------------------------------
  * Batched ensembles: many independent scalar grids of different sizes stepped
  * by fixed RK4 in one launch per step.  Each instance is the system that one

@@ -34,7 +34,7 @@ EXPORTED = [
     "phx_adjoint_create", "phx_adjoint_forward", "phx_adjoint_backward", "phx_adjoint_set_stream", "phx_adjoint_sync",
     "phx_adjoint_destroy",
     "phx_run_interval", "phx_source_groups", "phx_source_field", "phx_source_temporal", "phx_get_output_config",
-    "phx_step_at",
+    "phx_step_at", "phx_rk4_scalar_device_noalloc",
 ]
 PHX_ENS_RESERVOIR, PHX_ENS_PUMP, PHX_ENS_POTENTIAL = 1, 2, 4
 def stream_arg(stream_ptr: Optional[int]):
@@ -119,6 +119,8 @@ def lib():
         L.phx_physics_defaults.restype = C.c_double
         L.phx_rk4_scalar_device.argtypes = [C.c_int, C.POINTER(PhxPhysics), C.c_int, C.c_int, C.c_int, C.c_int,
                                             vp, vp, vp, vp, vp, vp, C.c_double, i64, vp]
+        L.phx_rk4_scalar_device_noalloc.argtypes = [C.c_int, C.POINTER(PhxPhysics), C.c_int, C.c_int, C.c_int, C.c_int,
+                                                    vp, vp, vp, vp, C.c_double, i64, vp]
         ip, dp = C.POINTER(C.c_int), C.POINTER(C.c_double)
         L.phx_ensemble_create.argtypes = [C.POINTER(vp), C.c_int, C.POINTER(PhxPhysics), C.c_int, ip, ip, dp, dp,
                                           C.c_int, C.c_int, C.c_int]
@@ -266,6 +268,14 @@ def rk4_scalar_device(precision: int, ph: PhxPhysics, N_c: int, N_r: int, period
                       psi_b: int, n_b: int, pump: int, pot: int, dt: float, n_steps: int, stream: int = 0):
     check(lib().phx_rk4_scalar_device(precision, C.byref(ph), N_c, N_r, int(periodic[0]), int(periodic[1]), psi_a,
                                       n_a, psi_b, n_b, pump, pot, dt, n_steps, stream))
+
+
+def rk4_scalar_device_noalloc(precision: int, ph: PhxPhysics, N_c: int, N_r: int, periodic: tuple, psi: int, n: int,
+                              pump: int, pot: int, dt: float, n_steps: int, stream: Optional[int]):
+    """phx_rk4_scalar_device_noalloc: `n_steps` RK4 steps in place on device pointers, with the
+    library's fixed device scratch as the second buffer (no device allocation)."""
+    check(lib().phx_rk4_scalar_device_noalloc(precision, C.byref(ph), N_c, N_r, int(periodic[0]), int(periodic[1]), psi, n,
+                                              pump, pot, dt, n_steps, stream))
 
 
 class Ensemble:

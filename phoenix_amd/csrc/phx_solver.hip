@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
@@ -1297,6 +1298,52 @@ static int rk4_device_t(const phx_physics* ph, int N_c, int N_r, int px, int py,
         if (n_a) PHX_HIP(hipMemcpyAsync(n_a, n_b, sizeof(real) * (size_t)N_c * N_r, hipMemcpyDeviceToDevice, st));
     }
     return PHX_OK;
+}
+
+/* Double buffer of phx_rk4_scalar_device_noalloc: one array per device, loaded with the code
+   object and kept until the process ends.  Host code never names it directly (on the host the
+   symbol is only a stand-in): it uses the device address that hipGetSymbolAddress returns,
+   looked up once per device. */
+constexpr size_t kScratchBytes = (size_t)PHX_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static int scratch_address(void** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    int dev = 0;
+    PHX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return fail(PHX_ERR_UNSUPPORTED, "device index " + std::to_string(dev));
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) return fail(PHX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        addr[dev] = p;
+    }
+    *out = addr[dev];
+    return PHX_OK;
+}
+
+extern "C" int phx_rk4_scalar_device_noalloc(int precision, const phx_physics* ph, int N_c, int N_r, int periodic_x,
+                                             int periodic_y, void* psi, void* n, const void* pump, const void* pot, double dt,
+                                             int64_t n_steps, void* stream) {
+    if (!ph || !psi || N_c <= 0 || N_r <= 0 || n_steps < 0) return fail(PHX_ERR_INVALID, "invalid argument");
+    if (pump && !n) return fail(PHX_ERR_INVALID, "a pump needs a reservoir");
+    if (precision != PHX_FP32 && precision != PHX_FP64) return fail(PHX_ERR_INVALID, "precision");
+    const size_t rsize = precision == PHX_FP32 ? 4 : 8, cells = (size_t)N_c * (size_t)N_r;
+    const size_t psi_bytes = (2 * cells * rsize + 255) & ~(size_t)255;
+    const size_t need = psi_bytes + (n ? cells * rsize : 0);
+    if (need > kScratchBytes)
+        return fail(PHX_ERR_INVALID, "grid of " + std::to_string(cells) + " cells needs " + std::to_string(need) +
+                                         " bytes of scratch; the allocation-free entry point holds " + std::to_string(kScratchBytes));
+    void* base = nullptr;
+    if (int r = scratch_address(&base)) return r;
+    void* psi_b = base;
+    void* n_b = n ? (char*)base + psi_bytes : nullptr;
+    hipStream_t st = (hipStream_t)stream;
+    if (precision == PHX_FP32) return rk4_device_t<float>(ph, N_c, N_r, periodic_x, periodic_y, psi, n, psi_b, n_b, pump, pot, dt, n_steps, st);
+    return rk4_device_t<double>(ph, N_c, N_r, periodic_x, periodic_y, psi, n, psi_b, n_b, pump, pot, dt, n_steps, st);
 }
 
 extern "C" int phx_rk4_scalar_device(int precision, const phx_physics* ph, int N_c, int N_r, int periodic_x, int periodic_y, void* psi_a,
