@@ -27,6 +27,7 @@ sys.path.insert(0, ROOT)
 # Must be set before the HIP runtime initialises (before torch is imported).
 os.environ.setdefault("HIP_FORCE_DEV_KERNARG", "1")
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "rk4_scalar_fused_1024_pmc.md")
+FLOW_PMC_SUMMARY = os.path.join(ROOT, "profiles", "rk4_scalar_flow_1024_pmc.md")
 
 METRIC = "µs/RK4 step at 1024² fp32/fp64 + % HBM roofline; batched & adjoint vs plain"
 PTH = 0.15 * 0.225 / 0.01
@@ -67,6 +68,7 @@ def time_rk4(prec: str, N: int, L: float, steps: int, warmup: int, kernel_sample
     s.sync()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    i0 = s.info
     t0 = time.perf_counter()
     e0.record(stream)
     done = s.step(steps)
@@ -75,6 +77,9 @@ def time_rk4(prec: str, N: int, L: float, steps: int, warmup: int, kernel_sample
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     total_ms = e0.elapsed_time(e1)
+    i1 = s.info
+    multi_steps = i1["multi_step_steps"] - i0["multi_step_steps"]
+    multi_launches = i1["multi_step_launches"] - i0["multi_step_launches"]
     assert done == steps
     # per-launch duration of the fused kernel: one event pair around each single-kernel step
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(kernel_samples)]
@@ -91,18 +96,23 @@ def time_rk4(prec: str, N: int, L: float, steps: int, warmup: int, kernel_sample
     s.close()
     return dict(total_ms=total_ms, wall_s=wall, us_per_step=1e3 * total_ms / steps,
                 kernel_us_mean=1e3 * float(np.mean(k_ms)), kernel_us_median=1e3 * float(np.median(k_ms)),
-                finite=finite, tile_rows=info["tile_rows"], prefetch_rows=info["prefetch_rows"])
+                finite=finite, tile_rows=info["tile_rows"], prefetch_rows=info["prefetch_rows"],
+                multi_step_steps=multi_steps, multi_step_launches=multi_launches)
 
 
-def measured_traffic(prec: str, ty: int, pf: int):
-    """HBM bytes per launch of the fused kernel for this tile shape, from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary (lines 'traffic <prec> ty=<TY> pf=<PF>: ... bytes=<B>'), or None."""
+def measured_traffic(prec: str, ty: int, pf: int, kernel: str = "fused"):
+    """HBM bytes per launch of the RK4 kernel for this tile shape, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summaries (lines 'traffic <prec> ty=<TY> pf=<PF>: ... bytes=<B>' for the
+    one-step kernel, 'traffic flow <prec> ty=<TY> pf=<PF>: ... bytes=<B>' per step of the multi-step
+    kernel, which the caller scales to one launch), or None."""
     import re
+    path = PMC_SUMMARY if kernel == "fused" else FLOW_PMC_SUMMARY
     try:
-        txt = open(PMC_SUMMARY).read()
+        txt = open(path).read()
     except OSError:
         return None
-    m = re.search(rf"^traffic {prec} ty={ty} pf={pf}:.*bytes=(\d+)", txt, re.M)
+    tag = "traffic" if kernel == "fused" else "traffic flow"
+    m = re.search(rf"^{tag} {prec} ty={ty} pf={pf}:.*bytes=(\d+)", txt, re.M)
     return int(m.group(1)) if m else None
 
 
@@ -439,9 +449,15 @@ def main():
     main_prec = "fp32" if "fp32" in res else "fp64"
     r = res[main_prec]
     bytes_per_cell = {"fp32": 28, "fp64": 56}
-    # The timed region launches exactly one kernel per step, back to back on one stream, so the
+    # The timed region launches one multi-step kernel per up to 500 steps (or, with PHX_RK4_FLOW=0,
+    # one kernel per step), back to back on one stream, so the
     # average launch duration over the region is total / steps (rocprof's per-kernel average agrees).
     kernel_s = r["us_per_step"] * 1e-6
+    flow = r["multi_step_steps"] == a.steps
+    per_launch = a.steps / max(r["multi_step_launches"], 1) if flow else 1
+    traffic = measured_traffic(main_prec, r["tile_rows"], r["prefetch_rows"], "flow" if flow else "fused")
+    if traffic is not None:
+        traffic = round(traffic * per_launch)
     achieved = bytes_per_cell[main_prec] * cells / kernel_s / 1e9
     out = {
         "metric": METRIC,
@@ -461,11 +477,12 @@ def main():
         "gcell_steps_per_s": round(cells / (r["us_per_step"] * 1e-6) / 1e9, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": measured_traffic(main_prec, r["tile_rows"], r["prefetch_rows"]),
-                     "kernel": f"rk4_scalar_fused<{'float' if main_prec == 'fp32' else 'double'},TY={r['tile_rows']},"
-                               f"PF={r['prefetch_rows']}>",
-                     "kernel_us": round(r["us_per_step"], 3),
-                     "algorithmic_bytes": bytes_per_cell[main_prec] * cells,
+                     "traffic": traffic,
+                     "kernel": f"{'rk4_scalar_flow' if flow else 'rk4_scalar_fused'}<{'float' if main_prec == 'fp32' else 'double'},"
+                               f"TY={r['tile_rows']},PF={r['prefetch_rows']}>",
+                     "steps_per_launch": per_launch,
+                     "kernel_us": round(r["us_per_step"] * per_launch, 3),
+                     "algorithmic_bytes": round(bytes_per_cell[main_prec] * cells * per_launch),
                      "bytes_per_cell": bytes_per_cell[main_prec]},
         "precisions": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                        for k, v in res.items()},

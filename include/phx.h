@@ -81,6 +81,10 @@ typedef struct {
     int32_t prefetch_rows;      /* rows it loads ahead of use */
     int32_t staged;             /* 1: the stage-by-stage path runs the step (K's in HBM: pulses, several
                                    non-constant temporal groups, complex potential envelopes, DP85, NSRK78) */
+    int64_t multi_step_steps;   /* fixed RK4 steps taken by the multi-step kernel (phx_step and
+                                   phx_run_interval run constant-source stretches that way, up to 500 steps
+                                   per launch; PHX_RK4_FLOW=0: never) */
+    int64_t multi_step_launches; /* its launches */
 } phx_info;
 
 /* Parse PHOENIX flags (argv[0] is the program name and is skipped), including
@@ -97,7 +101,12 @@ int phx_create(phx_solver** out, int precision, int argc, const char* const* arg
 int phx_initialize(phx_solver* s);
 
 /* Run up to n_steps iterate() calls; stops early when t >= t_max.
-   *n_done receives the number of accepted steps taken.
+   *n_done receives the number of accepted steps taken.  Stretches of fixed RK4 steps of the
+   scalar model with constant sources (no filter application or t_max inside) go to the device
+   as one launch of up to 500 steps in which every tile waits for its neighbours (bitwise the
+   same result as step by step); should
+   such a wait ever exceed ~2 s, the next phx_sync / phx_get_field returns PHX_ERR_HIP and the
+   state is undefined.  PHX_RK4_FLOW=0 launches step by step instead.
    Replaces: Solver::iterate (solver.cu:22-70) in the main loop (main.cu:80-117). */
 int phx_step(phx_solver* s, int64_t n_steps, int64_t* n_done);
 

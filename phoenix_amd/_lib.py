@@ -65,7 +65,7 @@ class PhxInfo(C.Structure):
         ("dx", C.c_double), ("dy", C.c_double), ("dt", C.c_double), ("t", C.c_double), ("t_max", C.c_double),
         ("fft_every", C.c_double), ("tolerance", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
         ("iteration", C.c_int64), ("rejected", C.c_int64), ("tile_rows", C.c_int32), ("prefetch_rows", C.c_int32),
-        ("staged", C.c_int32),
+        ("staged", C.c_int32), ("multi_step_steps", C.c_int64), ("multi_step_launches", C.c_int64),
     ]
 
     def as_dict(self):

@@ -40,7 +40,7 @@ HIP_FLAGS_FILE = {f: _UNROLL for f in ("phx_rk_generic.hip", "phx_rk_tab3.hip", 
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include")]
 
 # longest compiles first: the thread pool then finishes them side by side
-HIP_SOURCES = ["phx_rk_generic.hip", "phx_rk_dp45t.hip", "phx_rk4_f64.hip", "phx_rk4.hip", "phx_rk_tab6.hip", "phx_rk_tab6f.hip", "phx_rk_tab4.hip",
+HIP_SOURCES = ["phx_rk_generic.hip", "phx_rk_dp45t.hip", "phx_rk4_f64.hip", "phx_rk4.hip", "phx_rk4_flow.hip", "phx_rk4_flow_f64.hip", "phx_rk_tab6.hip", "phx_rk_tab6f.hip", "phx_rk_tab4.hip",
                "phx_rk_tab3.hip", "phx_fft.hip", "phx_solver.hip", "phx_ensemble.hip", "phx_adjoint.hip", "phx_ssfm.hip",
                "phx_staged.hip"]
 CXX_SOURCES = ["phx_config.cpp"]

@@ -119,37 +119,38 @@ __device__ __forceinline__ double2 load_cplx<double>(rsrc_t r, uint32_t off) {
 
 /* Loads and stores with a scalar byte offset (soffset) on top of the per-lane
    offset: one descriptor per array, the row offset in an SGPR.  Lanes whose
-   per-lane offset is out of range read 0 and do not write. */
-template <typename real>
-__device__ __forceinline__ real load_real_s(rsrc_t r, uint32_t off, uint32_t soff);
-template <>
-__device__ __forceinline__ float load_real_s<float>(rsrc_t r, uint32_t off, uint32_t soff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, 0));
+   per-lane offset is out of range read 0 and do not write.  CP is the cache policy:
+   0 for ordinary cached accesses, kDeviceScope for data that other workgroups of the
+   same launch write or read while it runs (no stale line in this CU's or this XCD's
+   cache is used, and a completed store is visible to every other CU). */
+constexpr int kDeviceScope = 16; /* sc1: device-scope coherence */
+template <typename real, int CP = 0>
+__device__ __forceinline__ real load_real_s(rsrc_t r, uint32_t off, uint32_t soff) {
+    if constexpr (sizeof(real) == 4)
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, soff, CP));
+    else
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, CP));
 }
-template <>
-__device__ __forceinline__ double load_real_s<double>(rsrc_t r, uint32_t off, uint32_t soff) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
-}
-template <typename real>
-__device__ __forceinline__ typename Cplx<real>::type load_cplx_s(rsrc_t r, uint32_t off, uint32_t soff);
-template <>
-__device__ __forceinline__ float2 load_cplx_s<float>(rsrc_t r, uint32_t off, uint32_t soff) {
-    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
-}
-template <>
-__device__ __forceinline__ double2 load_cplx_s<double>(rsrc_t r, uint32_t off, uint32_t soff) {
-    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, 0));
+template <typename real, int CP = 0>
+__device__ __forceinline__ typename Cplx<real>::type load_cplx_s(rsrc_t r, uint32_t off, uint32_t soff) {
+    if constexpr (sizeof(real) == 4)
+        return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, CP));
+    else
+        return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, off, soff, CP));
 }
 typedef int phx_v2i __attribute__((ext_vector_type(2)));
 typedef int phx_v4i __attribute__((ext_vector_type(4)));
+template <int CP = 0>
 __device__ __forceinline__ void store_real_s(rsrc_t r, uint32_t off, uint32_t soff, float v) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, off, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, off, soff, CP);
 }
+template <int CP = 0>
 __device__ __forceinline__ void store_real_s(rsrc_t r, uint32_t off, uint32_t soff, double v) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(phx_v2i, v), r, off, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(phx_v2i, v), r, off, soff, CP);
 }
+template <int CP = 0>
 __device__ __forceinline__ void store_cplx_s(rsrc_t r, uint32_t off, uint32_t soff, float2 v) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(phx_v2i, v), r, off, soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(phx_v2i, v), r, off, soff, CP);
 }
 /* A buffer store of more than 8 bytes reads its data registers after it issues.  The
    compiler separates it from a following VALU write of those registers only when its
@@ -158,8 +159,9 @@ __device__ __forceinline__ void store_cplx_s(rsrc_t r, uint32_t off, uint32_t so
    FMA's result, intermittently).  So the row offset joins the per-lane offset and soffset
    stays 0.  Out-of-range lanes stay out of range: kOutOfRange + soff < 2^32 for every
    soff < kMaxArrayBytes, and a descriptor of size 0 drops any offset. */
+template <int CP = 0>
 __device__ __forceinline__ void store_cplx_s(rsrc_t r, uint32_t off, uint32_t soff, double2 v) {
-    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(phx_v4i, v), r, off + soff, 0, 0);
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(phx_v4i, v), r, off + soff, 0, CP);
 }
 
 /* Largest array the whole-array descriptors address: offsets are 32-bit and the

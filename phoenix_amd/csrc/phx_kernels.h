@@ -51,6 +51,22 @@ struct Rk4Tune {
 template <typename real>
 hipError_t launch_rk4_scalar(const Rk4Args<real>& a, const Rk4Tune& t, hipStream_t s);
 
+/* Many RK4 steps in one launch (rk4_scalar_flow): one step counter per wavefront tile in
+   device memory (kFlowCounterStride ints apart), all equal to `base` when the launch starts
+   and base + steps after it;
+   *error is set (and the state undefined) if a tile waited more than spin_limit polls for
+   a neighbour.  Tiles: nstrips x nbands = ceil(Nc / 56) x ceil(Nr / ty). */
+constexpr int kFlowCounterStride = 32; /* ints: one 128-byte line per tile counter */
+struct Rk4Flow {
+    int* counters; /* tile t's counter at counters[t * kFlowCounterStride] */
+    int* error;
+    int base;
+    int steps;
+    int spin_limit;
+};
+template <typename real>
+hipError_t launch_rk4_flow(const Rk4Args<real>& a, const Rk4Tune& t, const Rk4Flow& f, hipStream_t s);
+
 /* One instance of an ensemble: its buffers, its size, and its first wavefront tile
    in the flattened tile list.  The kinetic weights carry the instance's dx, dy. */
 template <typename real>
@@ -161,11 +177,11 @@ template <typename real>
 hipError_t launch_rk_tab6(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
 template <typename real>
 hipError_t launch_rk_tab6f(const RkGenArgs<real>& a, RkTableau tab, hipStream_t s);
-/* adaptive DP45, TE-TM, FP32, tile height 32 or 42 (phx_rk_dp45t.hip) */
+/* adaptive DP45, TE-TM, FP32, tile height 32 (phx_rk_dp45t.hip) */
 hipError_t launch_dp45_tetm_tall(const RkGenArgs<float>& a, int ty, hipStream_t s);
 /* tile height of the adaptive DP45 kernel for a grid: the height among the compiled ones that
    minimises (wavefront tiles per SIMD, rounded up) x (row evaluations per tile), a VALU-bound
-   model; PHX_GEN_TY fixes it (16, 32 or 42) */
+   model; PHX_GEN_TY fixes it (16 or 32) */
 int dp45_adaptive_tile_rows(bool tetm, bool fp32, int Nc, int Nr);
 
 /* Staged Runge-Kutta step (phx_staged.hip): one launch per CALCULATE_K, INTERMEDIATE_SUM_K

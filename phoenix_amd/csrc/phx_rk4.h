@@ -46,7 +46,7 @@ constexpr int kOutW = kWave - 2 * kHalo; /* 56 */
    both east/west moves folded into DPP adds.  ISO (dx == dy, r == 1 exactly) drops
    the multiply by r; the two forms give identical bits for dx == dy, so a grid gets
    the same result from either kernel (an ensemble mixing both uses the general one). */
-template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool PERY>
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool PERY, int CP = 0, int CPS = CP>
 __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int band) {
     using v2 = typename Vec2<real>::type;
     const int lane = threadIdx.x & (kWave - 1);
@@ -100,10 +100,10 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
         const uint32_t grow = tile_row(y0 - kHalo + r, wrap, PERY, in); /* wave-uniform */
         const uint32_t sc = grow * row_c, sr = grow * row_r;
         const uint32_t nc = in ? all_c : 0u, nr = in ? all_r : 0u;
-        const auto x = load_cplx_s<real>(row_rsrc(a.psi_in, nc), off_c, sc);
+        const auto x = load_cplx_s<real, CP>(row_rsrc(a.psi_in, nc), off_c, sc);
         p0r[r] = x.x;
         p0i[r] = x.y;
-        if (RES) n0[r] = load_real_s<real>(row_rsrc(a.n_in, nr), off_r, sr);
+        if (RES) n0[r] = load_real_s<real, CP>(row_rsrc(a.n_in, nr), off_r, sr);
         if (PUMP) pp[r] = load_real_s<real>(row_rsrc(a.pump, nr), off_r, sr);
         if (POT) vv[r] = load_real_s<real>(row_rsrc(a.pot, nr), off_r, sr);
     };
@@ -189,8 +189,8 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
                 v2 o;
                 o.x = phx_fma(dt6, kr, acr[e]);
                 o.y = phx_fma(dt6, ki, aci[e]);
-                store_cplx_s(row_ok ? w_psi : w_psi_none, so_c, (uint32_t)grow * row_c, o);
-                if (RES) store_real_s(row_ok ? w_n : w_n_none, so_r, (uint32_t)grow * row_r, phx_fma(dt6, kn, acn[e]));
+                store_cplx_s<CPS>(row_ok ? w_psi : w_psi_none, so_c, (uint32_t)grow * row_c, o);
+                if (RES) store_real_s<CPS>(row_ok ? w_n : w_n_none, so_r, (uint32_t)grow * row_r, phx_fma(dt6, kn, acn[e]));
             }
         }
     }
@@ -205,6 +205,78 @@ __global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
     const int band = (tile / a.nstrips) * kWavesPerBlock + wave;
     if (band >= a.nbands) return;
     rk4_tile<real, TY, ISO, PF, RES, PUMP, POT, PERY>(a, strip, band);
+}
+
+/* Many steps in one launch, one resident wavefront per tile.  Before step k a tile waits
+   until the eight tiles whose cells its 4-cell halo covers have finished step k-1 (their
+   step counters reach base + k): their outputs of step k-1 are its inputs, and their reads
+   of step k-1, which covered its cells in the buffer it is about to overwrite, are done.
+   psi and n go through device-scope loads and stores (no stale line of another XCD's
+   write is used); a counter moves only after all stores of the step have completed.  Each
+   counter has a 128-byte line of its own: packed 32 to a line, the polls of neighbouring
+   tiles contended for the same lines and a step took 21 instead of 8 us at 1024^2.  A
+   wait longer than spin_limit polls sets *error and ends the wave (its neighbours then
+   time out too, so every wave exits); the state is then undefined.  Tiles of the last
+   band and strip must be at least kHalo cells deep/wide, so that a halo never reaches
+   past the neighbouring tile (rk4_flow_shape_ok). */
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool PERY>
+__global__ __launch_bounds__(256) void rk4_scalar_flow(Rk4Args<real> a, Rk4Flow f) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int tile = xcd_contiguous(blockIdx.x, gridDim.x);
+    const int strip = tile % a.nstrips;
+    const int band = (tile / a.nstrips) * kWavesPerBlock + wave;
+    if (band >= a.nbands) return; /* no tile: nobody watches this wave's counter */
+    const int me = band * a.nstrips + strip;
+    /* lanes 0..7 watch the neighbours (d = 0..8 without the centre 4) */
+    int watch = -1;
+    if (lane < 8) {
+        const int d = lane + (lane >= 4 ? 1 : 0);
+        int ns = strip + d % 3 - 1, nb = band + d / 3 - 1;
+        bool ok = true;
+        if (ns < 0 || ns >= a.nstrips) {
+            if (a.periodic_x)
+                ns = (ns + a.nstrips) % a.nstrips;
+            else
+                ok = false;
+        }
+        if (nb < 0 || nb >= a.nbands) {
+            if (PERY)
+                nb = (nb + a.nbands) % a.nbands;
+            else
+                ok = false;
+        }
+        if (ok) watch = nb * a.nstrips + ns;
+    }
+    int* const mine = f.counters + (size_t)me * kFlowCounterStride;
+    int* const theirs = watch < 0 ? nullptr : f.counters + (size_t)watch * kFlowCounterStride;
+    Rk4Args<real> s = a;
+    for (int k = 0; k < f.steps; k++) {
+        if (k > 0) {
+            const int need = f.base + k;
+            int polls = 0;
+            bool ready = theirs == nullptr;
+            for (;;) {
+                if (!ready) ready = __hip_atomic_load(theirs, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= need;
+                if (__builtin_amdgcn_ballot_w64(!ready) == 0) break;
+                if (++polls > f.spin_limit) {
+                    if (lane == 0) __hip_atomic_store(f.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    return;
+                }
+                __builtin_amdgcn_s_sleep(4);
+            }
+            asm volatile("" ::: "memory");
+        }
+        const bool even = (k & 1) == 0;
+        s.psi_in = even ? a.psi_in : a.psi_out;
+        s.psi_out = even ? (typename Vec2<real>::type*)a.psi_out : (typename Vec2<real>::type*)a.psi_in;
+        s.n_in = even ? a.n_in : a.n_out;
+        s.n_out = even ? a.n_out : (real*)a.n_in;
+        rk4_tile<real, TY, ISO, PF, RES, PUMP, POT, PERY, kDeviceScope>(s, strip, band);
+        /* all of this step's stores have completed before the counter moves */
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (lane == 0) __hip_atomic_store(mine, f.base + k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 /* ensemble: one flattened list of wavefront tiles over all instances; each wave looks up
@@ -308,6 +380,71 @@ hipError_t launch_rk4_scalar(const Rk4Args<real>& a, const Rk4Tune& t, hipStream
     return pot ? launch_flags<real, true, false, true>(a, t, s) : launch_flags<real, true, false, false>(a, t, s);
 }
 
+/* ---------------------------------------------------------------- many steps per launch */
+inline bool rk4_flow_shape_ok(int Nc, int Nr, int ty) {
+    const int last_rows = Nr - ((Nr - 1) / ty) * ty, last_cols = Nc - ((Nc - 1) / kOutW) * kOutW;
+    return Nr >= ty && Nc >= kOutW && last_rows >= kHalo && last_cols >= kHalo;
+}
+
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT>
+inline hipError_t launch_flow_ty(const Rk4Args<real>& a0, const Rk4Flow& f, hipStream_t s) {
+    Rk4Args<real> a = a0;
+    a.nstrips = (a.Nc + kOutW - 1) / kOutW;
+    a.nbands = (a.Nr + TY - 1) / TY;
+    const int nblocks = a.nstrips * ((a.nbands + kWavesPerBlock - 1) / kWavesPerBlock);
+    const void* fn = a.periodic_y ? (const void*)&rk4_scalar_flow<real, TY, ISO, PF, RES, PUMP, POT, true>
+                                  : (const void*)&rk4_scalar_flow<real, TY, ISO, PF, RES, PUMP, POT, false>;
+    /* every wave must be resident at once: the cooperative launch refuses a grid that is not */
+    int per_cu = 0, dev = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWave * kWavesPerBlock, 0) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return hipErrorNotSupported;
+    if ((long)per_cu * cus < nblocks) return hipErrorCooperativeLaunchTooLarge;
+    if (!f.counters) return hipSuccess; /* dry run: the kernel takes this grid */
+    Rk4Flow fl = f;
+    void* args[] = {&a, &fl};
+    return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(kWave * kWavesPerBlock), args, 0, s);
+}
+
+template <typename real, bool ISO, int PF, bool RES, bool PUMP, bool POT>
+inline hipError_t launch_flow_tys(const Rk4Args<real>& a, const Rk4Tune& t, const Rk4Flow& f, hipStream_t s) {
+    switch (t.ty) {
+        case 8: return launch_flow_ty<real, 8, ISO, PF, RES, PUMP, POT>(a, f, s);
+        case 10: return launch_flow_ty<real, 10, ISO, PF, RES, PUMP, POT>(a, f, s);
+        case 12: return launch_flow_ty<real, 12, ISO, PF, RES, PUMP, POT>(a, f, s);
+        case 20: return launch_flow_ty<real, 20, ISO, PF, RES, PUMP, POT>(a, f, s);
+        case 16: return launch_flow_ty<real, 16, ISO, PF, RES, PUMP, POT>(a, f, s);
+        default: return hipErrorNotSupported;
+    }
+}
+
+template <typename real, bool RES, bool PUMP, bool POT>
+inline hipError_t launch_flow_flags(const Rk4Args<real>& a, const Rk4Tune& t, const Rk4Flow& f, hipStream_t s) {
+    const bool iso = a.ph.kin_yx == real(1);
+    if constexpr (RES && PUMP && !POT) {
+        if (iso && t.pf == 3) return launch_flow_tys<real, true, 3, RES, PUMP, POT>(a, t, f, s);
+        if (iso && (t.pf == 4 || t.pf == 5)) return launch_flow_tys<real, true, 4, RES, PUMP, POT>(a, t, f, s);
+        if (iso && (t.pf == 6 || t.pf == 7)) return launch_flow_tys<real, true, 6, RES, PUMP, POT>(a, t, f, s);
+        if (iso && t.pf >= 8) return launch_flow_tys<real, true, 8, RES, PUMP, POT>(a, t, f, s);
+    }
+    return iso ? launch_flow_tys<real, true, 2, RES, PUMP, POT>(a, t, f, s) : launch_flow_tys<real, false, 2, RES, PUMP, POT>(a, t, f, s);
+}
+
+/* f.steps RK4 steps from a.psi_in / a.n_in; an even count leaves the result there, an odd
+   one in a.psi_out / a.n_out.  Errors: hipErrorNotSupported for a shape or tile height the
+   kernel does not take, hipErrorCooperativeLaunchTooLarge when its waves cannot all be
+   resident; the caller then steps launch by launch.  f.counters == nullptr: only check. */
+template <typename real>
+hipError_t launch_rk4_flow(const Rk4Args<real>& a, const Rk4Tune& t, const Rk4Flow& f, hipStream_t s) {
+    if ((size_t)a.Nc * (size_t)a.Nr * sizeof(typename Vec2<real>::type) > kMaxArrayBytes) return hipErrorInvalidValue;
+    if (!rk4_flow_shape_ok(a.Nc, a.Nr, t.ty) || f.steps <= 0) return hipErrorNotSupported;
+    const bool res = a.n_in != nullptr;
+    const bool pump = res && a.pump != nullptr;
+    const bool pot = a.pot != nullptr;
+    if (!res) return pot ? launch_flow_flags<real, false, false, true>(a, t, f, s) : launch_flow_flags<real, false, false, false>(a, t, f, s);
+    if (pump) return pot ? launch_flow_flags<real, true, true, true>(a, t, f, s) : launch_flow_flags<real, true, true, false>(a, t, f, s);
+    return pot ? launch_flow_flags<real, true, false, true>(a, t, f, s) : launch_flow_flags<real, true, false, false>(a, t, f, s);
+}
 
 /* ---------------------------------------------------------------- ensembles */
 template <typename real, int TY, bool ISO, bool RES, bool PUMP, bool POT>

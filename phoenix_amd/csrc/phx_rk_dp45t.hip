@@ -1,7 +1,7 @@
 /*
  * phx_rk_dp45t.hip -- the TE-TM instantiations of the adaptive Dormand-Prince 5(4) kernel in FP32
- * with tall tiles (32 and 42 output rows), in a translation unit of their own so that they compile
- * next to the others.  The tile height is chosen by launch_rk_generic (phx_rk_generic.hip).
+ * with 32-row tiles, in a translation unit of their own so that they compile next to the others.  The
+ * tile height is chosen by launch_rk_generic (phx_rk_generic.hip).
  */
 #include "phx_rk_generic.h"
 
@@ -19,7 +19,6 @@ static hipError_t launch_tetm(const RkGenArgs<float>& a, hipStream_t s) {
 hipError_t launch_dp45_tetm_tall(const RkGenArgs<float>& a, int ty, hipStream_t s) {
     if (!a.tetm) return hipErrorInvalidValue;
     if (ty == 32) return launch_tetm<32>(a, s);
-    if (ty == 42) return launch_tetm<42>(a, s);
     return hipErrorInvalidValue;
 }
 

@@ -142,19 +142,22 @@ int dp45_adaptive_tile_rows(bool tetm, bool fp32, int Nc, int Nr) {
     if (!(tetm && fp32)) return kRkGenMinTileRows; /* the tall tiles exist for the FP32 spinor kernel */
     const char* e = std::getenv("PHX_GEN_TY"); /* read at every launch: tests switch it within a process */
     const int env_ty = e ? std::atoi(e) : 0;
-    if (env_ty == 16 || env_ty == 32 || env_ty == 42) return env_ty;
+    if (env_ty == 16 || env_ty == 32) return env_ty;
     static const int simds = [] {
         int dev = 0, cus = 0;
         if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
             cus = 256;
         return 4 * std::max(cus, 1);
     }();
-    /* the kernel is bound by VALU issue (two waves per SIMD at its register count): a SIMD's time is
-       its number of tiles times the row evaluations of one tile, S (TY + 2 S) - S (S + 1) for S = 7 */
+    /* One wave per SIMD at the kernel's register count, and about the same VALU work per row
+       evaluation at both heights: a SIMD's time is its number of tiles times the row evaluations of
+       one tile, S (TY + 2 S) - S (S + 1) for S = 7.  At 2048^2 this predicts 32 rows 1.16x faster than
+       16 (1.21x measured).  A 42-row variant, predicted faster still, measured 1.24x slower than 32
+       rows and is not built. */
     constexpr int S = 7;
     int best = kRkGenMinTileRows;
     long best_cost = -1;
-    for (int ty : {16, 32, 42}) {
+    for (int ty : {16, 32}) {
         const long waves = (long)((Nc + 63 - 2 * S) / (64 - 2 * S)) * ((Nr + ty - 1) / ty);
         const long cost = ((waves + simds - 1) / simds) * (long)(S * (ty + 2 * S) - S * (S + 1));
         if (best_cost < 0 || cost < best_cost) best = ty, best_cost = cost;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cfloat>
 #include <cmath>
 #include <complex>
@@ -95,6 +96,13 @@ struct phx_solver {
     double graph_dt = 0;
     hipStream_t graph_stream = nullptr;
     bool graph_off = false;
+    /* the same runs as one launch of rk4_scalar_flow: per-tile step counters (all equal to
+       flow_base between launches) and the kernel's error word */
+    int* flow_counters = nullptr;
+    int* flow_error = nullptr;
+    int flow_base = 0;
+    int64_t flow_steps = 0, flow_launches = 0;
+    bool flow_off = false;
     /* staged path (phx_staged.hip): padded stage input and K slots, per-group source fields */
     bool staged = false;
     bool grouped = false; /* one source field per temporal group (staged path, SSFM with chains or pulses) */
@@ -734,6 +742,10 @@ int autotune_rk4(phx_solver* s) {
    iterate() has advanced to, which adaptTimeStep's updateKernelTime uploads, solver.cu:97),
    and s->dt leaves as adaptTimeStep sets it.  No t, iteration or FFT-guard bookkeeping. */
 template <typename real>
+int autotune_flow(phx_solver* s);
+int flow_check(phx_solver* s);
+
+template <typename real>
 int step_body_t(phx_solver* s, double t_pre, double t_adv, bool force_fixed) {
     auto& c = s->cfg;
     if (c.use_stochastic) {
@@ -749,6 +761,7 @@ int step_body_t(phx_solver* s, double t_pre, double t_adv, bool force_fixed) {
     } else if (c.iterator == "RK4" && !c.tetm && !c.use_stochastic && !imag && !s->staged) {
         if (!s->tuned) {
             if (int r = autotune_rk4<real>(s)) return r;
+            if (int r = autotune_flow<real>(s)) return r;
             s->tuned = true;
         }
         if (int r = rk4_launch<real>(s, t_pre, s->dt)) return r;
@@ -813,32 +826,238 @@ int iterate(phx_solver* s, bool force_fixed) {
     return s->precision == PHX_FP32 ? iterate_t<float>(s, force_fixed) : iterate_t<double>(s, force_fixed);
 }
 
-/* Fixed RK4 steps of the scalar model whose kernel arguments do not change from step to
-   step (constant temporal amplitudes, no noise) are replayed from a hipGraph of kGraphSteps
-   launches.  The host advances t and the FFT guard exactly as iterate() would; a block is
-   only replayed if no step in it reaches t_max or triggers the k-space filter.  Returns
-   the number of steps taken (0: not eligible, the caller iterates). */
-constexpr int kGraphSteps = 50; /* even: the buffer parity returns */
+constexpr int kGraphSteps = 50; /* graph replay: even, so the buffer parity returns */
+
+/* One launch of rk4_scalar_flow for `steps` (even) steps of the same run; returns 1 when the
+   kernel cannot take this grid or launch shape (the caller falls back to the graph). */
+constexpr int kFlowSpinLimit = 1 << 21; /* polls of about 1 us: a wait of ~2 s ends the launch */
+
+/* Kernel arguments of a fixed RK4 step of the scalar model from psi/n `in` to `out`, with the
+   per-stage temporal amplitudes at the handle's t (constant over a multi-step launch). */
+template <typename real>
+phx::Rk4Args<real> rk4_args(phx_solver* s, void* psi_in, void* n_in, void* psi_out, void* n_out) {
+    auto& c = s->cfg;
+    using v2 = typename phx::Vec2<real>::type;
+    phx::Rk4Args<real> a{};
+    a.psi_in = (const v2*)psi_in;
+    a.psi_out = (v2*)psi_out;
+    a.n_in = c.use_reservoir ? (const real*)n_in : nullptr;
+    a.n_out = c.use_reservoir ? (real*)n_out : nullptr;
+    a.pump = (const real*)s->pump[0];
+    a.pot = (const real*)s->pot[0];
+    a.Nc = c.N_c;
+    a.Nr = c.N_r;
+    a.periodic_x = c.periodic_x;
+    a.periodic_y = c.periodic_y;
+    a.dt = (real)s->dt;
+    static const double w[4] = {0.0, 0.5, 0.5, 1.0};
+    stage_amps<real>(c.pump, s->t, s->dt, w, 4, a.pamp);
+    stage_amps<real>(c.potential, s->t, s->dt, w, 4, a.vamp_h);
+    for (int i = 0; i < 4; i++) a.vamp_h[i] = (real)((double)a.vamp_h[i] * c.one_over_h_bar_s);
+    a.ph = phx::make_phys<real>(c);
+    return a;
+}
+
+/* Counters of the multi-step kernel, allocated on first use once the kernel is known to take
+   this grid with the handle's tile shape.  Returns 1 when it does not (or PHX_RK4_FLOW=0). */
+template <typename real>
+int flow_ready(phx_solver* s) {
+    if (s->flow_off) return 1;
+    if (const char* e = std::getenv("PHX_RK4_FLOW"))
+        if (std::atoi(e) == 0) return 1; /* PHX_RK4_FLOW=0: one launch per step */
+    auto& c = s->cfg;
+    const size_t counters = (size_t)((c.N_c + 55) / 56) * (size_t)((c.N_r + 7) / 8) * phx::kFlowCounterStride;
+    if (!s->flow_counters) {
+        const phx::Rk4Args<real> a = rk4_args<real>(s, s->psi[0][s->cur], s->n[0][s->cur], s->psi[0][s->cur ^ 1], s->n[0][s->cur ^ 1]);
+        if (phx::launch_rk4_flow<real>(a, s->tune, phx::Rk4Flow{nullptr, nullptr, 0, 2, 0}, s->stream) != hipSuccess) {
+            (void)hipGetLastError();
+            s->flow_off = true;
+            return 1;
+        }
+        if (int r = dev_alloc(s, (void**)&s->flow_counters, counters * sizeof(int))) return r; /* any tile height >= 8 */
+        if (int r = dev_alloc(s, (void**)&s->flow_error, sizeof(int))) return r;
+        s->flow_base = 0;
+    }
+    if (s->flow_base > (1 << 30)) { /* keep the counters far from overflow */
+        PHX_HIP(hipMemsetAsync(s->flow_counters, 0, counters * sizeof(int), s->stream));
+        s->flow_base = 0;
+    }
+    return PHX_OK;
+}
+
+/* One launch of the multi-step kernel: `steps` steps from psi/n `in` (`out` is the second buffer).
+   hipSuccess or the launcher's error (nothing ran). */
+template <typename real>
+hipError_t flow_launch(phx_solver* s, const phx::Rk4Tune& tune, void* psi_in, void* n_in, void* psi_out, void* n_out, int steps) {
+    int spin_limit = kFlowSpinLimit;
+    if (const char* e = std::getenv("PHX_FLOW_SPIN_LIMIT")) spin_limit = std::atoi(e); /* tests of the time-out path */
+    const phx::Rk4Args<real> a = rk4_args<real>(s, psi_in, n_in, psi_out, n_out);
+    const phx::Rk4Flow f{s->flow_counters, s->flow_error, s->flow_base, steps, spin_limit};
+    const hipError_t e = phx::launch_rk4_flow<real>(a, tune, f, s->stream);
+    if (e == hipSuccess) {
+        s->flow_base += steps;
+    } else {
+        (void)hipGetLastError();
+    }
+    return e;
+}
 
 template <typename real>
-int graph_block_t(phx_solver* s, int64_t* taken) {
+int flow_block_t(phx_solver* s, int steps) {
+    if (int r = flow_ready<real>(s)) return r;
+    if (flow_launch<real>(s, s->tune, s->psi[0][s->cur], s->n[0][s->cur], s->psi[0][s->cur ^ 1], s->n[0][s->cur ^ 1], steps) !=
+        hipSuccess) {
+        s->flow_off = true; /* nothing ran: step by graph from now on */
+        return 1;
+    }
+    s->flow_steps += steps;
+    s->flow_launches++;
+    return PHX_OK;
+}
+
+/* Tile shape of the multi-step kernel, timed on that kernel: every candidate runs kFlowTuneSteps
+   steps from the second buffer through a temporary one and back (an even count), so the state in
+   the current buffer is not touched.  Two passes, each candidate's faster one counts; shapes the
+   kernel does not take are skipped.  The winner replaces the one-step tuning's choice. */
+constexpr int kFlowTuneSteps = 40;
+
+template <typename real>
+int autotune_flow(phx_solver* s) {
+    static std::map<std::string, phx::Rk4Tune> cache;
+    auto& c = s->cfg;
+    if (std::getenv("PHX_RK4_TY")) return PHX_OK;
+    if (const char* e = std::getenv("PHX_AUTOTUNE"))
+        if (std::atoi(e) == 0) return PHX_OK;
+    const auto constant = [](const phx::Envelope& e) { return e.groupSize() != 1 || e.allConstant(); };
+    if (s->staged || c.iterator != "RK4" || c.tetm || c.use_stochastic || c.imag_time_amplitude != 0.0 || !constant(c.pump) ||
+        !constant(c.potential))
+        return PHX_OK;
+    const std::string key = std::to_string(sizeof(real)) + ":" + std::to_string(c.N_c) + "x" + std::to_string(c.N_r) + ":" +
+                            std::to_string(c.use_reservoir) + std::to_string(c.use_pumps) + std::to_string(c.use_potentials) +
+                            std::to_string(c.periodic_x) + std::to_string(c.periodic_y);
+    if (auto it = cache.find(key); it != cache.end()) {
+        s->tune = it->second;
+        return PHX_OK;
+    }
+    const phx::Rk4Tune one_step = s->tune;
+    s->tune.ty = 10; /* a shape the kernel has, for the availability check */
+    s->tune.pf = 2;
+    const int fr = flow_ready<real>(s);
+    s->tune = one_step;
+    if (fr) return fr < 0 ? fr : PHX_OK;
+    const size_t N = s->cells();
+    void *tmp_psi = nullptr, *tmp_n = nullptr;
+    PHX_HIP(hipMalloc(&tmp_psi, 2 * N * sizeof(real)));
+    if (c.use_reservoir && hipMalloc(&tmp_n, N * sizeof(real)) != hipSuccess) {
+        (void)hipFree(tmp_psi);
+        return fail(PHX_ERR_HIP, "allocation of the tuning buffer failed");
+    }
+    hipEvent_t e0, e1;
+    PHX_HIP(hipEventCreate(&e0));
+    PHX_HIP(hipEventCreate(&e1));
+    const int tys[] = {8, 10, 12, 16, 20};
+    const int pfs[] = {2, 3, 4, 6, 8};
+    const int npf = (c.use_reservoir && c.use_pumps && !c.use_potentials) ? 5 : 1;
+    std::vector<float> ms(5 * npf, 1e30f);
+    void* psi2 = s->psi[0][s->cur ^ 1];
+    void* n2 = s->n[0][s->cur ^ 1];
+    int rc = PHX_OK;
+    for (int pass = 0; pass < 2 && !rc; pass++) {
+        for (int cand = 0; cand < 5 * npf && !rc; cand++) {
+            phx::Rk4Tune t = one_step;
+            t.ty = tys[cand % 5];
+            t.pf = pfs[cand / 5];
+            if (hipEventRecord(e0, s->stream) != hipSuccess) rc = fail(PHX_ERR_HIP, "hipEventRecord");
+            if (!rc && flow_launch<real>(s, t, psi2, n2, tmp_psi, tmp_n, kFlowTuneSteps) != hipSuccess) continue;
+            float m = 0;
+            if (!rc && (hipEventRecord(e1, s->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                        hipEventElapsedTime(&m, e0, e1) != hipSuccess))
+                rc = fail(PHX_ERR_HIP, "event timing");
+            if (!rc) ms[cand] = std::min(ms[cand], m);
+        }
+    }
+    (void)hipStreamSynchronize(s->stream);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(tmp_psi);
+    if (tmp_n) (void)hipFree(tmp_n);
+    if (rc) return rc;
+    if (int r = flow_check(s)) return r;
+    int best = -1;
+    for (int cand = 0; cand < 5 * npf; cand++)
+        if (ms[cand] < 1e29f && (best < 0 || ms[cand] < ms[best])) best = cand;
+    if (best < 0) {
+        s->flow_off = true;
+        return PHX_OK;
+    }
+    s->tune.ty = tys[best % 5];
+    s->tune.pf = pfs[best / 5];
+    cache[key] = s->tune;
+    return PHX_OK;
+}
+
+/* The error word of rk4_scalar_flow, read after the stream has drained.  After a time-out the
+   counters no longer match flow_base: the multi-step launch stays off for this handle. */
+int flow_check(phx_solver* s) {
+    if (!s->flow_error) return PHX_OK;
+    int err = 0;
+    PHX_HIP(hipMemcpy(&err, s->flow_error, sizeof(int), hipMemcpyDeviceToHost));
+    if (err) {
+        s->flow_off = true;
+        PHX_HIP(hipMemset(s->flow_error, 0, sizeof(int))); /* reported once */
+        return fail(PHX_ERR_HIP, "multi-step RK4 launch: a tile waited too long for its neighbours; the state is undefined");
+    }
+    return PHX_OK;
+}
+
+/* Runs of fixed RK4 steps of the scalar model whose kernel arguments do not change from step
+   to step (constant temporal amplitudes, no noise) leave the one-launch-per-step path: up to
+   kFlowMaxSteps of them as one launch of the multi-step kernel, or, where that kernel does not
+   take the grid, kGraphSteps of them replayed from a captured hipGraph.  A run stops before the
+   step that reaches t_max, that fires the k-space filter, or that would pass t_stop (the
+   caller shortens that one); the host advances t and the FFT guard exactly as iterate()
+   would.  *taken: the steps run (0: not eligible, the caller iterates). */
+constexpr int64_t kFlowMaxSteps = 500;
+
+template <typename real>
+int block_t(phx_solver* s, int64_t max_steps, double t_stop, int64_t* taken) {
     auto& c = s->cfg;
     *taken = 0;
     const auto constant = [](const phx::Envelope& e) { return e.groupSize() != 1 || e.allConstant(); };
-    if (s->graph_off || !s->tuned || s->staged || c.iterator != "RK4" || c.tetm || c.use_stochastic || c.imag_time_amplitude != 0.0 ||
+    if (!s->tuned || s->staged || c.iterator != "RK4" || c.tetm || c.use_stochastic || c.imag_time_amplitude != 0.0 ||
         !constant(c.pump) || !constant(c.potential))
         return PHX_OK;
-    if (const char* e = std::getenv("PHX_GRAPH"))
-        if (std::atoi(e) == 0) return PHX_OK; /* PHX_GRAPH=0: launch by launch */
-    double t = s->t, fc = s->fft_cached_t;
-    for (int k = 0; k < kGraphSteps; k++) {
-        if (t >= c.t_max) return PHX_OK;
-        t = s->rnd(t + s->dt);
-        if (!(s->rnd(t - fc) < c.fft_every)) {
-            if (c.use_fft_mask || s->spec[0]) return PHX_OK;
-            fc = t;
+    double t = s->t, fc = s->fft_cached_t, t_graph = 0, fc_graph = 0;
+    int64_t m = 0;
+    const int64_t lim = std::min(max_steps, kFlowMaxSteps);
+    while (m < lim) {
+        if (t >= c.t_max || t >= t_stop) break;
+        const double tn = s->rnd(t + s->dt);
+        if (tn > t_stop) break;
+        double fcn = fc;
+        if (!(s->rnd(tn - fc) < c.fft_every)) {
+            if (c.use_fft_mask || s->spec[0]) break;
+            fcn = tn;
+        }
+        t = tn;
+        fc = fcn;
+        if (++m == kGraphSteps) t_graph = t, fc_graph = fc;
+    }
+    if (m >= 2) {
+        const int fr = flow_block_t<real>(s, (int)m);
+        if (fr < 0) return fr;
+        if (fr == PHX_OK) {
+            if (m & 1) s->cur ^= 1; /* an odd run leaves the result in the other buffer */
+            s->t = t;
+            s->fft_cached_t = fc;
+            s->iteration += m;
+            *taken = m;
+            return PHX_OK;
         }
     }
+    if (m < kGraphSteps || s->graph_off) return PHX_OK;
+    if (const char* e = std::getenv("PHX_GRAPH"))
+        if (std::atoi(e) == 0) return PHX_OK; /* PHX_GRAPH=0: launch by launch */
     if (!s->graph || s->graph_parity != s->cur || s->graph_dt != s->dt || s->graph_stream != s->stream) {
         if (s->graph) PHX_HIP(hipGraphExecDestroy(s->graph));
         s->graph = nullptr;
@@ -863,8 +1082,8 @@ int graph_block_t(phx_solver* s, int64_t* taken) {
         s->graph_stream = s->stream;
     }
     PHX_HIP(hipGraphLaunch(s->graph, s->stream));
-    s->t = t;
-    s->fft_cached_t = fc;
+    s->t = t_graph;
+    s->fft_cached_t = fc_graph;
     s->iteration += kGraphSteps;
     *taken = kGraphSteps;
     return PHX_OK;
@@ -935,9 +1154,11 @@ int phx_step(phx_solver* s, int64_t n_steps, int64_t* n_done) {
     if (!s || !s->initialized) return fail(PHX_ERR_STATE, "phx_step before phx_initialize");
     int64_t done = 0;
     while (done < n_steps) {
-        if (n_steps - done >= kGraphSteps) {
+        if (n_steps - done >= 2) {
             int64_t taken = 0;
-            const int r = s->precision == PHX_FP32 ? graph_block_t<float>(s, &taken) : graph_block_t<double>(s, &taken);
+            const double inf = std::numeric_limits<double>::infinity();
+            const int r = s->precision == PHX_FP32 ? block_t<float>(s, n_steps - done, inf, &taken)
+                                                   : block_t<double>(s, n_steps - done, inf, &taken);
             if (r < 0) {
                 if (n_done) *n_done = done;
                 return r;
@@ -990,6 +1211,17 @@ int phx_run_interval(phx_solver* s, int64_t* n_done, int* finished) {
            is exact in double, so rounding it once gives the real-type product. */
         const double target = s->rnd((double)s->out_iter * c.output_every);
         while (s->t < target) {
+            int64_t taken = 0;
+            const int br = s->precision == PHX_FP32 ? block_t<float>(s, kFlowMaxSteps, target, &taken)
+                                                    : block_t<double>(s, kFlowMaxSteps, target, &taken);
+            if (br < 0) {
+                if (n_done) *n_done = done;
+                return br;
+            }
+            if (taken) {
+                done += taken;
+                continue;
+            }
             const double dt = s->dt;
             bool forced = false;
             if (s->rnd(s->t + s->dt) > target) {
@@ -1047,7 +1279,7 @@ int phx_get_field(phx_solver* s, int field, void* host, size_t bytes) {
         if (int r = s->precision == PHX_FP32 ? spectrum_ready_t<float>(s) : spectrum_ready_t<double>(s)) return r;
     PHX_HIP(hipMemcpyAsync(host, *slot, bytes, hipMemcpyDeviceToHost, s->stream));
     PHX_HIP(hipStreamSynchronize(s->stream));
-    return PHX_OK;
+    return flow_check(s);
 }
 
 int phx_set_field(phx_solver* s, int field, const void* host, size_t bytes) {
@@ -1196,6 +1428,8 @@ int phx_get_info(phx_solver* s, phx_info* info) {
     info->tile_rows = s->tuned ? s->tune.ty : 0;
     info->prefetch_rows = s->tuned ? s->tune.pf : 0;
     info->staged = s->staged;
+    info->multi_step_steps = s->flow_steps;
+    info->multi_step_launches = s->flow_launches;
     return PHX_OK;
 }
 
@@ -1208,7 +1442,7 @@ int phx_set_stream(phx_solver* s, void* stream) {
 int phx_sync(phx_solver* s) {
     if (!s) return fail(PHX_ERR_INVALID, "null handle");
     if (s->stream) PHX_HIP(hipStreamSynchronize(s->stream));
-    return PHX_OK;
+    return flow_check(s);
 }
 
 void phx_destroy(phx_solver* s) {

@@ -2,8 +2,10 @@
 
 Bar (stated per test): relative L2 of psi and n after the given number of steps
   FP64: <= 1e-12, FP32: <= 1e-5  (the parity bar of this path).
-At the full 1024^2 size the oracle would take minutes, so size-independent
-properties are checked there instead (translation equivariance, FP32 vs FP64).
+The tests here use sizes up to 256^2; the benchmark workloads at their own sizes
+(1024^2 and larger) are compared with the oracle in test_gpu_workloads.py.  At
+1024^2 this file adds size-independent properties (translation equivariance,
+FP32 vs FP64).
 """
 import sys
 
