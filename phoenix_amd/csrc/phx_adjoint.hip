@@ -486,7 +486,11 @@ int adj_step(phx_adjoint* a, void* state, int lin, int ty) {
 
 /* Tile heights of both kernels, timed on the first backward call.  The timed
    launches read the current state and cotangent and write the other buffers,
-   so nothing advances; the gradient accumulators are cleared afterwards. */
+   so nothing advances; the gradient accumulators are cleared afterwards.
+   Three round-robin passes over the candidates, the fastest pass of each
+   counting: one pass in a fixed order timed the first candidates while the
+   clocks were still ramping (after a long idle or a different workload) and
+   could keep a tile height 40 % slower than the best. */
 template <typename real>
 int autotune(phx_adjoint* a) {
     hipEvent_t e0, e1;
@@ -503,28 +507,33 @@ int autotune(phx_adjoint* a) {
         ADJ_HIP(hipEventElapsedTime(&ms, e0, e1));
         return PHX_OK;
     };
-    float best = 1e30f;
+    const int fwd_tys[] = {8, 10, 12, 16, 20};
+    const int adj_tys[] = {16, 20, 24, 28, 32, 40, 48, 56, 64, 80};
+    float fwd_ms[5], adj_ms[10];
+    for (float& m : fwd_ms) m = 1e30f;
+    for (float& m : adj_ms) m = 1e30f;
     int rc = PHX_OK;
-    for (int ty : {8, 10, 12, 16, 20}) {
-        float ms = 0;
-        rc = timeit([&] { return fwd_step<real>(a, a->state_psi(a->ckpt_at(0)), a->state_n(a->ckpt_at(0)), a->state_psi(a->lam[1]),
-                                                a->state_n(a->lam[1]), ty); },
-                    ms);
-        if (rc) break;
-        if (ms < best) best = ms, a->ty_fwd = ty;
-    }
-    best = 1e30f;
-    for (int ty : {16, 20, 24, 28, 32, 40, 48, 56, 64, 80}) {
-        if (rc) break;
-        float ms = 0;
-        rc = timeit([&] { return adj_step<real>(a, a->ckpt_at(0), 0, ty); }, ms);
-        if (rc) break;
-        if (ms < best) best = ms, a->ty_adj = ty;
+    for (int pass = 0; pass < 3 && !rc; pass++) {
+        for (int i = 0; i < 5 && !rc; i++) {
+            float ms = 0;
+            rc = timeit([&] { return fwd_step<real>(a, a->state_psi(a->ckpt_at(0)), a->state_n(a->ckpt_at(0)), a->state_psi(a->lam[1]),
+                                                    a->state_n(a->lam[1]), fwd_tys[i]); },
+                        ms);
+            if (!rc) fwd_ms[i] = std::min(fwd_ms[i], ms);
+        }
+        for (int i = 0; i < 10 && !rc; i++) {
+            float ms = 0;
+            rc = timeit([&] { return adj_step<real>(a, a->ckpt_at(0), 0, adj_tys[i]); }, ms);
+            if (!rc) adj_ms[i] = std::min(adj_ms[i], ms);
+        }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    if (!rc) a->tuned = true;
-    return rc;
+    if (rc) return rc;
+    a->ty_fwd = fwd_tys[std::min_element(fwd_ms, fwd_ms + 5) - fwd_ms];
+    a->ty_adj = adj_tys[std::min_element(adj_ms, adj_ms + 10) - adj_ms];
+    a->tuned = true;
+    return PHX_OK;
 }
 
 template <typename real>
@@ -567,6 +576,7 @@ int backward_t(phx_adjoint* a, const void* gpsiT, const void* gnT, void* gP, voi
     if (!a->tuned)
         if (int r = autotune<real>(a)) return r;
     if (const char* e = std::getenv("PHX_ADJ_TY")) a->ty_adj = std::max(1, std::atoi(e)); /* tuning experiments */
+    if (const char* e = std::getenv("PHX_ADJ_FWD_TY")) a->ty_fwd = std::max(8, std::atoi(e));
     ADJ_HIP(hipMemsetAsync(a->gp, 0, rb, a->stream));
     ADJ_HIP(hipMemsetAsync(a->gv, 0, rb, a->stream));
     /* Segment j's states live in seg[j % 2].  While the main stream runs the adjoint

@@ -159,36 +159,33 @@ int launch_step(phx_ensemble* e, double dt, int parity) {
 
 /* Pick the tile height with the shortest step for this ensemble: every candidate
    launch reads the current buffers and writes the other ones, so the state
-   does not advance. */
+   does not advance.  Three round-robin passes, each candidate's fastest counting
+   (a single pass in a fixed order times the first candidates while the clocks ramp). */
 template <typename real>
 int autotune(phx_ensemble* e, double dt) {
     hipEvent_t e0, e1;
     ENS_HIP(hipEventCreate(&e0));
     ENS_HIP(hipEventCreate(&e1));
-    float best = 1e30f;
-    int best_ty = e->ty;
-    for (int ty : {8, 10, 12, 16, 20}) {
-        int rc = build_tables<real>(e, ty);
-        for (int w = 0; w < 2 && !rc; w++) rc = launch_step<real>(e, dt, e->cur);
-        if (!rc && hipEventRecord(e0, e->stream) != hipSuccess) rc = efail(PHX_ERR_HIP, "hipEventRecord");
-        for (int r = 0; r < 5 && !rc; r++) rc = launch_step<real>(e, dt, e->cur);
-        if (!rc && hipEventRecord(e1, e->stream) != hipSuccess) rc = efail(PHX_ERR_HIP, "hipEventRecord");
-        float ms = 0;
-        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-            rc = efail(PHX_ERR_HIP, "event timing");
-        if (rc) {
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-            return rc;
-        }
-        if (ms < best) {
-            best = ms;
-            best_ty = ty;
+    const int tys[] = {8, 10, 12, 16, 20};
+    float ms_best[5] = {1e30f, 1e30f, 1e30f, 1e30f, 1e30f};
+    int rc = PHX_OK;
+    for (int pass = 0; pass < 3 && !rc; pass++) {
+        for (int i = 0; i < 5 && !rc; i++) {
+            rc = build_tables<real>(e, tys[i]);
+            for (int w = 0; w < 2 && !rc; w++) rc = launch_step<real>(e, dt, e->cur);
+            if (!rc && hipEventRecord(e0, e->stream) != hipSuccess) rc = efail(PHX_ERR_HIP, "hipEventRecord");
+            for (int r = 0; r < 5 && !rc; r++) rc = launch_step<real>(e, dt, e->cur);
+            if (!rc && hipEventRecord(e1, e->stream) != hipSuccess) rc = efail(PHX_ERR_HIP, "hipEventRecord");
+            float ms = 0;
+            if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = efail(PHX_ERR_HIP, "event timing");
+            if (!rc) ms_best[i] = std::min(ms_best[i], ms);
         }
     }
-    ENS_HIP(hipEventDestroy(e0));
-    ENS_HIP(hipEventDestroy(e1));
-    e->ty = best_ty;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    e->ty = tys[std::min_element(ms_best, ms_best + 5) - ms_best];
     e->ty_auto = false;
     return PHX_OK;
 }
