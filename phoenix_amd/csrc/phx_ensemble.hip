@@ -147,6 +147,7 @@ int launch_step(phx_ensemble* e, double dt, int parity) {
     b.iso = e->iso;
     b.common.periodic_x = e->periodic_x;
     b.common.periodic_y = e->periodic_y;
+    b.common.alt_rows = phx::rk4_alt_rows();
     b.common.dt = (real)dt;
     for (int k = 0; k < 4; k++) {
         b.common.pamp[k] = 1;

@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cstdlib>
+
 #include "custom_rhs.h"
 #include "phx_tableaux.h"
 
@@ -34,11 +36,18 @@ struct Rk4Args {
     int Nc, Nr;
     int periodic_x, periodic_y;
     int nstrips, nbands; /* set by the launcher */
+    int alt_rows;        /* 1: odd bands of tiles stream their rows bottom to top (same results) */
     real dt;
     real pamp[4];   /* pump temporal amplitude per stage: a(t)(1-w)+w a(t+dt) */
     real vamp_h[4]; /* potential temporal amplitude per stage, divided by hbar */
     PhxPhys<real> ph;
 };
+
+/* Row order of the fused RK4 tiles (Rk4Args::alt_rows): alternating by band unless PHX_RK4_ALT=0. */
+inline int rk4_alt_rows() {
+    const char* e = std::getenv("PHX_RK4_ALT");
+    return e && std::atoi(e) == 0 ? 0 : 1;
+}
 
 /* Launch shape of the fused RK4 kernel.
    ty: output rows per wavefront tile (8, 10, 12, 16, 20, 24);

@@ -79,6 +79,12 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
     /* the same arrays with no records: output rows past the grid end store through them (dropped) */
     const rsrc_t w_psi_none = row_rsrc(a.psi_out, 0u), w_n_none = row_rsrc(a.n_out, 0u);
     const RowWrap wrap = make_row_wrap(Nr, kHalo);
+    /* tile row r is grid row row0 + rdir * r.  With alt_rows, odd bands stream bottom to top: the 8
+       halo rows that two vertically adjacent tiles share are then read by both at the same end of
+       their sweeps, close in time, so the second read finds them in L2.  North and south only swap
+       places in a sum, so every cell's result is the same either way. */
+    const bool up = a.alt_rows && (band & 1);
+    const int row0 = up ? y0 + TY + kHalo - 1 : y0 - kHalo, rdir = up ? -1 : 1;
 
     constexpr int E = TY + 2 * kHalo;
     real p0r[E], p0i[E], n0[E], pp[E], vv[E];
@@ -97,7 +103,7 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
 
     auto load_row = [&](int r) {
         bool in;
-        const uint32_t grow = tile_row(y0 - kHalo + r, wrap, PERY, in); /* wave-uniform */
+        const uint32_t grow = tile_row(row0 + rdir * r, wrap, PERY, in); /* wave-uniform */
         const uint32_t sc = grow * row_c, sr = grow * row_r;
         const uint32_t nc = in ? all_c : 0u, nr = in ? all_r : 0u;
         const auto x = load_cplx_s<real, CP>(row_rsrc(a.psi_in, nc), off_c, sc);
@@ -184,7 +190,7 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
                 rhs(y3r[e], y3i[e], y3r[e - 1] + y3r[e + 1], y3i[e - 1] + y3i[e + 1], RES ? yn3[e] : real(0),
                     pp[e] * a.pamp[3], vv[e] * a.vamp_h[3], kr, ki, kn);
                 /* rows past the last grid row store through an empty descriptor: no branch */
-                const int grow = y0 + e - kHalo;
+                const int grow = row0 + rdir * e;
                 const bool row_ok = grow < Nr;
                 v2 o;
                 o.x = phx_fma(dt6, kr, acr[e]);

@@ -385,6 +385,7 @@ int rk4_launch(phx_solver* s, double t_pre, double dt) {
     a.Nr = c.N_r;
     a.periodic_x = c.periodic_x;
     a.periodic_y = c.periodic_y;
+    a.alt_rows = phx::rk4_alt_rows();
     a.dt = (real)dt;
     static const double w[4] = {0.0, 0.5, 0.5, 1.0};
     stage_amps<real>(c.pump, t_pre, dt, w, 4, a.pamp);
@@ -849,6 +850,7 @@ phx::Rk4Args<real> rk4_args(phx_solver* s, void* psi_in, void* n_in, void* psi_o
     a.Nr = c.N_r;
     a.periodic_x = c.periodic_x;
     a.periodic_y = c.periodic_y;
+    a.alt_rows = phx::rk4_alt_rows();
     a.dt = (real)s->dt;
     static const double w[4] = {0.0, 0.5, 0.5, 1.0};
     stage_amps<real>(c.pump, s->t, s->dt, w, 4, a.pamp);
@@ -1510,6 +1512,7 @@ static int rk4_device_t(const phx_physics* ph, int N_c, int N_r, int px, int py,
     a.Nr = N_r;
     a.periodic_x = px;
     a.periodic_y = py;
+    a.alt_rows = phx::rk4_alt_rows();
     a.dt = (real)dt;
     for (int i = 0; i < 4; i++) {
         a.pamp[i] = 1;
