@@ -271,6 +271,252 @@ __global__ __launch_bounds__(kFftThreads) void fft_cols_filter(typename Vec2<rea
 }
 
 /* ------------------------------------------------------------------------------------------
+   Four-step transforms for N = N1 N2 (N1, N2 in {16, 32}).  With n = N2 n1 + n2 and k = k1 + N1 k2,
+       X[k1 + N1 k2] = sum_n2 W_N2^(n2 k2) W_N^(n2 k1) sum_n1 x[N2 n1 + n2] W_N1^(n1 k1),
+   so a thread loads one strided sub-sequence (n2 fixed, n1 = 0..N1-1) into registers, transforms
+   it there, applies the twiddle W_N^(n2 k1), and writes it to LDS transposed; after one barrier a
+   thread reads one sub-sequence of the other kind (k1 fixed) and transforms it in registers.  One
+   LDS round trip and one barrier per length-N transform instead of log4 N Stockham passes with two
+   barriers each.  Global accesses: in a row, the threads of a sequence read consecutive elements;
+   in the column pass, consecutive threads read consecutive columns. */
+
+/* in-register radix-2 FFT of R elements: X[k] = sum_j x[j] e^(SIGN 2 pi i j k / R); tw holds
+   W_N^m = e^(-2 pi i m / N) for m < N and R divides N (uniform indices: scalar loads) */
+template <typename real, int R, int SIGN>
+__device__ __forceinline__ void fft_reg(Cx<real> (&x)[R], const Cx<real>* __restrict__ tw, int N) {
+    constexpr int bits = R == 2 ? 1 : R == 4 ? 2 : R == 8 ? 3 : R == 16 ? 4 : R == 32 ? 5 : 6;
+    Cx<real> y[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+        int r = 0;
+#pragma unroll
+        for (int b = 0; b < bits; b++) r |= ((i >> b) & 1) << (bits - 1 - b);
+        y[r] = x[i];
+    }
+#pragma unroll
+    for (int len = 2; len <= R; len <<= 1) {
+        const int half = len >> 1;
+#pragma unroll
+        for (int j = 0; j < half; j++) {
+            Cx<real> w = tw[j * (N / len)];
+            if (SIGN > 0) w.i = -w.i;
+#pragma unroll
+            for (int i = 0; i < R; i += len) {
+                const Cx<real> u = y[i + j];
+                const Cx<real> v = j == 0 ? y[i + j + half] : cmul(y[i + j + half], w);
+                y[i + j] = {u.r + v.r, u.i + v.i};
+                y[i + j + half] = {u.r - v.r, u.i - v.i};
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < R; i++) x[i] = y[i];
+}
+
+/* LDS slot of (row, col) in a [rows][W] tile, W a power of two: the column is XORed with the row,
+   so that both the row-wise writes and the column-wise reads of the transpose spread over the banks
+   without padding (a tile stays within 64 KiB) */
+template <int W>
+__device__ __forceinline__ int swz(int row, int col) { return row * W + (col ^ (row & (W - 1))); }
+
+template <typename real, int SIGN>
+__device__ __forceinline__ Cx<real> twiddle(const Cx<real>* __restrict__ tw, int m) {
+    Cx<real> w = tw[m];
+    if (SIGN > 0) w.i = -w.i;
+    return w;
+}
+
+/* threads per workgroup of the four-step kernels */
+template <typename real>
+__host__ __device__ constexpr int fft4_threads() { return sizeof(real) == 4 ? 256 : 128; }
+
+/* rows: fft4_threads / max(N1, N2) rows per workgroup, LDS [rows][N1][N2] (swizzled) */
+template <typename real, int N1, int N2, int SIGN>
+__global__ __launch_bounds__(256) void fft4_rows(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw, int nrows) {
+    constexpr int N = N1 * N2, T = N1 > N2 ? N1 : N2, RPB = fft4_threads<real>() / T;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    Cx<real>* S = reinterpret_cast<Cx<real>*>(lds_raw);
+    const int lr = threadIdx.x / T, t = threadIdx.x - lr * T;
+    const int row = blockIdx.x * RPB + lr;
+    const bool live = row < nrows;
+    typename Vec2<real>::type* rp = data + (size_t)(live ? row : 0) * N;
+    Cx<real>* sr = S + lr * N;
+    if (t < N2) {
+        Cx<real> x[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) {
+            const auto v = rp[N2 * n1 + t];
+            x[n1] = {v.x, v.y};
+        }
+        fft_reg<real, N1, SIGN>(x, tw, N);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) sr[swz<N2>(k1, t)] = k1 == 0 ? x[0] : cmul(x[k1], twiddle<real, SIGN>(tw, t * k1));
+    }
+    __syncthreads();
+    if (t < N1 && live) {
+        Cx<real> y[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; n2++) y[n2] = sr[swz<N2>(t, n2)];
+        fft_reg<real, N2, SIGN>(y, tw, N);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) {
+            typename Vec2<real>::type o;
+            o.x = y[k2].r;
+            o.y = y[k2].i;
+            rp[t + N1 * k2] = o;
+        }
+    }
+}
+
+/* columns: CB adjacent columns per workgroup (thread = column + CB x sub-sequence), forward
+   transform, the k-space factor of MODE (as fft_cols_filter), inverse transform; LDS [CB][N1][N2]
+   for the forward transpose and [CB][N2][N1] for the inverse (swizzled).  CB = fft4_threads /
+   max(N1, N2): 8 FP32 / 4 FP64 columns at 1024^2, 128 / 256 workgroups.  Whole 128-byte row segments
+   (16 FP32 / 8 FP64 columns, 128 KiB of LDS) left 64 / 128 workgroups and took 34.0 / 25.3 us
+   instead of 19.4 / 19.7 us: the pass is bound by the parallelism, not by the segment width. */
+template <typename real, int N1, int N2, int CB, int MODE>
+__global__ __launch_bounds__(CB *(N1 > N2 ? N1 : N2)) void fft4_cols(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw,
+                                                 const real* __restrict__ mask, const Cx<real>* __restrict__ rowf,
+                                                 const Cx<real>* __restrict__ colf, int Nc, real Ntot, typename Vec2<real>::type* spec) {
+    constexpr int N = N1 * N2;
+    constexpr int SLAB = N;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    Cx<real>* S = reinterpret_cast<Cx<real>*>(lds_raw);
+    const int c = threadIdx.x % CB, t = threadIdx.x / CB;
+    const int col = fft_xcd_group(blockIdx.x, gridDim.x) * CB + c;
+    Cx<real>* sc = S + c * SLAB;
+    /* forward, first half: n2 = t */
+    if (t < N2) {
+        Cx<real> x[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) {
+            const auto v = data[(size_t)(N2 * n1 + t) * Nc + col];
+            x[n1] = {v.x, v.y};
+        }
+        fft_reg<real, N1, -1>(x, tw, N);
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) sc[swz<N2>(k1, t)] = k1 == 0 ? x[0] : cmul(x[k1], twiddle<real, -1>(tw, t * k1));
+    }
+    __syncthreads();
+    /* forward, second half: k1 = t holds X[t + N1 k2], k2 = 0..N2-1 */
+    Cx<real> y[N2];
+    if (t < N1) {
+#pragma unroll
+        for (int n2 = 0; n2 < N2; n2++) y[n2] = sc[swz<N2>(t, n2)];
+        fft_reg<real, N2, -1>(y, tw, N);
+        if constexpr (MODE == kColForward) {
+#pragma unroll
+            for (int k2 = 0; k2 < N2; k2++) {
+                typename Vec2<real>::type o;
+                o.x = y[k2].r;
+                o.y = y[k2].i;
+                data[(size_t)(t + N1 * k2) * Nc + col] = o;
+            }
+        } else if constexpr (MODE == kColPropagate) {
+            const Cx<real> cf = colf[col];
+#pragma unroll
+            for (int k2 = 0; k2 < N2; k2++) y[k2] = cmul(cmul(y[k2], cf), rowf[t + N1 * k2]);
+        } else {
+            const int cs = (col + (Nc >> 1)) & (Nc - 1);
+            real mk[N2];
+#pragma unroll
+            for (int k2 = 0; k2 < N2; k2++) mk[k2] = mask[(size_t)(((t + N1 * k2) + (N >> 1)) & (N - 1)) * Nc + cs];
+#pragma unroll
+            for (int k2 = 0; k2 < N2; k2++) {
+                y[k2].r = y[k2].r / Ntot * mk[k2];
+                y[k2].i = y[k2].i / Ntot * mk[k2];
+                if (spec) {
+                    typename Vec2<real>::type o;
+                    o.x = y[k2].r;
+                    o.y = y[k2].i;
+                    spec[(size_t)(t + N1 * k2) * Nc + col] = o;
+                }
+            }
+        }
+    }
+    if constexpr (MODE == kColForward) return;
+    __syncthreads(); /* the forward transpose slab is reused */
+    /* inverse, first half: k1 = t; Z[p] = sum_k2 X[t + N1 k2] W_N2^(-p k2), times W_N^(-p t) */
+    if (t < N1) {
+        fft_reg<real, N2, +1>(y, tw, N);
+#pragma unroll
+        for (int p = 0; p < N2; p++) sc[swz<N1>(p, t)] = p == 0 ? y[0] : cmul(y[p], twiddle<real, +1>(tw, p * t));
+    }
+    __syncthreads();
+    /* inverse, second half: p = t; x[t + N2 q] = sum_k1 S[t][k1] W_N1^(-q k1) */
+    if (t < N2) {
+        Cx<real> w[N1];
+#pragma unroll
+        for (int k1 = 0; k1 < N1; k1++) w[k1] = sc[swz<N1>(t, k1)];
+        fft_reg<real, N1, +1>(w, tw, N);
+#pragma unroll
+        for (int q = 0; q < N1; q++) {
+            typename Vec2<real>::type o;
+            o.x = w[q].r;
+            o.y = w[q].i;
+            data[(size_t)(t + N2 * q) * Nc + col] = o;
+        }
+    }
+}
+
+/* Lengths of the four-step path: (N1, N2) with N1 N2 = N */
+inline bool fft4_split(int N, int& N1, int& N2) {
+    switch (N) {
+        case 256: N1 = 16, N2 = 16; return true;
+        case 512: N1 = 32, N2 = 16; return true;
+        case 1024: N1 = 32, N2 = 32; return true;
+        default: return false;
+    }
+}
+
+static bool fft4_enabled() {
+    const char* e = std::getenv("PHX_FFT4");
+    return !(e && std::atoi(e) == 0);
+}
+
+template <typename real, int N1, int N2, int SIGN>
+static hipError_t launch_fft4_rows_t(typename Vec2<real>::type* psi, const void* tw, int Nr, hipStream_t s) {
+    constexpr int T = N1 > N2 ? N1 : N2, RPB = fft4_threads<real>() / T;
+    const size_t lds = (size_t)RPB * N1 * N2 * sizeof(Cx<real>);
+    hipLaunchKernelGGL((fft4_rows<real, N1, N2, SIGN>), dim3((Nr + RPB - 1) / RPB), dim3(fft4_threads<real>()), lds, s, psi,
+                       (const Cx<real>*)tw, Nr);
+    return hipGetLastError();
+}
+
+template <typename real, int SIGN>
+static hipError_t launch_fft4_rows(typename Vec2<real>::type* psi, const void* tw, int Nc, int Nr, hipStream_t s) {
+    switch (Nc) {
+        case 256: return launch_fft4_rows_t<real, 16, 16, SIGN>(psi, tw, Nr, s);
+        case 512: return launch_fft4_rows_t<real, 32, 16, SIGN>(psi, tw, Nr, s);
+        case 1024: return launch_fft4_rows_t<real, 32, 32, SIGN>(psi, tw, Nr, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <typename real, int N1, int N2, int MODE>
+static hipError_t launch_fft4_cols_t(typename Vec2<real>::type* psi, const void* tw, const real* mask, const void* rowf,
+                                     const void* colf, int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec) {
+    constexpr int T = N1 > N2 ? N1 : N2, CB = fft4_threads<real>() / T;
+    if (Nc % CB) return hipErrorInvalidValue;
+    const size_t lds = (size_t)CB * N1 * N2 * sizeof(Cx<real>);
+    const real ntot = (real)((double)Nc * (double)Nr);
+    hipLaunchKernelGGL((fft4_cols<real, N1, N2, CB, MODE>), dim3(Nc / CB), dim3(CB * T), lds, s, psi, (const Cx<real>*)tw, mask,
+                       (const Cx<real>*)rowf, (const Cx<real>*)colf, Nc, ntot, spec);
+    return hipGetLastError();
+}
+
+template <typename real, int MODE>
+static hipError_t launch_fft4_cols(typename Vec2<real>::type* psi, const void* tw, const real* mask, const void* rowf, const void* colf,
+                                   int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec) {
+    switch (Nr) {
+        case 256: return launch_fft4_cols_t<real, 16, 16, MODE>(psi, tw, mask, rowf, colf, Nc, Nr, s, spec);
+        case 512: return launch_fft4_cols_t<real, 32, 16, MODE>(psi, tw, mask, rowf, colf, Nc, Nr, s, spec);
+        case 1024: return launch_fft4_cols_t<real, 32, 32, MODE>(psi, tw, mask, rowf, colf, Nc, Nr, s, spec);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------
    Direct DFT for lengths that are not powers of two (any even N; the reference validates even
    grid sizes, system_validate_input.cpp:19-56): out[k] = sum_j x[j] w^((j k) mod N) with
    w = e^(-+2 pi i / N) from the twiddle table, one output per thread and pass, the sequence in
@@ -419,11 +665,12 @@ static int bfly_bound(int butterflies) {
     return kb;
 }
 
-/* PHX_FFT_PAD=0 keeps the unpadded LDS layout (tuning runs); padding is used where the padded
-   buffer fits the 64 KiB a workgroup may hold */
+/* PHX_FFT_PAD=1 pads the LDS index of the Stockham kernels (one slot per 32) where the padded buffer
+   fits the 64 KiB a workgroup may hold: it removes the 4-way bank conflicts of the first passes'
+   writes but measured slower at 1024^2 (39.4 vs 37.4 us FP32, 51.8 vs 49.5 us FP64), so it is off */
 static bool pad_lds(size_t elems, size_t elem_size) {
-    static const bool env_off = [] { const char* e = std::getenv("PHX_FFT_PAD"); return e && std::atoi(e) == 0; }();
-    return !env_off && lds_elems<true>(elems) * elem_size <= 65536;
+    static const bool env_on = [] { const char* e = std::getenv("PHX_FFT_PAD"); return e && std::atoi(e) == 1; }();
+    return env_on && lds_elems<true>(elems) * elem_size <= 65536;
 }
 
 template <typename real, int SIGN, bool PAD>
@@ -487,6 +734,13 @@ static hipError_t launch_cols(typename Vec2<real>::type* psi, const void* tw_r, 
 template <typename real, int MODE>
 static hipError_t fft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, const void* rowf,
                                 const void* colf, int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec = nullptr) {
+    int n1, n2;
+    if (fft4_enabled() && fft4_split(Nc, n1, n2) && fft4_split(Nr, n1, n2) && Nc % (fft4_threads<real>() / 16) == 0) {
+        hipError_t e = launch_fft4_rows<real, -1>(psi, tw_c, Nc, Nr, s);
+        if (e == hipSuccess) e = launch_fft4_cols<real, MODE>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
+        if (e != hipSuccess || MODE == kColForward) return e;
+        return launch_fft4_rows<real, +1>(psi, tw_c, Nc, Nr, s);
+    }
     hipError_t e = launch_rows<real, -1>(psi, tw_c, Nc, Nr, s);
     if (e != hipSuccess) return e;
     /* column block width: as wide as LDS and registers allow, but no wider than keeps
