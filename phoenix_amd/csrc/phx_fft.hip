@@ -326,6 +326,13 @@ __device__ __forceinline__ Cx<real> twiddle(const Cx<real>* __restrict__ tw, int
     return w;
 }
 
+/* LDS elements between the column slabs of fft4_cols beyond N: the slabs of the CB columns a wave
+   touches then start on different banks (N apart, the same (k1, t) of every column fell on one
+   bank).  SQ_LDS_BANK_CONFLICT per LDS instruction at 1024^2, FP32 / FP64: skew 0 21 / -, 2 1 / 2,
+   4 2 / 4, 8 7 / 14; the pass time moved by under 1 % between skews 2 and 8 (parallelism-bound). */
+template <typename real>
+__host__ __device__ constexpr int fft4_col_skew() { return 2; }
+
 /* threads per workgroup of the four-step kernels */
 template <typename real>
 __host__ __device__ constexpr int fft4_threads() { return sizeof(real) == 4 ? 256 : 128; }
@@ -379,7 +386,7 @@ __global__ __launch_bounds__(CB *(N1 > N2 ? N1 : N2)) void fft4_cols(typename Ve
                                                  const real* __restrict__ mask, const Cx<real>* __restrict__ rowf,
                                                  const Cx<real>* __restrict__ colf, int Nc, real Ntot, typename Vec2<real>::type* spec) {
     constexpr int N = N1 * N2;
-    constexpr int SLAB = N;
+    constexpr int SLAB = N + fft4_col_skew<real>();
     extern __shared__ __align__(16) unsigned char lds_raw[];
     Cx<real>* S = reinterpret_cast<Cx<real>*>(lds_raw);
     const int c = threadIdx.x % CB, t = threadIdx.x / CB;
@@ -498,7 +505,11 @@ static hipError_t launch_fft4_cols_t(typename Vec2<real>::type* psi, const void*
                                      const void* colf, int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec) {
     constexpr int T = N1 > N2 ? N1 : N2, CB = fft4_threads<real>() / T;
     if (Nc % CB) return hipErrorInvalidValue;
-    const size_t lds = (size_t)CB * N1 * N2 * sizeof(Cx<real>);
+    const size_t lds = (size_t)CB * (N1 * N2 + fft4_col_skew<real>()) * sizeof(Cx<real>);
+    static const hipError_t attr = lds > 65536 ? hipFuncSetAttribute((const void*)&fft4_cols<real, N1, N2, CB, MODE>,
+                                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds)
+                                               : hipSuccess; /* beyond 64 KiB of LDS (gfx950 grants up to 160 KiB) */
+    if (attr != hipSuccess) return attr;
     const real ntot = (real)((double)Nc * (double)Nr);
     hipLaunchKernelGGL((fft4_cols<real, N1, N2, CB, MODE>), dim3(Nc / CB), dim3(CB * T), lds, s, psi, (const Cx<real>*)tw, mask,
                        (const Cx<real>*)rowf, (const Cx<real>*)colf, Nc, ntot, spec);
@@ -736,10 +747,15 @@ static hipError_t fft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c
                                 const void* colf, int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec = nullptr) {
     int n1, n2;
     if (fft4_enabled() && fft4_split(Nc, n1, n2) && fft4_split(Nr, n1, n2) && Nc % (fft4_threads<real>() / 16) == 0) {
-        hipError_t e = launch_fft4_rows<real, -1>(psi, tw_c, Nc, Nr, s);
+        /* FP32 rows stay on the Stockham kernel: its 4 K waves per 1024 rows (four elements a thread)
+           beat the four-step row kernel's 512 at 1024^2 (6.95 / 9.88 vs 7.88 / 10.14 us forward /
+           inverse); in FP64 the four-step rows are faster (9.05 / 13.81 vs 10.0 / 15.7 us).  Both
+           leave each row transformed in natural order, in place. */
+        constexpr bool rows4 = sizeof(real) == 8;
+        hipError_t e = rows4 ? launch_fft4_rows<real, -1>(psi, tw_c, Nc, Nr, s) : launch_rows<real, -1>(psi, tw_c, Nc, Nr, s);
         if (e == hipSuccess) e = launch_fft4_cols<real, MODE>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
         if (e != hipSuccess || MODE == kColForward) return e;
-        return launch_fft4_rows<real, +1>(psi, tw_c, Nc, Nr, s);
+        return rows4 ? launch_fft4_rows<real, +1>(psi, tw_c, Nc, Nr, s) : launch_rows<real, +1>(psi, tw_c, Nc, Nr, s);
     }
     hipError_t e = launch_rows<real, -1>(psi, tw_c, Nc, Nr, s);
     if (e != hipSuccess) return e;
