@@ -149,11 +149,13 @@ int dp45_adaptive_tile_rows(bool tetm, bool fp32, int Nc, int Nr) {
             cus = 256;
         return 4 * std::max(cus, 1);
     }();
-    /* One wave per SIMD at the kernel's register count, and about the same VALU work per row
-       evaluation at both heights: a SIMD's time is its number of tiles times the row evaluations of
-       one tile, S (TY + 2 S) - S (S + 1) for S = 7.  At 2048^2 this predicts 32 rows 1.16x faster than
-       16 (1.21x measured).  A 42-row variant, predicted faster still, measured 1.24x slower than 32
-       rows and is not built. */
+    /* One wave per SIMD at the kernel's register count (over 256 VGPRs + AGPRs at every height),
+       and about the same VALU work per row evaluation at every height (145-148 instructions): a
+       SIMD's time is its number of tiles (rounded up) times the row evaluations of one tile,
+       S (TY + 2 S) - S (S + 1) for S = 7.  At 2048^2 this predicts 32 rows 1.16x faster than 16
+       (1.21x measured).  Other heights that the model ranks near or above 32 measured slower and
+       are not built: 42 rows (2009 tiles, 2 rounds) 1.24x slower, 28 rows (3034 tiles, 3 full
+       rounds of 238 row evaluations against 3 x 266) 1.19x slower. */
     constexpr int S = 7;
     int best = kRkGenMinTileRows;
     long best_cost = -1;
