@@ -33,10 +33,11 @@ CXX = os.environ.get("CXX", "g++")
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wno-unused-result", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", os.path.join(ROOT, "include")]
 # The streaming kernels keep their row windows in registers only when the row loop is
-# unrolled completely; the 7-stage spinor kernels exceed LLVM's default pragma threshold.
+# unrolled completely; the 7-stage spinor kernels exceed LLVM's default pragma threshold, and so
+# does the FP64 RK4 kernel at 24 rows with 8 prefetched rows (4 KiB of scratch per lane without it).
 _UNROLL = ["-mllvm", "-pragma-unroll-threshold=2000000"]
 HIP_FLAGS_FILE = {f: _UNROLL for f in ("phx_rk_generic.hip", "phx_rk_tab3.hip", "phx_rk_tab4.hip", "phx_rk_tab6.hip",
-                                     "phx_rk_tab6f.hip", "phx_rk_dp45t.hip")}
+                                     "phx_rk_tab6f.hip", "phx_rk_dp45t.hip", "phx_rk4_f64.hip")}
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I", os.path.join(ROOT, "include")]
 
 # longest compiles first: the thread pool then finishes them side by side
