@@ -85,6 +85,9 @@ typedef struct {
                                    phx_run_interval run constant-source stretches that way, up to 500 steps
                                    per launch; PHX_RK4_FLOW=0: never) */
     int64_t multi_step_launches; /* its launches */
+    int32_t complex_reservoir;  /* 1: the reservoir fields are complex ({re, im} per cell: PHX_N_PLUS /
+                                   PHX_N_MINUS take 2 N_c N_r reals), as the reference's n becomes under
+                                   --imagTime with a reservoir or a complex temporal pump envelope */
 } phx_info;
 
 /* Parse PHOENIX flags (argv[0] is the program name and is skipped), including
@@ -174,7 +177,8 @@ int phx_get_output_config(phx_solver* s, phx_output_config* oc);
 int phx_apply_fft_filter(phx_solver* s);
 
 /* Copy a field to / from host memory (full grid, layout above).  bytes must equal
-   N_c*N_r*elements*sizeof(real).  Replaces the host mirror of CUDAMatrix
+   N_c*N_r*elements*sizeof(real), elements = 2 for psi, the spectra and (with
+   phx_info.complex_reservoir) the reservoir, 1 otherwise.  Replaces the host mirror of CUDAMatrix
    (cuda_matrix.cuh:281-306,375-434) used by the reference's output path. */
 int phx_get_field(phx_solver* s, int field, void* host, size_t bytes);
 int phx_set_field(phx_solver* s, int field, const void* host, size_t bytes);

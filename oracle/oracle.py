@@ -4,8 +4,8 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py as the parity checker.  The product package
 (phoenix_amd) never imports this module.
 
-Arrays use the product layout: full grid, row-major (N_r, N_c); psi as complex128,
-n / P / V as float64.
+Arrays use the product layout: full grid, row-major (N_r, N_c); psi as complex128, n as float64
+(complex128 when the parameters set complex_reservoir), P / V as float64.
 """
 from __future__ import annotations
 
@@ -42,6 +42,7 @@ class OrcParams(C.Structure):
         ("src_groups", C.c_int * 3),
         ("src_field", C.c_void_p * 2 * MAX_GROUPS * 3),
         ("src_amp", C.c_double * 2 * MAX_GROUPS * 3), ("src_amp_next", C.c_double * 2 * MAX_GROUPS * 3),
+        ("complex_reservoir", C.c_int),
     ]
 
 
@@ -86,6 +87,7 @@ def lib():
         L.orc_iter_attempt.argtypes = [P, cs, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double, C.c_int64, dp, dp]
         L.orc_adapt_dt_pow.argtypes = [C.c_double] * 7 + [dp, dp]
         L.orc_imag_normalize.argtypes = [P, dp, dp]
+        L.orc_imag_normalize_reservoir.argtypes = [P, dp, dp]
         L.orc_ssfm_run.argtypes = [P, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_int, C.c_int64]
         L.orc_iter_adaptive_run.argtypes = [P, cs, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double, C.c_double,
                                             C.c_double, C.c_double, C.c_int, C.POINTER(C.c_int64)]
@@ -166,16 +168,32 @@ def _f64(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float64).copy()
 
 
+def _res(p, n, shape, keep_none=False):
+    """A reservoir array in the layout the parameters ask for: complex128 with complex_reservoir,
+    float64 otherwise; None stays None with keep_none, else zeros."""
+    if n is None:
+        if keep_none:
+            return None
+        n = np.zeros(shape)
+    return _c128(n) if p.complex_reservoir else _f64(n)
+
+
+def _rdp(a):
+    if a is None:
+        return None
+    return _dp(a.view(np.float64)) if np.iscomplexobj(a) else _dp(a)
+
+
 def rk4_run(p: OrcParams, psi, n=None, pump=None, pot=None, dt=0.04, steps=1, psi_minus=None, n_minus=None,
             pump_minus=None, pot_minus=None, step0=0):
     psi = _c128(psi)
     pm = _c128(psi_minus) if psi_minus is not None else None
-    n = _f64(n) if n is not None else np.zeros(psi.shape)
-    nm = _f64(n_minus) if n_minus is not None else (np.zeros(psi.shape) if psi_minus is not None else None)
+    n = _res(p, n, psi.shape)
+    nm = _res(p, n_minus, psi.shape) if (n_minus is not None or psi_minus is not None) else None
     pump = _f64(pump)
     pot = _f64(pot)
     rc = lib().orc_rk4_run(C.byref(p), _dp(psi.view(np.float64)), _dp(pm.view(np.float64)) if pm is not None else None,
-                           _dp(n), _dp(nm), _dp(pump), _dp(_f64(pump_minus)), _dp(pot), _dp(_f64(pot_minus)),
+                           _rdp(n), _rdp(nm), _dp(pump), _dp(_f64(pump_minus)), _dp(pot), _dp(_f64(pot_minus)),
                            dt, steps, step0)
     if rc:
         raise RuntimeError(f"orc_rk4_run failed: {rc}")
@@ -205,12 +223,12 @@ def rk4_adjoint(p: OrcParams, psi0, n0, pump, pot, dt, steps, lam_psi_T, lam_n_T
 def dp45_attempt(p, psi, n, pump=None, pot=None, dt=0.04, noise_dt=None, step=0, psi_minus=None, n_minus=None,
                  pump_minus=None, pot_minus=None):
     psi = _c128(psi)
-    n = _f64(n)
+    n = _res(p, n, psi.shape, keep_none=True)
     pm = _c128(psi_minus) if psi_minus is not None else None
-    nm = _f64(n_minus)
+    nm = _res(p, n_minus, psi.shape, keep_none=True)
     es, nr = C.c_double(), C.c_double()
     rc = lib().orc_dp45_attempt(C.byref(p), _dp(psi.view(np.float64)), _dp(pm.view(np.float64)) if pm is not None else None,
-                                _dp(n), _dp(nm), _dp(_f64(pump)), _dp(_f64(pump_minus)), _dp(_f64(pot)),
+                                _rdp(n), _rdp(nm), _dp(_f64(pump)), _dp(_f64(pump_minus)), _dp(_f64(pot)),
                                 _dp(_f64(pot_minus)), dt, dt if noise_dt is None else noise_dt, step,
                                 C.byref(es), C.byref(nr))
     if rc:
@@ -230,13 +248,13 @@ def adapt_dt(err_sum, norm, tol, dt, dt_min, dt_max):
 def dp45_run(p, psi, n, pump=None, pot=None, t=0.0, dt=0.04, t_max=1.0, tol=1e-6, dt_min=1e-4, dt_max=0.1,
              max_rejections=1000, psi_minus=None, n_minus=None, pump_minus=None, pot_minus=None):
     psi = _c128(psi)
-    n = _f64(n)
+    n = _res(p, n, psi.shape, keep_none=True)
     pm = _c128(psi_minus) if psi_minus is not None else None
-    nm = _f64(n_minus)
+    nm = _res(p, n_minus, psi.shape, keep_none=True)
     tt, dd = C.c_double(t), C.c_double(dt)
     nrej = C.c_int64()
     acc = lib().orc_dp45_run(C.byref(p), _dp(psi.view(np.float64)), _dp(pm.view(np.float64)) if pm is not None else None,
-                             _dp(n), _dp(nm), _dp(_f64(pump)), _dp(_f64(pump_minus)), _dp(_f64(pot)),
+                             _rdp(n), _rdp(nm), _dp(_f64(pump)), _dp(_f64(pump_minus)), _dp(_f64(pot)),
                              _dp(_f64(pot_minus)), C.byref(tt), C.byref(dd), t_max, tol, dt_min, dt_max,
                              max_rejections, C.byref(nrej))
     if acc < 0:
@@ -265,10 +283,10 @@ def iter_run(p, key, psi, n=None, pump=None, pot=None, dt=0.04, steps=1, psi_min
     """`steps` fixed steps of the iterator `key` (orc_iter_run); returns as rk4_run."""
     psi = _c128(psi)
     pm = _c128(psi_minus) if psi_minus is not None else None
-    n = _f64(n) if n is not None else np.zeros(psi.shape)
-    nm = _f64(n_minus) if n_minus is not None else (np.zeros(psi.shape) if psi_minus is not None else None)
+    n = _res(p, n, psi.shape)
+    nm = _res(p, n_minus, psi.shape) if (n_minus is not None or psi_minus is not None) else None
     rc = lib().orc_iter_run(C.byref(p), key.encode(), _dp(psi.view(np.float64)),
-                            _dp(pm.view(np.float64)) if pm is not None else None, _dp(n), _dp(nm), _dp(_f64(pump)),
+                            _dp(pm.view(np.float64)) if pm is not None else None, _rdp(n), _rdp(nm), _dp(_f64(pump)),
                             _dp(_f64(pump_minus)), _dp(_f64(pot)), _dp(_f64(pot_minus)), dt, steps, step0)
     if rc:
         raise RuntimeError(f"orc_iter_run({key}) failed: {rc}")
@@ -281,12 +299,12 @@ def iter_attempt(p, key, psi, n=None, pump=None, pot=None, dt=0.04, noise_dt=Non
                  pump_minus=None, pot_minus=None):
     """One adaptive attempt of `key` (orc_iter_attempt); returns as dp45_attempt."""
     psi = _c128(psi)
-    n = _f64(n) if n is not None else np.zeros(psi.shape)
+    n = _res(p, n, psi.shape)
     pm = _c128(psi_minus) if psi_minus is not None else None
-    nm = _f64(n_minus) if n_minus is not None else (np.zeros(psi.shape) if psi_minus is not None else None)
+    nm = _res(p, n_minus, psi.shape) if (n_minus is not None or psi_minus is not None) else None
     es, nr = C.c_double(), C.c_double()
     rc = lib().orc_iter_attempt(C.byref(p), key.encode(), _dp(psi.view(np.float64)),
-                                _dp(pm.view(np.float64)) if pm is not None else None, _dp(n), _dp(nm), _dp(_f64(pump)),
+                                _dp(pm.view(np.float64)) if pm is not None else None, _rdp(n), _rdp(nm), _dp(_f64(pump)),
                                 _dp(_f64(pump_minus)), _dp(_f64(pot)), _dp(_f64(pot_minus)), dt,
                                 dt if noise_dt is None else noise_dt, step, C.byref(es), C.byref(nr))
     if rc:
@@ -307,13 +325,13 @@ def iter_adaptive_run(p, key, psi, n, pump=None, pot=None, t=0.0, dt=0.04, t_max
                       max_rejections=1000, psi_minus=None, n_minus=None, pump_minus=None, pot_minus=None):
     """Adaptive run of `key` until t >= t_max (orc_iter_adaptive_run); returns as dp45_run."""
     psi = _c128(psi)
-    n = _f64(n)
+    n = _res(p, n, psi.shape, keep_none=True)
     pm = _c128(psi_minus) if psi_minus is not None else None
-    nm = _f64(n_minus)
+    nm = _res(p, n_minus, psi.shape, keep_none=True)
     tt, dd = C.c_double(t), C.c_double(dt)
     nrej = C.c_int64()
     acc = lib().orc_iter_adaptive_run(C.byref(p), key.encode(), _dp(psi.view(np.float64)),
-                                      _dp(pm.view(np.float64)) if pm is not None else None, _dp(n), _dp(nm),
+                                      _dp(pm.view(np.float64)) if pm is not None else None, _rdp(n), _rdp(nm),
                                       _dp(_f64(pump)), _dp(_f64(pump_minus)), _dp(_f64(pot)), _dp(_f64(pot_minus)),
                                       C.byref(tt), C.byref(dd), t_max, tol, dt_min, dt_max, max_rejections,
                                       C.byref(nrej))
@@ -330,10 +348,10 @@ def ssfm_run(p, psi, n=None, pump=None, pot=None, dt=0.04, steps=1, psi_minus=No
     """`steps` split-step Fourier steps (orc_ssfm_run); returns as rk4_run."""
     psi = _c128(psi)
     pm = _c128(psi_minus) if psi_minus is not None else None
-    n = _f64(n) if n is not None else np.zeros(psi.shape)
-    nm = _f64(n_minus) if n_minus is not None else (np.zeros(psi.shape) if psi_minus is not None else None)
+    n = _res(p, n, psi.shape)
+    nm = _res(p, n_minus, psi.shape) if (n_minus is not None or psi_minus is not None) else None
     rc = lib().orc_ssfm_run(C.byref(p), _dp(psi.view(np.float64)), _dp(pm.view(np.float64)) if pm is not None else None,
-                            _dp(n), _dp(nm), _dp(_f64(pump)), _dp(_f64(pump_minus)), _dp(_f64(pot)), _dp(_f64(pot_minus)),
+                            _rdp(n), _rdp(nm), _dp(_f64(pump)), _dp(_f64(pump_minus)), _dp(_f64(pot)), _dp(_f64(pot_minus)),
                             dt, steps, step0)
     if rc:
         raise RuntimeError(f"orc_ssfm_run failed: {rc}")

@@ -20,6 +20,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <float.h>
+#include <complex.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -128,8 +129,9 @@ void orc_gaussian_field(double* psi, int N, double amplitude, uint64_t seed) {
 typedef struct {
     const orc_params* p;
     int h, sgc, sgr, ro, N2h, S, kmax, ncomp;
+    int nel;                                /* doubles per reservoir cell: 1, or 2 (complex reservoir) */
     double *psi[2], *bpsi[2], *kpsi[2];     /* complex: 2 doubles per cell */
-    double *n[2], *bn[2], *kn[2];           /* real */
+    double *n[2], *bn[2], *kn[2];           /* real, or complex with complex_reservoir */
     double *pump[2], *pot[2];               /* real */
     double *xi;                             /* complex noise per padded cell, per subgrid */
     double *src[3][ORC_MAX_GROUPS][2];      /* source chains per group and component (padded) */
@@ -152,14 +154,17 @@ static int ws_init(ws_t* w, const orc_params* p, int kmax) {
     w->N2h = w->ro * (w->sgr + 2 * w->h);
     w->S = p->subgrids_columns * p->subgrids_rows;
     w->ncomp = p->tetm ? 2 : 1;
+    /* imaginary time advances the reservoir by -i dt: it needs the complex reservoir */
+    if (p->imag_time_amplitude != 0.0 && p->use_reservoir && !p->complex_reservoir) return -1;
+    w->nel = p->complex_reservoir ? 2 : 1;
     size_t cells = (size_t)w->S * w->N2h;
     for (int c = 0; c < w->ncomp; c++) {
         w->psi[c] = xcalloc(2 * cells, sizeof(double));
         w->bpsi[c] = xcalloc(2 * cells, sizeof(double));
         w->kpsi[c] = xcalloc(2 * cells * kmax, sizeof(double));
-        w->n[c] = xcalloc(cells, sizeof(double));
-        w->bn[c] = xcalloc(cells, sizeof(double));
-        w->kn[c] = xcalloc(cells * kmax, sizeof(double));
+        w->n[c] = xcalloc(w->nel * cells, sizeof(double));
+        w->bn[c] = xcalloc(w->nel * cells, sizeof(double));
+        w->kn[c] = xcalloc(w->nel * cells * kmax, sizeof(double));
         w->pump[c] = xcalloc(cells, sizeof(double));
         w->pot[c] = xcalloc(cells, sizeof(double));
     }
@@ -309,6 +314,48 @@ static double pump_term(const ws_t* w, int comp, size_t i, double wt, double pam
     return kn;
 }
 
+/* complex-reservoir pump term: k_n += sum_g P_g a_g(w) with the full complex amplitude
+   (kernel_gp_compute.cuh:70-75; the single-field form has a real amplitude) */
+static void pump_term_c(const ws_t* w, int comp, size_t i, double wt, double pamp, double* knr, double* kni) {
+    const orc_params* p = w->p;
+    if (p->src_groups[0] > 0) {
+        for (int g = 0; g < p->src_groups[0]; g++) {
+            double ar, ai;
+            src_amp(p, 0, g, wt, &ar, &ai);
+            const double v = w->src[0][g][comp][i];
+            *knr += v * ar;
+            *kni += v * ai;
+        }
+        return;
+    }
+    if (p->use_pump) *knr += w->pump[comp][i] * pamp;
+}
+
+/* complex-reservoir terms of one component: k_psi += (1/hbar) g_r n (-i psi) + 1/2 R n psi,
+   k_n = -(gamma_r + R |psi|^2) n + pump (+ R n / dV with noise), n = (nr, ni) complex
+   (kernel_gp_compute.cuh:66-82,166-181) */
+static void cres_terms(const ws_t* w, int comp, size_t i, size_t ki, double wt, double pamp, int stoch, double nr, double ni,
+                       double pr, double pi, double nrm, double* wr, double* wi) {
+    const orc_params* p = w->p;
+    const double ohb = p->one_over_h_bar_s;
+    const double mr = MI_RE(pr, pi), mi = MI_IM(pr, pi);
+    const double gr = ohb * p->g_r;
+    *wr += (gr * nr) * mr - (gr * ni) * mi;
+    *wi += (gr * nr) * mi + (gr * ni) * mr;
+    const double hr = 0.5 * p->R;
+    *wr += (hr * nr) * pr - (hr * ni) * pi;
+    *wi += (hr * nr) * pi + (hr * ni) * pr;
+    const double d = -(p->gamma_r + p->R * nrm);
+    double knr = d * nr, kni = d * ni;
+    pump_term_c(w, comp, i, wt, pamp, &knr, &kni);
+    if (stoch) {
+        knr += p->R * nr / p->dV;
+        kni += p->R * ni / p->dV;
+    }
+    w->kn[comp][2 * ki] = knr;
+    w->kn[comp][2 * ki + 1] = kni;
+}
+
 static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
     const orc_params* p = w->p;
     const int ch = w->h - index;
@@ -339,7 +386,9 @@ static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
                     wi += ohb * p->g_c * nrm * mi;
                     wr -= 0.5 * p->gamma_c * pr;
                     wi -= 0.5 * p->gamma_c * pi;
-                    if (p->use_reservoir) {
+                    if (p->use_reservoir && w->nel == 2) {
+                        cres_terms(w, 0, i, ki, wt, pamp, stoch, inn[2 * i], inn[2 * i + 1], pr, pi, nrm, &wr, &wi);
+                    } else if (p->use_reservoir) {
                         const double nv = inn[i];
                         wr += ohb * p->g_r * nv * mr;
                         wi += ohb * p->g_r * nv * mi;
@@ -387,7 +436,8 @@ static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
                     const double ppr = P[2 * i], ppi = P[2 * i + 1], pmr = M[2 * i], pmi = M[2 * i + 1];
                     double regp_r = p->m2_over_dx2_p_dy2 * ppr + horp_r + verp_r, regp_i = p->m2_over_dx2_p_dy2 * ppi + horp_i + verp_i;
                     double regm_r = p->m2_over_dx2_p_dy2 * pmr + horm_r + verm_r, regm_i = p->m2_over_dx2_p_dy2 * pmi + horm_i + verm_i;
-                    const double np_ = NP[i], nm_ = NM[i];
+                    const int cres = w->nel == 2;
+                    const double np_ = cres ? 0.0 : NP[i], nm_ = cres ? 0.0 : NM[i];
                     const double nrmp = ppr * ppr + ppi * ppi, nrmm = pmr * pmr + pmi * pmi;
                     for (int comp = 0; comp < 2; comp++) {
                         const double sr = comp ? pmr : ppr, si = comp ? pmi : ppi;
@@ -406,8 +456,13 @@ static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
                             res_i += ohb * v * mi;
                         }
                         res_r += ohb * p->g_c * nself * mr;  res_i += ohb * p->g_c * nself * mi;
-                        res_r += ohb * p->g_r * nv * mr;     res_i += ohb * p->g_r * nv * mi;
-                        res_r += 0.5 * p->R * nv * sr;       res_i += 0.5 * p->R * nv * si;
+                        if (!cres) {
+                            res_r += ohb * p->g_r * nv * mr;     res_i += ohb * p->g_r * nv * mi;
+                            res_r += 0.5 * p->R * nv * sr;       res_i += 0.5 * p->R * nv * si;
+                        } else {
+                            const double* NC = comp ? NM : NP;
+                            cres_terms(w, comp, i, ki, wt, pamp, stoch, NC[2 * i], NC[2 * i + 1], sr, si, nself, &res_r, &res_i);
+                        }
                         res_r -= 0.5 * p->gamma_c * sr;      res_i -= 0.5 * p->gamma_c * si;
                         res_r += ohb * p->g_pm * nother * mr; res_i += ohb * p->g_pm * nother * mi;
                         res_r += ohb * p->delta_LT * MI_RE(cr, ci);
@@ -419,10 +474,12 @@ static void calc_k(ws_t* w, int index, double wt, int from_buffer) {
                         }
                         w->kpsi[comp][2 * ki] = res_r;
                         w->kpsi[comp][2 * ki + 1] = res_i;
-                        double kn = -(p->gamma_r + p->R * nself) * nv;
-                        kn = pump_term(w, comp, i, wt, pamp, kn);
-                        if (stoch) kn += p->R * nv / p->dV;
-                        w->kn[comp][ki] = kn;
+                        if (!cres) {
+                            double kn = -(p->gamma_r + p->R * nself) * nv;
+                            kn = pump_term(w, comp, i, wt, pamp, kn);
+                            if (stoch) kn += p->R * nv / p->dV;
+                            w->kn[comp][ki] = kn;
+                        }
                     }
                 }
             }
@@ -453,6 +510,15 @@ static inline void ksum(const ws_t* w, const double* kf, int elem, size_t off, i
 static void add_dw(const ws_t* w, size_t i, double wsum, double* out_r, double* out_i) {
     const orc_params* p = w->p;
     if (!(p->use_stochastic && p->stochastic_amplitude > 0.0)) return;
+    if (p->use_reservoir && w->nel == 2) {
+        /* complex reservoir: principal square root of (R n + gamma_c) / (4 dV), times w_sum xi */
+        const double complex z = ((p->R * w->n[0][2 * i] + p->gamma_c) + I * (p->R * w->n[0][2 * i + 1])) / (4.0 * p->dV);
+        const double complex sq = csqrt(z);
+        const double xr = wsum * w->xi[2 * i], xi = wsum * w->xi[2 * i + 1];
+        *out_r += xr * creal(sq) - xi * cimag(sq);
+        *out_i += xr * cimag(sq) + xi * creal(sq);
+        return;
+    }
     double s = p->use_reservoir ? sqrt((p->R * w->n[0][i] + p->gamma_c) / (4.0 * p->dV)) : sqrt(p->gamma_c / (4.0 * p->dV));
     *out_r += wsum * w->xi[2 * i] * s;
     *out_i += wsum * w->xi[2 * i + 1] * s;
@@ -482,7 +548,13 @@ static void sum_to_buffer(ws_t* w, int index, int nw, const double* wts) {
                     add_dw(w, i, wsum, &orr, &oi);
                     w->bpsi[comp][2 * i] = orr;
                     w->bpsi[comp][2 * i + 1] = oi;
-                    if (w->p->use_reservoir) {
+                    if (w->p->use_reservoir && w->nel == 2) {
+                        /* complex reservoir: input + dt sum, or + (0, -dt) sum in imaginary time */
+                        ksum(w, w->kn[comp], 2, i, nw, slot, wts, s);
+                        const double* x = w->n[comp];
+                        w->bn[comp][2 * i] = imag ? x[2 * i] + dt * s[1] : x[2 * i] + dt * s[0];
+                        w->bn[comp][2 * i + 1] = imag ? x[2 * i + 1] - dt * s[0] : x[2 * i + 1] + dt * s[1];
+                    } else if (w->p->use_reservoir) {
                         ksum(w, w->kn[comp], 1, i, nw, slot, wts, s);
                         w->bn[comp][i] = w->n[comp][i] + dt * s[0];
                     }
@@ -524,6 +596,17 @@ static void final_sum(ws_t* w, int index, int nw, const double* wts) {
                 if (w->p->use_reservoir)
                     for (int comp = 0; comp < w->ncomp; comp++) {
                         double s[2];
+                        if (w->nel == 2) {
+                            ksum(w, w->kn[comp], 2, i, nw, slot, wts, s);
+                            if (imag) {
+                                w->n[comp][2 * i] += dt * s[1];
+                                w->n[comp][2 * i + 1] -= dt * s[0];
+                            } else {
+                                w->n[comp][2 * i] += dt * s[0];
+                                w->n[comp][2 * i + 1] += dt * s[1];
+                            }
+                            continue;
+                        }
                         ksum(w, w->kn[comp], 1, i, nw, slot, wts, s);
                         w->n[comp][i] += dt * s[0];
                     }
@@ -533,7 +616,7 @@ static void final_sum(ws_t* w, int index, int nw, const double* wts) {
 static void sync_state(ws_t* w) {
     for (int c = 0; c < w->ncomp; c++) {
         sync_halos(w, w->psi[c], 2);
-        if (w->p->use_reservoir) sync_halos(w, w->n[c], 1);
+        if (w->p->use_reservoir) sync_halos(w, w->n[c], w->nel);
     }
 }
 
@@ -559,7 +642,7 @@ static void load_state(ws_t* w, double* psi_plus, double* psi_minus, double* n_p
                        const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus) {
     const orc_params* p = w->p;
     to_sub(w, w->psi[0], psi_plus, 2);
-    if (p->use_reservoir) to_sub(w, w->n[0], n_plus, 1);
+    if (p->use_reservoir) to_sub(w, w->n[0], n_plus, w->nel);
     /* (a source given as a chain has no single field: NULL) */
     if (p->use_pump && pump_plus) { to_sub(w, w->pump[0], pump_plus, 1); sync_halos(w, w->pump[0], 1); }
     if (p->use_potential && pot_plus) { to_sub(w, w->pot[0], pot_plus, 1); sync_halos(w, w->pot[0], 1); }
@@ -573,7 +656,7 @@ static void load_state(ws_t* w, double* psi_plus, double* psi_minus, double* n_p
             }
     if (p->tetm) {
         to_sub(w, w->psi[1], psi_minus, 2);
-        if (p->use_reservoir) to_sub(w, w->n[1], n_minus, 1);
+        if (p->use_reservoir) to_sub(w, w->n[1], n_minus, w->nel);
         if (p->use_pump && pump_minus) { to_sub(w, w->pump[1], pump_minus, 1); sync_halos(w, w->pump[1], 1); }
         if (p->use_potential && pot_minus) { to_sub(w, w->pot[1], pot_minus, 1); sync_halos(w, w->pot[1], 1); }
     }
@@ -582,10 +665,10 @@ static void load_state(ws_t* w, double* psi_plus, double* psi_minus, double* n_p
 static void store_state(ws_t* w, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus, int from_buffer) {
     const orc_params* p = w->p;
     to_full(w, from_buffer ? w->bpsi[0] : w->psi[0], psi_plus, 2);
-    if (p->use_reservoir) to_full(w, from_buffer ? w->bn[0] : w->n[0], n_plus, 1);
+    if (p->use_reservoir) to_full(w, from_buffer ? w->bn[0] : w->n[0], n_plus, w->nel);
     if (p->tetm) {
         to_full(w, from_buffer ? w->bpsi[1] : w->psi[1], psi_minus, 2);
-        if (p->use_reservoir) to_full(w, from_buffer ? w->bn[1] : w->n[1], n_minus, 1);
+        if (p->use_reservoir) to_full(w, from_buffer ? w->bn[1] : w->n[1], n_minus, w->nel);
     }
 }
 
@@ -706,7 +789,7 @@ int64_t orc_dp45_run(const orc_params* p, double* psi_plus, double* psi_minus, d
     double *tp[2] = {0, 0}, *tn[2] = {0, 0};
     for (int c = 0; c < nc; c++) {
         tp[c] = malloc(2 * N * sizeof(double));
-        tn[c] = malloc(N * sizeof(double));
+        tn[c] = malloc((p->complex_reservoir ? 2 : 1) * N * sizeof(double));
     }
     int64_t accepted = 0, rejected = 0;
     int64_t step = 0;
@@ -717,7 +800,7 @@ int64_t orc_dp45_run(const orc_params* p, double* psi_plus, double* psi_minus, d
         for (;;) {
             for (int c = 0; c < nc; c++) {
                 memcpy(tp[c], ps[c], 2 * N * sizeof(double));
-                if (p->use_reservoir) memcpy(tn[c], ns[c], N * sizeof(double));
+                if (p->use_reservoir) memcpy(tn[c], ns[c], (p->complex_reservoir ? 2 : 1) * N * sizeof(double));
             }
             double es, nm, nd, fe;
             if (orc_dp45_attempt(p, tp[0], tp[1], tn[0], tn[1], pump_plus, pump_minus, pot_plus, pot_minus, *dt, noise_dt, step, &es, &nm)) return -1;
@@ -733,7 +816,7 @@ int64_t orc_dp45_run(const orc_params* p, double* psi_plus, double* psi_minus, d
         }
         for (int c = 0; c < nc; c++) {
             memcpy(ps[c], tp[c], 2 * N * sizeof(double));
-            if (p->use_reservoir) memcpy(ns[c], tn[c], N * sizeof(double));
+            if (p->use_reservoir) memcpy(ns[c], tn[c], (p->complex_reservoir ? 2 : 1) * N * sizeof(double));
         }
         accepted++;
         step++;
@@ -1042,7 +1125,15 @@ static void error_sum(const ws_t* w, int nw, const double* wts, double* err_sum,
    psi_k *= sqrt(amp / (S_k dV)), S_k = sum |psi_k|^2 (< 1e-10 -> 1).  The reference sums its padded
    subgrids including halo cells; this sums each grid cell once.  Without a reservoir the reservoir
    normalisation of the reference scales zeros. */
+static void ws_imag_normalize_field(ws_t* w, double* const* f, int el);
+
 static void ws_imag_normalize(ws_t* w) {
+    ws_imag_normalize_field(w, w->psi, 2);
+    if (w->p->use_reservoir) ws_imag_normalize_field(w, w->n, w->nel);
+}
+
+/* f[comp] *= sqrt(amp / (sum |f[comp]|^2 dV)) over the interior cells (el doubles per cell) */
+static void ws_imag_normalize_field(ws_t* w, double* const* f, int el) {
     const orc_params* p = w->p;
     for (int comp = 0; comp < w->ncomp; comp++) {
         double sum = 0.0;
@@ -1050,16 +1141,15 @@ static void ws_imag_normalize(ws_t* w) {
             for (int r = w->h; r < w->h + w->sgr; r++)
                 for (int c = w->h; c < w->h + w->sgc; c++) {
                     size_t i = (size_t)g * w->N2h + (size_t)r * w->ro + c;
-                    sum += w->psi[comp][2 * i] * w->psi[comp][2 * i] + w->psi[comp][2 * i + 1] * w->psi[comp][2 * i + 1];
+                    for (int e = 0; e < el; e++) sum += f[comp][el * i + e] * f[comp][el * i + e];
                 }
         if (sum < 1e-10) sum = 1.0;
-        const double f = sqrt(p->imag_time_amplitude / (sum * p->dV));
+        const double sc = sqrt(p->imag_time_amplitude / (sum * p->dV));
         for (int g = 0; g < w->S; g++)
             for (int r = w->h; r < w->h + w->sgr; r++)
                 for (int c = w->h; c < w->h + w->sgc; c++) {
                     size_t i = (size_t)g * w->N2h + (size_t)r * w->ro + c;
-                    w->psi[comp][2 * i] *= f;
-                    w->psi[comp][2 * i + 1] *= f;
+                    for (int e = 0; e < el; e++) f[comp][el * i + e] *= sc;
                 }
     }
 }
@@ -1074,6 +1164,20 @@ void orc_imag_normalize(const orc_params* p, double* psi_plus, double* psi_minus
         if (sum < 1e-10) sum = 1.0;
         const double s = sqrt(p->imag_time_amplitude / (sum * p->dV));
         for (size_t i = 0; i < 2 * N; i++) f[comp][i] *= s;
+    }
+}
+
+void orc_imag_normalize_reservoir(const orc_params* p, double* n_plus, double* n_minus) {
+    const size_t N = (size_t)p->N_c * p->N_r;
+    const int el = p->complex_reservoir ? 2 : 1;
+    double* f[2] = {n_plus, n_minus};
+    for (int comp = 0; comp < 2; comp++) {
+        if (!f[comp]) continue;
+        double sum = 0.0;
+        for (size_t i = 0; i < el * N; i++) sum += f[comp][i] * f[comp][i];
+        if (sum < 1e-10) sum = 1.0;
+        const double s = sqrt(p->imag_time_amplitude / (sum * p->dV));
+        for (size_t i = 0; i < el * N; i++) f[comp][i] *= s;
     }
 }
 
@@ -1169,7 +1273,7 @@ int64_t orc_iter_adaptive_run(const orc_params* p, const char* key, double* psi_
     double *tp[2] = {0, 0}, *tn[2] = {0, 0};
     for (int c = 0; c < nc; c++) {
         tp[c] = malloc(2 * N * sizeof(double));
-        tn[c] = malloc(N * sizeof(double));
+        tn[c] = malloc((p->complex_reservoir ? 2 : 1) * N * sizeof(double));
     }
     int64_t accepted = 0, rejected = 0, step = 0;
     int64_t rc = 0;
@@ -1180,7 +1284,7 @@ int64_t orc_iter_adaptive_run(const orc_params* p, const char* key, double* psi_
         for (;;) {
             for (int c = 0; c < nc; c++) {
                 memcpy(tp[c], ps[c], 2 * N * sizeof(double));
-                if (p->use_reservoir) memcpy(tn[c], ns[c], N * sizeof(double));
+                if (p->use_reservoir) memcpy(tn[c], ns[c], (p->complex_reservoir ? 2 : 1) * N * sizeof(double));
             }
             double es, nm, nd, fe;
             if (orc_iter_attempt(p, key, tp[0], tp[1], tn[0], tn[1], pump_plus, pump_minus, pot_plus, pot_minus, *dt, noise_dt, step,
@@ -1199,9 +1303,12 @@ int64_t orc_iter_adaptive_run(const orc_params* p, const char* key, double* psi_
         }
         for (int c = 0; c < nc; c++) {
             memcpy(ps[c], tp[c], 2 * N * sizeof(double));
-            if (p->use_reservoir) memcpy(ns[c], tn[c], N * sizeof(double));
+            if (p->use_reservoir) memcpy(ns[c], tn[c], (p->complex_reservoir ? 2 : 1) * N * sizeof(double));
         }
-        if (p->imag_time_amplitude != 0.0) orc_imag_normalize(p, ps[0], nc > 1 ? ps[1] : NULL);
+        if (p->imag_time_amplitude != 0.0) {
+            orc_imag_normalize(p, ps[0], nc > 1 ? ps[1] : NULL);
+            if (p->use_reservoir) orc_imag_normalize_reservoir(p, ns[0], nc > 1 ? ns[1] : NULL);
+        }
         accepted++;
         step++;
     }
@@ -1311,6 +1418,7 @@ static int ssfm_linear(const orc_params* p, double* f, double dt) {
 int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
                  const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
                  double dt, int steps, int64_t step0) {
+    if (p->complex_reservoir) return -4; /* the split-step iterator keeps a real reservoir here */
     const size_t N = (size_t)p->N_c * p->N_r;
     const int nc = p->tetm ? 2 : 1;
     double* psi[2] = {psi_plus, psi_minus};

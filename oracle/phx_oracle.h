@@ -62,6 +62,14 @@ typedef struct {
     int src_groups[3];
     const double* src_field[3][ORC_MAX_GROUPS][2];
     double src_amp[3][ORC_MAX_GROUPS][2], src_amp_next[3][ORC_MAX_GROUPS][2];
+    /* Complex reservoir (the reference stores n as Type::complex, cuda_matrix.cuh): != 0 makes the
+       n arrays of every call complex ({re, im} per cell).  n then takes the pump chain's full complex
+       amplitude sum_g P_g a_g(w) (kernel_gp_compute.cuh:70-75), advances by -i dt in imaginary time
+       like psi (runge_sum_to_input_k with complex_dt, cuda_macro.cuh:160-178), enters k_psi as
+       complex products, sets the noise amplitude through a complex square root
+       (kernel_summation.cuh:209-225), and is normalised in imaginary time
+       (solver_normalize_imag_time.cu:13-45).  Imaginary time with a reservoir requires it. */
+    int complex_reservoir;
 } orc_params;
 
 /* Fill derived constants exactly as SystemParameters::calculateAuto does.
@@ -134,6 +142,8 @@ int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, doubl
    |psi_k|^2 over the grid (S_k < 1e-10 -> 1); psi_minus may be NULL.
    (solver_normalize_imag_time.cu:13-45) */
 void orc_imag_normalize(const orc_params* p, double* psi_plus, double* psi_minus);
+/* The same normalisation of the (complex) reservoir fields n_plus / n_minus (NULL: skipped). */
+void orc_imag_normalize_reservoir(const orc_params* p, double* n_plus, double* n_minus);
 int orc_adapt_dt_pow(double err_sum, double norm, double tolerance, double dt, double dt_min, double dt_max, double power,
                      double* new_dt, double* final_error);
 int64_t orc_iter_adaptive_run(const orc_params* p, const char* key, double* psi_plus, double* psi_minus, double* n_plus,

@@ -66,6 +66,7 @@ class PhxInfo(C.Structure):
         ("fft_every", C.c_double), ("tolerance", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
         ("iteration", C.c_int64), ("rejected", C.c_int64), ("tile_rows", C.c_int32), ("prefetch_rows", C.c_int32),
         ("staged", C.c_int32), ("multi_step_steps", C.c_int64), ("multi_step_launches", C.c_int64),
+        ("complex_reservoir", C.c_int32),
     ]
 
     def as_dict(self):
@@ -208,15 +209,20 @@ class Solver:
         i = self.info
         return i["N_r"], i["N_c"]
 
+    def _dtype(self, fid: int):
+        """Host dtype of a field: complex for psi and the spectra, and for the reservoir when it is complex"""
+        cplx = fid in COMPLEX_FIELDS or (fid in (2, 3) and self.info["complex_reservoir"])
+        return self.cplx if cplx else self.real
+
     def get(self, field: str) -> np.ndarray:
         fid = FIELDS[field]
-        out = np.empty(self._shape(), self.cplx if fid in COMPLEX_FIELDS else self.real)
+        out = np.empty(self._shape(), self._dtype(fid))
         check(lib().phx_get_field(self._h, fid, out.ctypes.data, out.nbytes))
         return out
 
     def set(self, field: str, value: np.ndarray):
         fid = FIELDS[field]
-        v = np.ascontiguousarray(value, self.cplx if fid in COMPLEX_FIELDS else self.real)
+        v = np.ascontiguousarray(value, self._dtype(fid))
         if v.shape != self._shape():
             raise ValueError(f"{field}: expected shape {self._shape()}, got {v.shape}")
         check(lib().phx_set_field(self._h, fid, v.ctypes.data, v.nbytes))

@@ -609,9 +609,9 @@ std::string finalize_config(SystemConfig& c) {
         if (c.adaptive && !found->can_adapt) c.adaptive = false;
         c.halo = phx::rk_tableau_stages(c.adaptive ? found->adaptive : found->fixed);
     }
-    /* imaginary time turns the reference's (complex-stored) reservoir complex; the reservoir here is real */
-    if (c.imag_time_amplitude != 0.0 && c.use_reservoir)
-        return "--imagTime with a reservoir (pump or initial reservoir) is not provided by this library";
+    /* imaginary time turns the reference's (complex-stored) reservoir complex: n advances by -i dt
+       (runge_sum_to_input_k with complex_dt, cuda_macro.cuh:160-178) */
+    c.complex_reservoir = c.imag_time_amplitude != 0.0 && c.use_reservoir;
     /* calculateAuto (system.cpp:116-160) */
     if (c.h_bar_s < 0) c.h_bar_s = c.h_bar / c.e_e * 1E12;
     if (c.m_eff < 0) c.m_eff = c.m_e / c.e_e * 1E8;
@@ -641,10 +641,14 @@ std::string finalize_config(SystemConfig& c) {
     if (c.dt_max < 0) bad += "dt_max cannot be negative; ";
     if (c.dt_min < 0) bad += "dt_min cannot be negative; ";
     if (!bad.empty()) return "Invalid input! " + bad;
-    /* the pump enters a real reservoir: complex temporal envelopes on the pump need a complex n */
-    if (c.use_pumps)
+    /* a complex temporal pump amplitude makes n complex (k_n += P_g a_g(w), kernel_gp_compute.cuh:70-75) */
+    if (c.use_pumps && c.use_reservoir)
         for (int g = 0; g < c.pump.groupSize(); g++)
-            if (c.pump.groups[g].kind & TMP_IEXP) return "complex (iexp) temporal pump envelopes are not provided by this library";
+            if (c.pump.groups[g].kind & TMP_IEXP) c.complex_reservoir = true;
+    /* the split-step iterator here keeps a real reservoir */
+    if (c.complex_reservoir && c.iterator == "SSFM")
+        return "the split-step iterator with a complex reservoir (--imagTime or a complex temporal pump envelope with a "
+               "reservoir) is not provided by this library";
     return "";
 }
 

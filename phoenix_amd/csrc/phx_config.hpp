@@ -81,6 +81,10 @@ struct SystemConfig {
     Envelope pump, potential, pulse, fft_mask, initial_state, initial_reservoir;
     bool use_reservoir = false, use_pumps = false, use_pulses = false, use_potentials = false, use_fft_mask = false,
          use_stochastic = false;
+    /* complex reservoir: n held as {re, im} per cell.  The reference stores n complex throughout; it
+       becomes complex in imaginary time (n advances by -i dt like psi) and under a complex temporal
+       pump amplitude.  Set by finalize_config for those cases; served by the staged path. */
+    bool complex_reservoir = false;
     std::string path = "data/";
     std::vector<std::string> output_keys = {"mat", "scalar"};
     /* --historyMatrix x0 x1 y0 y1 increment, --historyTime start every, --fileoutPrecision

@@ -203,7 +203,8 @@ enum class StagedOp { K, SUM, FINAL };
 template <typename real>
 struct StagedArgs {
     using v2 = typename Vec2<real>::type;
-    /* step-start state and output (full grid, row-major); x_n[0] == nullptr: no reservoir */
+    /* step-start state and output (full grid, row-major); x_n[0] == nullptr: no reservoir.  With
+       cres the reservoir arrays (x_n, out_n, y_n, k_n) hold complex {re, im} cells. */
     const v2* x_psi[2];
     const real* x_n[2];
     v2* out_psi[2];
@@ -220,10 +221,12 @@ struct StagedArgs {
     int npump, npot, npulse;
     /* this stage's temporal amplitudes a_g(w) = a_g(t)(1 - w) + w a_g(t + dt) */
     real pump_amp[kStagedMaxGroups];
+    real pump_amp_im[kStagedMaxGroups]; /* imaginary parts (complex reservoir only) */
     v2 pot_amp[kStagedMaxGroups];
     v2 pulse_amp[kStagedMaxGroups];
     const v2* noise; /* this step's noise field (full grid) or nullptr */
     int stochastic, tetm;
+    int cres; /* complex reservoir */
     int Nc, Nr, H, P; /* P = Nc + 2H */
     size_t N, padN;   /* Nc Nr, (Nr + 2H) P */
     int periodic_x, periodic_y;
@@ -238,7 +241,7 @@ struct StagedArgs {
     int ne;
     int eslot[kStagedMaxStages];
     real e[kStagedMaxStages];
-    real dt, dtr, dti; /* psi advances by (dtr + i dti) sum, the reservoir by dt */
+    real dt, dtr, dti; /* psi advances by (dtr + i dti) sum; a real reservoir by dt, a complex one like psi */
     double* partials;  /* FINAL with ne > 0: (error, norm) per block */
     PhxPhys<real> ph;
 };

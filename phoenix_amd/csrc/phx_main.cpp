@@ -191,7 +191,7 @@ void output_initial(phx_solver* s, const Out& o) {
         const std::string sfx = pm[k];
         if (o.want({"all", "mat", k == 0 ? "initial_plus" : "initial_minus", "initial"})) {
             if (field<real>(s, k == 0 ? PHX_PSI_PLUS : PHX_PSI_MINUS, true, z)) write_full(o, "initial_wavefunction_" + sfx, z, true, o.in.t);
-            if (o.in.use_reservoir && field<real>(s, k == 0 ? PHX_N_PLUS : PHX_N_MINUS, false, z))
+            if (o.in.use_reservoir && field<real>(s, k == 0 ? PHX_N_PLUS : PHX_N_MINUS, o.in.complex_reservoir != 0, z))
                 write_full(o, "initial_reservoir_" + sfx, z, true, o.in.t);
         }
         struct Src {
@@ -287,7 +287,7 @@ void output_matrices(phx_solver* s, const Out& o, int x0, int x1, int y0, int y1
         if (o.in.use_reservoir &&
             o.want({"mat", "reservoir", "n", k == 0 ? "reservoir_plus" : "reservoir_minus", k == 0 ? "n_plus" : "n_minus", "plus",
                     "rv", "all"}) &&
-            field<real>(s, k == 0 ? PHX_N_PLUS : PHX_N_MINUS, false, z))
+            field<real>(s, k == 0 ? PHX_N_PLUS : PHX_N_MINUS, o.in.complex_reservoir != 0, z))
             write_matrix(o, prefix + "reservoir_" + sfx + suffix, z, x0, x1, y0, y1, inc, head);
         /* the spectrum of the last FFT time (matrix.fft_plus / fft_minus) */
         if (o.in.fft_every < o.in.t_max && o.want({"fft_mask", "fft", k == 0 ? "fft_plus" : "fft_minus", "plus", "mat", "all"}) &&

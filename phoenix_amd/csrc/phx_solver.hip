@@ -198,7 +198,9 @@ int initialize_t(phx_solver* s) {
                                              (sizeof(real) == 4 ? "8192 (powers of two) or 4096" : "4096 (powers of two) or 2048") +
                                              " points per axis");
     /* The staged path serves what the fused kernels do not: more than seven stages, pulses,
-       several non-constant temporal groups of a source, complex temporal potential envelopes. */
+       several non-constant temporal groups of a source, complex temporal potential envelopes, the
+       complex reservoir. */
+    const size_t nel = c.complex_reservoir ? 2 : 1; /* reals per reservoir cell */
     const auto complex_temporal = [](const phx::Envelope& e) {
         for (int g = 0; g < e.groupSize(); g++)
             if (e.groups[g].kind & phx::TMP_IEXP) return true;
@@ -208,15 +210,15 @@ int initialize_t(phx_solver* s) {
                  (c.use_potentials && c.potential.groupSize() > 1 && !c.potential.allConstant()) ||
                  (c.use_potentials && complex_temporal(c.potential));
     /* SSFM takes the group fields in its own pointwise kernels (phx_ssfm.hip) */
-    s->staged = !s->ssfm && (s->grouped || (s->iter && (!phx::rk_tableau_fused(s->iter->fixed) ||
-                                                        !phx::rk_tableau_fused(s->iter->adaptive))));
+    s->staged = !s->ssfm && (s->grouped || c.complex_reservoir ||
+                             (s->iter && (!phx::rk_tableau_fused(s->iter->fixed) || !phx::rk_tableau_fused(s->iter->adaptive))));
     if (s->staged) s->grouped = true;
 
     for (int k = 0; k < ncomp; k++) {
         for (int b = 0; b < 2; b++) {
             if (int r = dev_alloc(s, &s->psi[k][b], 2 * N * sizeof(real))) return r;
             if (c.use_reservoir)
-                if (int r = dev_alloc(s, &s->n[k][b], N * sizeof(real))) return r;
+                if (int r = dev_alloc(s, &s->n[k][b], nel * N * sizeof(real))) return r;
         }
         /* staged path: one field per temporal group (group g at g N); the fused kernels: one
            field, all groups summed (constant envelopes) or the single group */
@@ -255,8 +257,8 @@ int initialize_t(phx_solver* s) {
             if (int r = dev_alloc(s, &s->st_y_psi[k], 2 * pn * sizeof(real))) return r;
             if (int r = dev_alloc(s, &s->st_k_psi[k], (size_t)H * 2 * pn * sizeof(real))) return r;
             if (c.use_reservoir) {
-                if (int r = dev_alloc(s, &s->st_y_n[k], pn * sizeof(real))) return r;
-                if (int r = dev_alloc(s, &s->st_k_n[k], (size_t)H * pn * sizeof(real))) return r;
+                if (int r = dev_alloc(s, &s->st_y_n[k], nel * pn * sizeof(real))) return r;
+                if (int r = dev_alloc(s, &s->st_k_n[k], (size_t)H * nel * pn * sizeof(real))) return r;
             }
         }
         const size_t nb = phx::staged_final_blocks(c.N_c, c.N_r);
@@ -307,8 +309,11 @@ int initialize_t(phx_solver* s) {
     }
     for (int k = 0; k < ncomp; k++) {
         if (int r = upload_complex<real>(s, s->psi[k][0], psi0[k])) return r;
-        if (c.use_reservoir)
-            if (int r = upload_real<real>(s, s->n[k][0], n0[k])) return r;
+        if (c.use_reservoir) {
+            /* the reference's reservoir is complex-stored; a real one keeps the real part */
+            if (int r = c.complex_reservoir ? upload_complex<real>(s, s->n[k][0], n0[k]) : upload_real<real>(s, s->n[k][0], n0[k]))
+                return r;
+        }
         const uint32_t pol = k == 0 ? phx::POL_PLUS : phx::POL_MINUS;
         std::vector<std::complex<double>> f;
         /* one spatial field per temporal group (solver_initialization.cu:120-150) */
@@ -489,6 +494,7 @@ int staged_attempt(phx_solver* s, phx::RkTableau tab_id, double t_k, double dt) 
     a.noise = c.use_stochastic ? (const v2*)s->noise : nullptr;
     a.stochastic = c.use_stochastic;
     a.tetm = c.tetm;
+    a.cres = c.complex_reservoir;
     a.Nc = c.N_c;
     a.Nr = c.N_r;
     a.H = s->staged_H;
@@ -517,7 +523,7 @@ int staged_attempt(phx_solver* s, phx::RkTableau tab_id, double t_k, double dt) 
             for (int g = 0; g < ng[q]; g++) {
                 const real re = amp0[q][g].real() * (real(1) - w) + w * amp1[q][g].real();
                 const real im = amp0[q][g].imag() * (real(1) - w) + w * amp1[q][g].imag();
-                if (q == 0) a.pump_amp[g] = re;
+                if (q == 0) a.pump_amp[g] = re, a.pump_amp_im[g] = im;
                 v2 z;
                 z.x = re;
                 z.y = im;
@@ -810,6 +816,12 @@ int iterate_t(phx_solver* s, bool force_fixed) {
         PHX_HIP(phx::launch_imag_normalize<real>((v2*)s->psi[0][s->cur], c.tetm ? (v2*)s->psi[1][s->cur] : nullptr,
                                                  s->cells(), c.imag_time_amplitude, c.dV, s->norm_partials, s->norm_sums,
                                                  s->stream));
+        /* the reservoir to the same amplitude (solver_normalize_imag_time.cu:16-28); with a reservoir
+           imaginary time makes it complex, so it has the layout of psi */
+        if (c.use_reservoir)
+            PHX_HIP(phx::launch_imag_normalize<real>((v2*)s->n[0][s->cur], c.tetm ? (v2*)s->n[1][s->cur] : nullptr,
+                                                     s->cells(), c.imag_time_amplitude, c.dV, s->norm_partials, s->norm_sums,
+                                                     s->stream));
     }
     s->iteration++;
     if (s->rnd(s->t - s->fft_cached_t) < c.fft_every) return 1; /* FFT guard in the build's real (solver.cu:62-63) */
@@ -1100,7 +1112,7 @@ int field_info(phx_solver* s, int field, void*** slot, size_t* elems) {
         case PHX_PSI_PLUS:
         case PHX_PSI_MINUS: *slot = &s->psi[k][s->cur]; *elems = 2 * N; break;
         case PHX_N_PLUS:
-        case PHX_N_MINUS: *slot = &s->n[k][s->cur]; *elems = N; break;
+        case PHX_N_MINUS: *slot = &s->n[k][s->cur]; *elems = s->cfg.complex_reservoir ? 2 * N : N; break;
         case PHX_PUMP_PLUS:
         case PHX_PUMP_MINUS: *slot = &s->pump[k]; *elems = N; break;
         case PHX_POTENTIAL_PLUS:
@@ -1432,6 +1444,7 @@ int phx_get_info(phx_solver* s, phx_info* info) {
     info->staged = s->staged;
     info->multi_step_steps = s->flow_steps;
     info->multi_step_launches = s->flow_launches;
+    info->complex_reservoir = s->cfg.complex_reservoir;
     return PHX_OK;
 }
 

@@ -11,8 +11,9 @@
  *     and the FP64 per-block sums of |dt sum_j e_j K_j(psi+)|^2 and |psi+'|^2.
  * It covers any tableau (the 12- and 13-stage DP85 and NSRK78 included), pump, potential
  * and pulse chains with any number of temporal groups (kernel_gp_compute.cuh:68-100),
- * complex temporal envelopes on the potential and the pulse, TE-TM, noise and imaginary
- * time.
+ * complex temporal envelopes on the potential and the pulse, TE-TM, noise, imaginary
+ * time and the complex reservoir (n as {re, im} cells, which imaginary time with a reservoir
+ * and complex temporal pump amplitudes need).
  *
  * Layout: the step-start state X and the output are the solver's full-grid arrays; the
  * stage inputs Y and the K slots live on a grid padded by H = number of stages on every
@@ -76,10 +77,61 @@ __device__ __forceinline__ real noise_sigma(const StagedArgs<real>& a, long gi) 
     return phx_sqrt((RES ? phx_fma(a.ph.R, n, a.ph.gamma_c) : a.ph.gamma_c) / (real(4) * a.ph.dV));
 }
 
+/* principal square root of x + i y: u = sqrt((|z| + |x|) / 2), then (u, y / 2u) for x >= 0 and
+   (|y| / 2u, u sign y) for x < 0 (no cancellation in either branch) */
+template <typename real>
+__device__ __forceinline__ typename Vec2<real>::type csqrt_principal(real x, real y) {
+    typename Vec2<real>::type z;
+    const real r = hypot(x, y);
+    if (r == real(0)) {
+        z.x = real(0);
+        z.y = real(0);
+        return z;
+    }
+    const real u = phx_sqrt((r + fabs(x)) * real(0.5));
+    if (x >= real(0)) {
+        z.x = u;
+        z.y = y / (real(2) * u);
+    } else {
+        z.x = fabs(y) / (real(2) * u);
+        z.y = copysign(u, y);
+    }
+    return z;
+}
+
+/* the noise increment w_sum xi s of a stage sum, s = sqrt((R n+ + gamma_c) / (4 dV)); with a complex
+   reservoir s is the principal complex square root and the product complex */
+template <typename real, bool RES, bool CRES>
+__device__ __forceinline__ void noise_inc(const StagedArgs<real>& a, long gi, real& nzr, real& nzi) {
+    using v2 = typename Vec2<real>::type;
+    const real xr = a.wsum * a.noise[gi].x, xi = a.wsum * a.noise[gi].y;
+    if constexpr (CRES) {
+        const v2 n = reinterpret_cast<const v2*>(a.x_n[0])[gi];
+        const real d = real(4) * a.ph.dV;
+        const v2 sq = csqrt_principal<real>(phx_fma(a.ph.R, n.x, a.ph.gamma_c) / d, (a.ph.R * n.y) / d);
+        nzr = xr * sq.x - xi * sq.y;
+        nzi = xr * sq.y + xi * sq.x;
+    } else {
+        const real sg = noise_sigma<real, RES>(a, gi);
+        nzr = xr * sg;
+        nzi = xi * sg;
+    }
+}
+
+/* a complex stage sum of the reservoir: x + (dtr + i dti) (sr + i si), as for psi */
+template <typename real>
+__device__ __forceinline__ typename Vec2<real>::type cres_advance(const StagedArgs<real>& a, typename Vec2<real>::type x, real sr,
+                                                                  real si) {
+    typename Vec2<real>::type y;
+    y.x = phx_fma(a.dtr, sr, phx_fma(-a.dti, si, x.x));
+    y.y = phx_fma(a.dtr, si, phx_fma(a.dti, sr, x.y));
+    return y;
+}
+
 } // namespace
 
 /* ------------------------------------------------------------------ CALCULATE_K */
-template <typename real, bool TETM, bool RES>
+template <typename real, bool TETM, bool RES, bool CRES>
 __global__ __launch_bounds__(kBx * kBy) void staged_k(StagedArgs<real> a) {
     using v2 = typename Vec2<real>::type;
     constexpr int NC = TETM ? 2 : 1;
@@ -97,13 +149,24 @@ __global__ __launch_bounds__(kBx * kBy) void staged_k(StagedArgs<real> a) {
         }
         return a.y_psi[k][(size_t)r * P + c];
     };
-    auto res = [&](int k, int r, int c) -> real {
-        if (!RES) return real(0);
+    /* the reservoir at a cell: (Re n, Im n); Im n = 0 for a real reservoir */
+    auto res = [&](int k, int r, int c) -> v2 {
+        v2 z = cz<real>();
+        if (!RES) return z;
+        if constexpr (CRES) {
+            if (first) {
+                const long g = m.idx(r, c);
+                return g >= 0 ? reinterpret_cast<const v2*>(a.x_n[k])[g] : z;
+            }
+            return reinterpret_cast<const v2*>(a.y_n[k])[(size_t)r * P + c];
+        }
         if (first) {
             const long g = m.idx(r, c);
-            return g >= 0 ? a.x_n[k][g] : real(0);
+            z.x = g >= 0 ? a.x_n[k][g] : real(0);
+            return z;
         }
-        return a.y_n[k][(size_t)r * P + c];
+        z.x = a.y_n[k][(size_t)r * P + c];
+        return z;
     };
     const long gi = m.idx(pr, pc); /* sources and noise: 0 outside a zero-boundary domain */
     const PhxPhys<real>& ph = a.ph;
@@ -121,11 +184,16 @@ __global__ __launch_bounds__(kBx * kBy) void staged_k(StagedArgs<real> a) {
     }
 #pragma unroll
     for (int k = 0; k < NC; k++) {
-        const real nv = res(k, pr, pc);
+        const v2 nc = res(k, pr, pc);
+        const real nv = nc.x;
         /* sources: sum over temporal groups g of field_g(cell) * a_g(w) */
-        real pmp = 0, vre = 0, vim = 0, fre = 0, fim = 0;
+        real pmp = 0, pmi = 0, vre = 0, vim = 0, fre = 0, fim = 0;
         if (gi >= 0) {
-            for (int g = 0; g < a.npump; g++) pmp = phx_fma(a.pump[k][(size_t)g * a.N + gi], a.pump_amp[g], pmp);
+            for (int g = 0; g < a.npump; g++) {
+                const real pv = a.pump[k][(size_t)g * a.N + gi];
+                pmp = phx_fma(pv, a.pump_amp[g], pmp);
+                if (CRES) pmi = phx_fma(pv, a.pump_amp_im[g], pmi);
+            }
             for (int g = 0; g < a.npot; g++) {
                 const real v = a.pot[k][(size_t)g * a.N + gi];
                 vre = phx_fma(v, a.pot_amp[g].x, vre);
@@ -159,6 +227,14 @@ __global__ __launch_bounds__(kBx * kBy) void staged_k(StagedArgs<real> a) {
             const real ti = phx_fma(ph.kin_x, ewi[k], ph.kin_y * nsi[k]);
             phx_rhs_scalar<real>(ph, cen[k].x, cen[k].y, tr, ti, nv, pmp, poth, kr, ki, kn);
         }
+        if constexpr (CRES) {
+            /* Im n terms of (1/hbar) g_r n (-i psi) + 1/2 R n psi and of k_n (kernel_gp_compute.cuh:66-82):
+               the hooks above took n = Re n */
+            const real ni = nc.y;
+            const real hr = real(0.5) * ph.R;
+            kr = phx_fma(ph.gr_h * ni, cen[k].x, phx_fma(-(hr * ni), cen[k].y, kr));
+            ki = phx_fma(ph.gr_h * ni, cen[k].y, phx_fma(hr * ni, cen[k].x, ki));
+        }
         /* imaginary part of a complex potential: (1/hbar)(i V_im)(-i psi) = V_im/hbar psi */
         const real vih = vim * ph.one_over_h_bar_s;
         kr = phx_fma(vih, cen[k].x, kr);
@@ -178,12 +254,22 @@ __global__ __launch_bounds__(kBx * kBy) void staged_k(StagedArgs<real> a) {
         kv.x = kr;
         kv.y = ki;
         a.k_psi[k][o] = kv;
-        if (RES) a.k_n[k][o] = kn;
+        if constexpr (CRES) {
+            /* Im k_n = -(gamma_r + R |psi|^2) Im n + sum_g P_g Im a_g (+ R Im n / dV with noise) */
+            real kni = phx_fma(phx_fma(-ph.R, nrm[k], -ph.gamma_r), nc.y, pmi);
+            if (a.stochastic) kni = phx_fma(ph.R / ph.dV, nc.y, kni);
+            v2 knv;
+            knv.x = kn;
+            knv.y = kni;
+            reinterpret_cast<v2*>(a.k_n[k])[o] = knv;
+        } else if (RES) {
+            a.k_n[k][o] = kn;
+        }
     }
 }
 
 /* ----------------------------------------------------------- INTERMEDIATE_SUM_K */
-template <typename real, bool TETM, bool RES>
+template <typename real, bool TETM, bool RES, bool CRES>
 __global__ __launch_bounds__(kBx * kBy) void staged_sum(StagedArgs<real> a) {
     using v2 = typename Vec2<real>::type;
     constexpr int NC = TETM ? 2 : 1;
@@ -195,32 +281,39 @@ __global__ __launch_bounds__(kBx * kBy) void staged_sum(StagedArgs<real> a) {
     const long gi = m.idx(pr, pc);
     const size_t pi = (size_t)pr * a.P + pc;
     real nzr = 0, nzi = 0;
-    if (a.stochastic && gi >= 0) {
-        const real sg = noise_sigma<real, RES>(a, gi);
-        nzr = a.wsum * a.noise[gi].x * sg;
-        nzi = a.wsum * a.noise[gi].y * sg;
-    }
+    if (a.stochastic && gi >= 0) noise_inc<real, RES, CRES>(a, gi, nzr, nzi);
 #pragma unroll
     for (int k = 0; k < NC; k++) {
-        real sr = 0, si = 0, sn = 0;
+        real sr = 0, si = 0, sn = 0, sni = 0;
         for (int j = 0; j < a.nw; j++) {
             const size_t o = (size_t)a.slot[j] * a.padN + pi;
             const v2 kv = a.k_psi[k][o];
             sr = phx_fma(a.w[j], kv.x, sr);
             si = phx_fma(a.w[j], kv.y, si);
-            if (RES) sn = phx_fma(a.w[j], a.k_n[k][o], sn);
+            if constexpr (CRES) {
+                const v2 kn = reinterpret_cast<const v2*>(a.k_n[k])[o];
+                sn = phx_fma(a.w[j], kn.x, sn);
+                sni = phx_fma(a.w[j], kn.y, sni);
+            } else if (RES) {
+                sn = phx_fma(a.w[j], a.k_n[k][o], sn);
+            }
         }
         const v2 x = gi >= 0 ? a.x_psi[k][gi] : cz<real>();
         v2 y;
         y.x = phx_fma(a.dtr, sr, phx_fma(-a.dti, si, x.x)) + nzr;
         y.y = phx_fma(a.dtr, si, phx_fma(a.dti, sr, x.y)) + nzi;
         a.y_psi[k][pi] = y;
-        if (RES) a.y_n[k][pi] = phx_fma(a.dt, sn, gi >= 0 ? a.x_n[k][gi] : real(0));
+        if constexpr (CRES) {
+            const v2 xn = gi >= 0 ? reinterpret_cast<const v2*>(a.x_n[k])[gi] : cz<real>();
+            reinterpret_cast<v2*>(a.y_n[k])[pi] = cres_advance<real>(a, xn, sn, sni);
+        } else if (RES) {
+            a.y_n[k][pi] = phx_fma(a.dt, sn, gi >= 0 ? a.x_n[k][gi] : real(0));
+        }
     }
 }
 
 /* --------------------------------------------------------- FINAL_SUM_K / ERROR_K */
-template <typename real, bool TETM, bool RES>
+template <typename real, bool TETM, bool RES, bool CRES>
 __global__ __launch_bounds__(kBx * kBy) void staged_final(StagedArgs<real> a) {
     using v2 = typename Vec2<real>::type;
     constexpr int NC = TETM ? 2 : 1;
@@ -232,27 +325,33 @@ __global__ __launch_bounds__(kBx * kBy) void staged_final(StagedArgs<real> a) {
         const long gi = (long)r * a.Nc + c;
         const size_t pi = (size_t)(r + a.H) * a.P + (c + a.H);
         real nzr = 0, nzi = 0;
-        if (a.stochastic) {
-            const real sg = noise_sigma<real, RES>(a, gi);
-            nzr = a.wsum * a.noise[gi].x * sg;
-            nzi = a.wsum * a.noise[gi].y * sg;
-        }
+        if (a.stochastic) noise_inc<real, RES, CRES>(a, gi, nzr, nzi);
 #pragma unroll
         for (int k = 0; k < NC; k++) {
-            real sr = 0, si = 0, sn = 0;
+            real sr = 0, si = 0, sn = 0, sni = 0;
             for (int j = 0; j < a.nw; j++) {
                 const size_t o = (size_t)a.slot[j] * a.padN + pi;
                 const v2 kv = a.k_psi[k][o];
                 sr = phx_fma(a.w[j], kv.x, sr);
                 si = phx_fma(a.w[j], kv.y, si);
-                if (RES) sn = phx_fma(a.w[j], a.k_n[k][o], sn);
+                if constexpr (CRES) {
+                    const v2 kn = reinterpret_cast<const v2*>(a.k_n[k])[o];
+                    sn = phx_fma(a.w[j], kn.x, sn);
+                    sni = phx_fma(a.w[j], kn.y, sni);
+                } else if (RES) {
+                    sn = phx_fma(a.w[j], a.k_n[k][o], sn);
+                }
             }
             const v2 x = a.x_psi[k][gi];
             v2 y;
             y.x = phx_fma(a.dtr, sr, phx_fma(-a.dti, si, x.x)) + nzr;
             y.y = phx_fma(a.dtr, si, phx_fma(a.dti, sr, x.y)) + nzi;
             a.out_psi[k][gi] = y;
-            if (RES) a.out_n[k][gi] = phx_fma(a.dt, sn, a.x_n[k][gi]);
+            if constexpr (CRES) {
+                reinterpret_cast<v2*>(a.out_n[k])[gi] = cres_advance<real>(a, reinterpret_cast<const v2*>(a.x_n[k])[gi], sn, sni);
+            } else if (RES) {
+                a.out_n[k][gi] = phx_fma(a.dt, sn, a.x_n[k][gi]);
+            }
             if (k == 0 && a.ne > 0) {
                 /* runge_sum_to_error (kernel_summation.cuh:270-289); with -tetm the psi+
                    error counts twice (cuda_macro.cuh:244) */
@@ -302,15 +401,15 @@ dim3 ext_grid(const StagedArgs<real>& a) {
     return dim3((unsigned)((w + kBx - 1) / kBx), (unsigned)((h + kBy - 1) / kBy));
 }
 
-template <typename real, bool TETM, bool RES>
+template <typename real, bool TETM, bool RES, bool CRES>
 hipError_t launch_kind(const StagedArgs<real>& a, StagedOp op, hipStream_t s) {
     const dim3 blk(kBx, kBy);
     switch (op) {
-        case StagedOp::K: hipLaunchKernelGGL((staged_k<real, TETM, RES>), ext_grid(a), blk, 0, s, a); break;
-        case StagedOp::SUM: hipLaunchKernelGGL((staged_sum<real, TETM, RES>), ext_grid(a), blk, 0, s, a); break;
+        case StagedOp::K: hipLaunchKernelGGL((staged_k<real, TETM, RES, CRES>), ext_grid(a), blk, 0, s, a); break;
+        case StagedOp::SUM: hipLaunchKernelGGL((staged_sum<real, TETM, RES, CRES>), ext_grid(a), blk, 0, s, a); break;
         case StagedOp::FINAL: {
             const dim3 g((unsigned)((a.Nc + kBx - 1) / kBx), (unsigned)((a.Nr + kBy - 1) / kBy));
-            hipLaunchKernelGGL((staged_final<real, TETM, RES>), g, blk, 0, s, a);
+            hipLaunchKernelGGL((staged_final<real, TETM, RES, CRES>), g, blk, 0, s, a);
             break;
         }
     }
@@ -325,8 +424,10 @@ hipError_t launch_staged(const StagedArgs<real>& a, StagedOp op, hipStream_t s) 
         a.npot > kStagedMaxGroups || a.npulse > kStagedMaxGroups)
         return hipErrorInvalidValue;
     const bool res = a.x_n[0] != nullptr;
-    if (a.tetm) return res ? launch_kind<real, true, true>(a, op, s) : launch_kind<real, true, false>(a, op, s);
-    return res ? launch_kind<real, false, true>(a, op, s) : launch_kind<real, false, false>(a, op, s);
+    if (a.cres && !res) return hipErrorInvalidValue;
+    if (a.cres) return a.tetm ? launch_kind<real, true, true, true>(a, op, s) : launch_kind<real, false, true, true>(a, op, s);
+    if (a.tetm) return res ? launch_kind<real, true, true, false>(a, op, s) : launch_kind<real, true, false, false>(a, op, s);
+    return res ? launch_kind<real, false, true, false>(a, op, s) : launch_kind<real, false, false, false>(a, op, s);
 }
 
 template hipError_t launch_staged<float>(const StagedArgs<float>&, StagedOp, hipStream_t);

@@ -82,13 +82,28 @@ def test_envelope_flags_enable_reservoir_and_pump():
     assert not s.info["use_reservoir"]
 
 
+def test_complex_reservoir_configurations():
+    """n is complex under a complex temporal pump amplitude or --imagTime with a reservoir; a real
+    pump envelope or imaginary time without a reservoir keep it real (the staged path that serves it is
+    chosen at phx_initialize, on the device)."""
+    pump = ["--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss"]
+    cases = [(pump + ["time", "iexp", 0, 1, 1], 1), (["--imagTime", 1.0] + pump, 1), (pump, 0),
+             (pump + ["time", "cos", 0, 1, 1], 0), (["--imagTime", 1.0], 0)]
+    for args, want in cases:
+        i = make(["--N", 32, 32] + args).info
+        assert i["complex_reservoir"] == want, args
+        if want:
+            assert i["use_reservoir"] == 1
+
+
 @pytest.mark.parametrize("args,code", [
     (["--N", 31, 32], -1),               # odd dimension
     (["--N", 32, 32, "--tstep", 0], -1),  # dt <= 0
     (["--N", 32, 32, "--tstep", 2, "--tmax", 1], -1),  # dt > t_max
-    (["--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss", "time", "iexp", 0, 1, 1], -3),  # complex pump amplitude
+    # a complex reservoir on the split-step iterator (the RK iterators take it on the staged path)
+    (["-ssfm", "--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss", "time", "iexp", 0, 1, 1], -3),
     (["--iterator", "NoSuchKey"], -3),    # unknown key (solver_factory.hpp:59)
-    (["--imagTime", 1.0, "--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss"], -3),  # real reservoir
+    (["-ssfm", "--imagTime", 1.0, "--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss"], -3),
     (["--N", "abc", 32], -1),
     (["--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "notatype"], -1),
 ])
