@@ -16,6 +16,8 @@ constexpr int kWavesPerBlock = 4;
    l+1 for wave_shl:1 and lane l-1 for wave_shr:1; the edge lane reads 0. */
 constexpr int kDppWaveShl1 = 0x130;
 constexpr int kDppWaveShr1 = 0x138;
+/* quad_perm [1, 0, 3, 2]: lane l reads lane l ^ 1 (adjacent lanes swap) */
+constexpr int kDppPairSwap = 0xB1;
 /* buffer descriptor word 3 for GFX9-family raw buffers */
 constexpr int kBufferFlags = 0x00020000;
 /* byte offset beyond any row: raw buffer loads there return 0 */

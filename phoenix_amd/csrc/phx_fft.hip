@@ -25,6 +25,7 @@
 #include <cstdlib>
 
 #include "custom_rhs.h"
+#include "phx_device.h"
 #include "phx_kernels.h"
 
 namespace phx {
@@ -333,7 +334,7 @@ __device__ __forceinline__ Cx<real> twiddle(const Cx<real>* __restrict__ tw, int
 template <typename real>
 __host__ __device__ constexpr int fft4_col_skew() { return 2; }
 
-/* threads per workgroup of the four-step kernels */
+/* threads per workgroup of the four-step row kernel (one thread per sub-sequence) */
 template <typename real>
 __host__ __device__ constexpr int fft4_threads() { return sizeof(real) == 4 ? 256 : 128; }
 
@@ -375,93 +376,143 @@ __global__ __launch_bounds__(256) void fft4_rows(typename Vec2<real>::type* data
     }
 }
 
-/* columns: CB adjacent columns per workgroup (thread = column + CB x sub-sequence), forward
-   transform, the k-space factor of MODE (as fft_cols_filter), inverse transform; LDS [CB][N1][N2]
-   for the forward transpose and [CB][N2][N1] for the inverse (swizzled).  CB = fft4_threads /
-   max(N1, N2): 8 FP32 / 4 FP64 columns at 1024^2, 128 / 256 workgroups.  Whole 128-byte row segments
-   (16 FP32 / 8 FP64 columns, 128 KiB of LDS) left 64 / 128 workgroups and took 34.0 / 25.3 us
-   instead of 19.4 / 19.7 us: the pass is bound by the parallelism, not by the segment width. */
+/* Lane pairs: the column pass gives each sub-sequence of R = 2H points to two adjacent lanes (p = 0, 1),
+   one radix-2 stage between them (one DPP lane swap per element) and an H-point transform in each lane, so
+   that a column has 2 max(N1, N2) threads.  pair_dit: lane p holds x[2j + p] on entry and X[m + p H]
+   on return (X[m] = E[m] + w^m O[m], X[m + H] = E[m] - w^m O[m], E / O the transforms of the even /
+   odd points); pair_dif: lane p holds x[m + p H] on entry and X[2k + p] on return (the halves
+   x[m] + x[m + H] and (x[m] - x[m + H]) w^m transformed); w = e^(SIGN 2 pi i / R). */
+template <typename real>
+__device__ __forceinline__ Cx<real> shfl_pair(Cx<real> v) {
+    return {dpp_move<kDppPairSwap>(v.r), dpp_move<kDppPairSwap>(v.i)};
+}
+
+template <typename real, int H, int SIGN>
+__device__ __forceinline__ void pair_dit(Cx<real> (&x)[H], const Cx<real>* __restrict__ tw, int N, int p) {
+    fft_reg<real, H, SIGN>(x, tw, N);
+    const int step = N / (2 * H);
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+        const Cx<real> b = shfl_pair(x[m]);
+        const Cx<real> e = p ? b : x[m], o = p ? x[m] : b;
+        const Cx<real> q = m == 0 ? o : cmul(o, twiddle<real, SIGN>(tw, m * step));
+        const Cx<real> sum = {e.r + q.r, e.i + q.i}, dif = {e.r - q.r, e.i - q.i};
+        x[m] = p ? dif : sum;
+    }
+}
+
+template <typename real, int H, int SIGN>
+__device__ __forceinline__ void pair_dif(Cx<real> (&x)[H], const Cx<real>* __restrict__ tw, int N, int p) {
+    const int step = N / (2 * H);
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+        const Cx<real> b = shfl_pair(x[m]);
+        const Cx<real> sum = {x[m].r + b.r, x[m].i + b.i}, d = {b.r - x[m].r, b.i - x[m].i};
+        const Cx<real> dw = m == 0 ? d : cmul(d, twiddle<real, SIGN>(tw, m * step));
+        x[m] = p ? dw : sum;
+    }
+    fft_reg<real, H, SIGN>(x, tw, N);
+}
+
+/* threads per workgroup of the column pass */
+constexpr int kFft4ColThreads = 256;
+
+/* columns: CB adjacent columns per workgroup, 2 max(N1, N2) threads per column (lane pairs above):
+   forward transform, the k-space factor of MODE (as fft_cols_filter), inverse transform; LDS
+   [CB][N1][N2] for the forward transpose and [CB][N2][N1] for the inverse (swizzled).
+   thread = p + 2 (column + CB sub-sequence).  At 1024^2: 4 columns per workgroup, 256 workgroups,
+   1024 waves (one lane per sub-sequence gave 512 waves, half the SIMDs, and 16.9 / 19.5 us FP32 /
+   FP64).  Whole 128-byte row segments per workgroup (16 FP32 columns) were slower: the pass is bound
+   by its parallelism, not by the segment width. */
 template <typename real, int N1, int N2, int CB, int MODE>
-__global__ __launch_bounds__(CB *(N1 > N2 ? N1 : N2)) void fft4_cols(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw,
-                                                 const real* __restrict__ mask, const Cx<real>* __restrict__ rowf,
-                                                 const Cx<real>* __restrict__ colf, int Nc, real Ntot, typename Vec2<real>::type* spec) {
-    constexpr int N = N1 * N2;
+__global__ __launch_bounds__(2 * CB *(N1 > N2 ? N1 : N2)) void fft4_cols(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw,
+                                                     const real* __restrict__ mask, const Cx<real>* __restrict__ rowf,
+                                                     const Cx<real>* __restrict__ colf, int Nc, real Ntot, typename Vec2<real>::type* spec) {
+    constexpr int N = N1 * N2, H1 = N1 / 2, H2 = N2 / 2;
     constexpr int SLAB = N + fft4_col_skew<real>();
     extern __shared__ __align__(16) unsigned char lds_raw[];
     Cx<real>* S = reinterpret_cast<Cx<real>*>(lds_raw);
-    const int c = threadIdx.x % CB, t = threadIdx.x / CB;
+    const int p = threadIdx.x & 1, r = threadIdx.x >> 1;
+    const int c = r % CB, t = r / CB;
     const int col = fft_xcd_group(blockIdx.x, gridDim.x) * CB + c;
     Cx<real>* sc = S + c * SLAB;
-    /* forward, first half: n2 = t */
+    /* forward, first half: n2 = t; lane p holds n1 = 2j + p, then k1 = m + p H1 */
     if (t < N2) {
-        Cx<real> x[N1];
+        Cx<real> x[H1];
 #pragma unroll
-        for (int n1 = 0; n1 < N1; n1++) {
-            const auto v = data[(size_t)(N2 * n1 + t) * Nc + col];
-            x[n1] = {v.x, v.y};
+        for (int j = 0; j < H1; j++) {
+            const auto v = data[(size_t)(N2 * (2 * j + p) + t) * Nc + col];
+            x[j] = {v.x, v.y};
         }
-        fft_reg<real, N1, -1>(x, tw, N);
+        pair_dit<real, H1, -1>(x, tw, N, p);
 #pragma unroll
-        for (int k1 = 0; k1 < N1; k1++) sc[swz<N2>(k1, t)] = k1 == 0 ? x[0] : cmul(x[k1], twiddle<real, -1>(tw, t * k1));
+        for (int m = 0; m < H1; m++) {
+            const int k1 = m + p * H1;
+            sc[swz<N2>(k1, t)] = k1 == 0 ? x[m] : cmul(x[m], twiddle<real, -1>(tw, t * k1));
+        }
     }
     __syncthreads();
-    /* forward, second half: k1 = t holds X[t + N1 k2], k2 = 0..N2-1 */
-    Cx<real> y[N2];
+    /* forward, second half: k1 = t; lane p reads n2 = 2j + p, then holds X[t + N1 k2], k2 = m + p H2 */
+    Cx<real> y[H2];
     if (t < N1) {
 #pragma unroll
-        for (int n2 = 0; n2 < N2; n2++) y[n2] = sc[swz<N2>(t, n2)];
-        fft_reg<real, N2, -1>(y, tw, N);
+        for (int j = 0; j < H2; j++) y[j] = sc[swz<N2>(t, 2 * j + p)];
+        pair_dit<real, H2, -1>(y, tw, N, p);
         if constexpr (MODE == kColForward) {
 #pragma unroll
-            for (int k2 = 0; k2 < N2; k2++) {
+            for (int m = 0; m < H2; m++) {
                 typename Vec2<real>::type o;
-                o.x = y[k2].r;
-                o.y = y[k2].i;
-                data[(size_t)(t + N1 * k2) * Nc + col] = o;
+                o.x = y[m].r;
+                o.y = y[m].i;
+                data[(size_t)(t + N1 * (m + p * H2)) * Nc + col] = o;
             }
         } else if constexpr (MODE == kColPropagate) {
             const Cx<real> cf = colf[col];
 #pragma unroll
-            for (int k2 = 0; k2 < N2; k2++) y[k2] = cmul(cmul(y[k2], cf), rowf[t + N1 * k2]);
+            for (int m = 0; m < H2; m++) y[m] = cmul(cmul(y[m], cf), rowf[t + N1 * (m + p * H2)]);
         } else {
             const int cs = (col + (Nc >> 1)) & (Nc - 1);
-            real mk[N2];
+            real mk[H2];
 #pragma unroll
-            for (int k2 = 0; k2 < N2; k2++) mk[k2] = mask[(size_t)(((t + N1 * k2) + (N >> 1)) & (N - 1)) * Nc + cs];
+            for (int m = 0; m < H2; m++) mk[m] = mask[(size_t)(((t + N1 * (m + p * H2)) + (N >> 1)) & (N - 1)) * Nc + cs];
 #pragma unroll
-            for (int k2 = 0; k2 < N2; k2++) {
-                y[k2].r = y[k2].r / Ntot * mk[k2];
-                y[k2].i = y[k2].i / Ntot * mk[k2];
+            for (int m = 0; m < H2; m++) {
+                y[m].r = y[m].r / Ntot * mk[m];
+                y[m].i = y[m].i / Ntot * mk[m];
                 if (spec) {
                     typename Vec2<real>::type o;
-                    o.x = y[k2].r;
-                    o.y = y[k2].i;
-                    spec[(size_t)(t + N1 * k2) * Nc + col] = o;
+                    o.x = y[m].r;
+                    o.y = y[m].i;
+                    spec[(size_t)(t + N1 * (m + p * H2)) * Nc + col] = o;
                 }
             }
         }
     }
     if constexpr (MODE == kColForward) return;
     __syncthreads(); /* the forward transpose slab is reused */
-    /* inverse, first half: k1 = t; Z[p] = sum_k2 X[t + N1 k2] W_N2^(-p k2), times W_N^(-p t) */
+    /* inverse, first half: k1 = t; the pair holds k2 in halves and leaves Z[p'], p' = 2j + p,
+       Z[p'] = sum_k2 X[t + N1 k2] W_N2^(-p' k2), then times W_N^(-p' t) */
     if (t < N1) {
-        fft_reg<real, N2, +1>(y, tw, N);
+        pair_dif<real, H2, +1>(y, tw, N, p);
 #pragma unroll
-        for (int p = 0; p < N2; p++) sc[swz<N1>(p, t)] = p == 0 ? y[0] : cmul(y[p], twiddle<real, +1>(tw, p * t));
+        for (int j = 0; j < H2; j++) {
+            const int q = 2 * j + p;
+            sc[swz<N1>(q, t)] = q == 0 ? y[j] : cmul(y[j], twiddle<real, +1>(tw, q * t));
+        }
     }
     __syncthreads();
-    /* inverse, second half: p = t; x[t + N2 q] = sum_k1 S[t][k1] W_N1^(-q k1) */
+    /* inverse, second half: p' = t; lane p reads k1 = 2j + p, then holds x[t + N2 q], q = m + p H1 */
     if (t < N2) {
-        Cx<real> w[N1];
+        Cx<real> w[H1];
 #pragma unroll
-        for (int k1 = 0; k1 < N1; k1++) w[k1] = sc[swz<N1>(t, k1)];
-        fft_reg<real, N1, +1>(w, tw, N);
+        for (int j = 0; j < H1; j++) w[j] = sc[swz<N1>(t, 2 * j + p)];
+        pair_dit<real, H1, +1>(w, tw, N, p);
 #pragma unroll
-        for (int q = 0; q < N1; q++) {
+        for (int m = 0; m < H1; m++) {
             typename Vec2<real>::type o;
-            o.x = w[q].r;
-            o.y = w[q].i;
-            data[(size_t)(t + N2 * q) * Nc + col] = o;
+            o.x = w[m].r;
+            o.y = w[m].i;
+            data[(size_t)(t + N2 * (m + p * H1)) * Nc + col] = o;
         }
     }
 }
@@ -503,7 +554,7 @@ static hipError_t launch_fft4_rows(typename Vec2<real>::type* psi, const void* t
 template <typename real, int N1, int N2, int MODE>
 static hipError_t launch_fft4_cols_t(typename Vec2<real>::type* psi, const void* tw, const real* mask, const void* rowf,
                                      const void* colf, int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec) {
-    constexpr int T = N1 > N2 ? N1 : N2, CB = fft4_threads<real>() / T;
+    constexpr int T = N1 > N2 ? N1 : N2, CB = kFft4ColThreads / (2 * T);
     if (Nc % CB) return hipErrorInvalidValue;
     const size_t lds = (size_t)CB * (N1 * N2 + fft4_col_skew<real>()) * sizeof(Cx<real>);
     static const hipError_t attr = lds > 65536 ? hipFuncSetAttribute((const void*)&fft4_cols<real, N1, N2, CB, MODE>,
@@ -511,7 +562,7 @@ static hipError_t launch_fft4_cols_t(typename Vec2<real>::type* psi, const void*
                                                : hipSuccess; /* beyond 64 KiB of LDS (gfx950 grants up to 160 KiB) */
     if (attr != hipSuccess) return attr;
     const real ntot = (real)((double)Nc * (double)Nr);
-    hipLaunchKernelGGL((fft4_cols<real, N1, N2, CB, MODE>), dim3(Nc / CB), dim3(CB * T), lds, s, psi, (const Cx<real>*)tw, mask,
+    hipLaunchKernelGGL((fft4_cols<real, N1, N2, CB, MODE>), dim3(Nc / CB), dim3(2 * CB * T), lds, s, psi, (const Cx<real>*)tw, mask,
                        (const Cx<real>*)rowf, (const Cx<real>*)colf, Nc, ntot, spec);
     return hipGetLastError();
 }
@@ -746,7 +797,7 @@ template <typename real, int MODE>
 static hipError_t fft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, const void* rowf,
                                 const void* colf, int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec = nullptr) {
     int n1, n2;
-    if (fft4_enabled() && fft4_split(Nc, n1, n2) && fft4_split(Nr, n1, n2) && Nc % (fft4_threads<real>() / 16) == 0) {
+    if (fft4_enabled() && fft4_split(Nc, n1, n2) && fft4_split(Nr, n1, n2) && Nc % (kFft4ColThreads / 32) == 0) {
         /* FP32 rows stay on the Stockham kernel: its 4 K waves per 1024 rows (four elements a thread)
            beat the four-step row kernel's 512 at 1024^2 (6.95 / 9.88 vs 7.88 / 10.14 us forward /
            inverse); in FP64 the four-step rows are faster (9.05 / 13.81 vs 10.0 / 15.7 us).  Both
