@@ -10,6 +10,10 @@
  *                           inverse FFT -- all without leaving LDS;
  *   3. fft_rows<inverse>  : inverse row FFTs.
  * Each kernel reads and writes the grid once (6 passes over N_c N_r complex).
+ * Lengths 256, 512 and 1024 on both axes take the four-step kernels instead (fft4_rows,
+ * fft4_cols: N = N1 N2, sub-transforms in registers, one LDS transpose per transform, two
+ * lanes per sub-sequence); other powers of two the Stockham kernels below, other even
+ * lengths direct DFTs.
  *
  * The transforms are unnormalised DFTs with the forward sign e^{-2 pi i jk/N}
  * (cuFFT/FFTW convention).  Algorithm: Stockham autosort, radix 4 with a final
@@ -327,58 +331,9 @@ __device__ __forceinline__ Cx<real> twiddle(const Cx<real>* __restrict__ tw, int
     return w;
 }
 
-/* LDS elements between the column slabs of fft4_cols beyond N: the slabs of the CB columns a wave
-   touches then start on different banks (N apart, the same (k1, t) of every column fell on one
-   bank).  SQ_LDS_BANK_CONFLICT per LDS instruction at 1024^2, FP32 / FP64: skew 0 21 / -, 2 1 / 2,
-   4 2 / 4, 8 7 / 14; the pass time moved by under 1 % between skews 2 and 8 (parallelism-bound). */
-template <typename real>
-__host__ __device__ constexpr int fft4_col_skew() { return 2; }
-
-/* threads per workgroup of the four-step row kernel (one thread per sub-sequence) */
-template <typename real>
-__host__ __device__ constexpr int fft4_threads() { return sizeof(real) == 4 ? 256 : 128; }
-
-/* rows: fft4_threads / max(N1, N2) rows per workgroup, LDS [rows][N1][N2] (swizzled) */
-template <typename real, int N1, int N2, int SIGN>
-__global__ __launch_bounds__(256) void fft4_rows(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw, int nrows) {
-    constexpr int N = N1 * N2, T = N1 > N2 ? N1 : N2, RPB = fft4_threads<real>() / T;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    Cx<real>* S = reinterpret_cast<Cx<real>*>(lds_raw);
-    const int lr = threadIdx.x / T, t = threadIdx.x - lr * T;
-    const int row = blockIdx.x * RPB + lr;
-    const bool live = row < nrows;
-    typename Vec2<real>::type* rp = data + (size_t)(live ? row : 0) * N;
-    Cx<real>* sr = S + lr * N;
-    if (t < N2) {
-        Cx<real> x[N1];
-#pragma unroll
-        for (int n1 = 0; n1 < N1; n1++) {
-            const auto v = rp[N2 * n1 + t];
-            x[n1] = {v.x, v.y};
-        }
-        fft_reg<real, N1, SIGN>(x, tw, N);
-#pragma unroll
-        for (int k1 = 0; k1 < N1; k1++) sr[swz<N2>(k1, t)] = k1 == 0 ? x[0] : cmul(x[k1], twiddle<real, SIGN>(tw, t * k1));
-    }
-    __syncthreads();
-    if (t < N1 && live) {
-        Cx<real> y[N2];
-#pragma unroll
-        for (int n2 = 0; n2 < N2; n2++) y[n2] = sr[swz<N2>(t, n2)];
-        fft_reg<real, N2, SIGN>(y, tw, N);
-#pragma unroll
-        for (int k2 = 0; k2 < N2; k2++) {
-            typename Vec2<real>::type o;
-            o.x = y[k2].r;
-            o.y = y[k2].i;
-            rp[t + N1 * k2] = o;
-        }
-    }
-}
-
-/* Lane pairs: the column pass gives each sub-sequence of R = 2H points to two adjacent lanes (p = 0, 1),
+/* Lane pairs: the four-step kernels give each sub-sequence of R = 2H points to two adjacent lanes (p = 0, 1),
    one radix-2 stage between them (one DPP lane swap per element) and an H-point transform in each lane, so
-   that a column has 2 max(N1, N2) threads.  pair_dit: lane p holds x[2j + p] on entry and X[m + p H]
+   that a row or column has 2 max(N1, N2) threads.  pair_dit: lane p holds x[2j + p] on entry and X[m + p H]
    on return (X[m] = E[m] + w^m O[m], X[m + H] = E[m] - w^m O[m], E / O the transforms of the even /
    odd points); pair_dif: lane p holds x[m + p H] on entry and X[2k + p] on return (the halves
    x[m] + x[m + H] and (x[m] - x[m + H]) w^m transformed); w = e^(SIGN 2 pi i / R). */
@@ -414,8 +369,61 @@ __device__ __forceinline__ void pair_dif(Cx<real> (&x)[H], const Cx<real>* __res
     fft_reg<real, H, SIGN>(x, tw, N);
 }
 
-/* threads per workgroup of the column pass */
-constexpr int kFft4ColThreads = 256;
+/* LDS elements between the column slabs of fft4_cols beyond N: the slabs of the CB columns a wave
+   touches then start on different banks (N apart, the same (k1, t) of every column fell on one
+   bank).  SQ_LDS_BANK_CONFLICT per LDS instruction at 1024^2, FP32 / FP64: skew 0 21 / -, 2 1 / 2,
+   4 2 / 4, 8 7 / 14; the pass time moved by under 1 % between skews 2 and 8 (parallelism-bound). */
+template <typename real>
+__host__ __device__ constexpr int fft4_col_skew() { return 2; }
+
+/* threads per workgroup of the four-step kernels (two lanes per sub-sequence) */
+constexpr int kFft4Threads = 256;
+
+/* rows: kFft4Threads / (2 max(N1, N2)) rows per workgroup, two lanes per sub-sequence (lane pairs
+   above), LDS [rows][N1][N2] (swizzled).  A wave holds one 1024-point row: each load and store
+   instruction moves two contiguous 32-element runs of it. */
+template <typename real, int N1, int N2, int SIGN>
+__global__ __launch_bounds__(kFft4Threads) void fft4_rows(typename Vec2<real>::type* data, const Cx<real>* __restrict__ tw, int nrows) {
+    constexpr int N = N1 * N2, T = N1 > N2 ? N1 : N2, H1 = N1 / 2, H2 = N2 / 2, RPB = kFft4Threads / (2 * T);
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    Cx<real>* S = reinterpret_cast<Cx<real>*>(lds_raw);
+    const int p = threadIdx.x & 1, r = threadIdx.x >> 1;
+    const int lr = r / T, t = r - lr * T;
+    const int row = blockIdx.x * RPB + lr;
+    const bool live = row < nrows;
+    typename Vec2<real>::type* rp = data + (size_t)(live ? row : 0) * N;
+    Cx<real>* sr = S + lr * N;
+    /* n2 = t; lane p holds n1 = 2j + p, then k1 = m + p H1 */
+    if (t < N2) {
+        Cx<real> x[H1];
+#pragma unroll
+        for (int j = 0; j < H1; j++) {
+            const auto v = rp[N2 * (2 * j + p) + t];
+            x[j] = {v.x, v.y};
+        }
+        pair_dit<real, H1, SIGN>(x, tw, N, p);
+#pragma unroll
+        for (int m = 0; m < H1; m++) {
+            const int k1 = m + p * H1;
+            sr[swz<N2>(k1, t)] = k1 == 0 ? x[m] : cmul(x[m], twiddle<real, SIGN>(tw, t * k1));
+        }
+    }
+    __syncthreads();
+    /* k1 = t; lane p reads n2 = 2j + p, then holds X[t + N1 k2], k2 = m + p H2 */
+    if (t < N1 && live) {
+        Cx<real> y[H2];
+#pragma unroll
+        for (int j = 0; j < H2; j++) y[j] = sr[swz<N2>(t, 2 * j + p)];
+        pair_dit<real, H2, SIGN>(y, tw, N, p);
+#pragma unroll
+        for (int m = 0; m < H2; m++) {
+            typename Vec2<real>::type o;
+            o.x = y[m].r;
+            o.y = y[m].i;
+            rp[t + N1 * (m + p * H2)] = o;
+        }
+    }
+}
 
 /* columns: CB adjacent columns per workgroup, 2 max(N1, N2) threads per column (lane pairs above):
    forward transform, the k-space factor of MODE (as fft_cols_filter), inverse transform; LDS
@@ -532,11 +540,20 @@ static bool fft4_enabled() {
     return !(e && std::atoi(e) == 0);
 }
 
+/* rows of the four-step path by the four-step row kernel (PHX_FFT4_ROWS=0: the Stockham one).  With
+   lane pairs (1024 waves at 1024^2) it is the faster: FP32 7.38 / 9.93 us forward / inverse against
+   7.41 / 10.02 for the Stockham rows, FP64 9.3 / 11.5 against 10.0 / 15.7; with one lane per
+   sub-sequence (512 waves) it had been slower in FP32 (7.88 / 10.14). */
+static bool fft4_rows_enabled() {
+    static const bool on = [] { const char* e = std::getenv("PHX_FFT4_ROWS"); return !(e && std::atoi(e) == 0); }();
+    return on;
+}
+
 template <typename real, int N1, int N2, int SIGN>
 static hipError_t launch_fft4_rows_t(typename Vec2<real>::type* psi, const void* tw, int Nr, hipStream_t s) {
-    constexpr int T = N1 > N2 ? N1 : N2, RPB = fft4_threads<real>() / T;
+    constexpr int T = N1 > N2 ? N1 : N2, RPB = kFft4Threads / (2 * T);
     const size_t lds = (size_t)RPB * N1 * N2 * sizeof(Cx<real>);
-    hipLaunchKernelGGL((fft4_rows<real, N1, N2, SIGN>), dim3((Nr + RPB - 1) / RPB), dim3(fft4_threads<real>()), lds, s, psi,
+    hipLaunchKernelGGL((fft4_rows<real, N1, N2, SIGN>), dim3((Nr + RPB - 1) / RPB), dim3(kFft4Threads), lds, s, psi,
                        (const Cx<real>*)tw, Nr);
     return hipGetLastError();
 }
@@ -554,7 +571,7 @@ static hipError_t launch_fft4_rows(typename Vec2<real>::type* psi, const void* t
 template <typename real, int N1, int N2, int MODE>
 static hipError_t launch_fft4_cols_t(typename Vec2<real>::type* psi, const void* tw, const real* mask, const void* rowf,
                                      const void* colf, int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec) {
-    constexpr int T = N1 > N2 ? N1 : N2, CB = kFft4ColThreads / (2 * T);
+    constexpr int T = N1 > N2 ? N1 : N2, CB = kFft4Threads / (2 * T);
     if (Nc % CB) return hipErrorInvalidValue;
     const size_t lds = (size_t)CB * (N1 * N2 + fft4_col_skew<real>()) * sizeof(Cx<real>);
     static const hipError_t attr = lds > 65536 ? hipFuncSetAttribute((const void*)&fft4_cols<real, N1, N2, CB, MODE>,
@@ -797,12 +814,9 @@ template <typename real, int MODE>
 static hipError_t fft_roundtrip(typename Vec2<real>::type* psi, const void* tw_c, const void* tw_r, const real* mask, const void* rowf,
                                 const void* colf, int Nc, int Nr, hipStream_t s, typename Vec2<real>::type* spec = nullptr) {
     int n1, n2;
-    if (fft4_enabled() && fft4_split(Nc, n1, n2) && fft4_split(Nr, n1, n2) && Nc % (kFft4ColThreads / 32) == 0) {
-        /* FP32 rows stay on the Stockham kernel: its 4 K waves per 1024 rows (four elements a thread)
-           beat the four-step row kernel's 512 at 1024^2 (6.95 / 9.88 vs 7.88 / 10.14 us forward /
-           inverse); in FP64 the four-step rows are faster (9.05 / 13.81 vs 10.0 / 15.7 us).  Both
-           leave each row transformed in natural order, in place. */
-        constexpr bool rows4 = sizeof(real) == 8;
+    if (fft4_enabled() && fft4_split(Nc, n1, n2) && fft4_split(Nr, n1, n2) && Nc % (kFft4Threads / 32) == 0) {
+        /* both row kernels leave each row transformed in natural order, in place */
+        const bool rows4 = fft4_rows_enabled();
         hipError_t e = rows4 ? launch_fft4_rows<real, -1>(psi, tw_c, Nc, Nr, s) : launch_rows<real, -1>(psi, tw_c, Nc, Nr, s);
         if (e == hipSuccess) e = launch_fft4_cols<real, MODE>(psi, tw_r, mask, rowf, colf, Nc, Nr, s, spec);
         if (e != hipSuccess || MODE == kColForward) return e;
