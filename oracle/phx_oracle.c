@@ -1418,7 +1418,7 @@ static int ssfm_linear(const orc_params* p, double* f, double dt) {
 int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
                  const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
                  double dt, int steps, int64_t step0) {
-    if (p->complex_reservoir) return -4; /* the split-step iterator keeps a real reservoir here */
+    if (p->complex_reservoir) return -4; /* the split-step iterator keeps a real reservoir here (see below) */
     const size_t N = (size_t)p->N_c * p->N_r;
     const int nc = p->tetm ? 2 : 1;
     double* psi[2] = {psi_plus, psi_minus};
@@ -1509,6 +1509,13 @@ int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, doubl
                 psi[k][2 * i + 1] += ri;
             }
             if (p->use_reservoir && !p->tetm) n[0][i] = nn[0][i];
+        }
+        /* normalizeImaginaryTimePropagation after every iterate (solver.cu:55-56): the split-step step
+           itself takes no -i dt (split_step_fourier_method.cu:25-71), so psi and the real reservoir are
+           only rescaled */
+        if (p->imag_time_amplitude != 0.0) {
+            orc_imag_normalize(p, psi[0], nc > 1 ? psi[1] : NULL);
+            if (p->use_reservoir) orc_imag_normalize_reservoir(p, n[0], nc > 1 ? n[1] : NULL);
         }
     }
     free(xi);

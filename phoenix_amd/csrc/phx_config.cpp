@@ -610,8 +610,9 @@ std::string finalize_config(SystemConfig& c) {
         c.halo = phx::rk_tableau_stages(c.adaptive ? found->adaptive : found->fixed);
     }
     /* imaginary time turns the reference's (complex-stored) reservoir complex: n advances by -i dt
-       (runge_sum_to_input_k with complex_dt, cuda_macro.cuh:160-178) */
-    c.complex_reservoir = c.imag_time_amplitude != 0.0 && c.use_reservoir;
+       (runge_sum_to_input_k with complex_dt, cuda_macro.cuh:160-178).  The split-step step takes no
+       -i dt (split_step_fourier_method.cu:25-71): there n stays real and is only normalised. */
+    c.complex_reservoir = c.imag_time_amplitude != 0.0 && c.use_reservoir && c.iterator != "SSFM";
     /* calculateAuto (system.cpp:116-160) */
     if (c.h_bar_s < 0) c.h_bar_s = c.h_bar / c.e_e * 1E12;
     if (c.m_eff < 0) c.m_eff = c.m_e / c.e_e * 1E8;
@@ -647,8 +648,8 @@ std::string finalize_config(SystemConfig& c) {
             if (c.pump.groups[g].kind & TMP_IEXP) c.complex_reservoir = true;
     /* the split-step iterator here keeps a real reservoir */
     if (c.complex_reservoir && c.iterator == "SSFM")
-        return "the split-step iterator with a complex reservoir (--imagTime or a complex temporal pump envelope with a "
-               "reservoir) is not provided by this library";
+        return "the split-step iterator with a complex temporal pump envelope and a reservoir (a complex reservoir) is "
+               "not provided by this library";
     return "";
 }
 

@@ -816,12 +816,15 @@ int iterate_t(phx_solver* s, bool force_fixed) {
         PHX_HIP(phx::launch_imag_normalize<real>((v2*)s->psi[0][s->cur], c.tetm ? (v2*)s->psi[1][s->cur] : nullptr,
                                                  s->cells(), c.imag_time_amplitude, c.dV, s->norm_partials, s->norm_sums,
                                                  s->stream));
-        /* the reservoir to the same amplitude (solver_normalize_imag_time.cu:16-28); with a reservoir
-           imaginary time makes it complex, so it has the layout of psi */
-        if (c.use_reservoir)
-            PHX_HIP(phx::launch_imag_normalize<real>((v2*)s->n[0][s->cur], c.tetm ? (v2*)s->n[1][s->cur] : nullptr,
-                                                     s->cells(), c.imag_time_amplitude, c.dV, s->norm_partials, s->norm_sums,
-                                                     s->stream));
+        /* the reservoir to the same amplitude (solver_normalize_imag_time.cu:16-28).  With the Runge-Kutta
+           iterators imaginary time makes it complex (the layout of psi); the split-step one keeps it real,
+           normalised here as N_c N_r / 2 pairs of reals (N_c, N_r are even): the same sum of squares and the
+           same scale on every element */
+        if (c.use_reservoir) {
+            const size_t ncell = c.complex_reservoir ? s->cells() : s->cells() / 2;
+            PHX_HIP(phx::launch_imag_normalize<real>((v2*)s->n[0][s->cur], c.tetm ? (v2*)s->n[1][s->cur] : nullptr, ncell,
+                                                     c.imag_time_amplitude, c.dV, s->norm_partials, s->norm_sums, s->stream));
+        }
     }
     s->iteration++;
     if (s->rnd(s->t - s->fft_cached_t) < c.fft_every) return 1; /* FFT guard in the build's real (solver.cu:62-63) */

@@ -83,12 +83,13 @@ def test_envelope_flags_enable_reservoir_and_pump():
 
 
 def test_complex_reservoir_configurations():
-    """n is complex under a complex temporal pump amplitude or --imagTime with a reservoir; a real
-    pump envelope or imaginary time without a reservoir keep it real (the staged path that serves it is
+    """n is complex under a complex temporal pump amplitude or --imagTime with a reservoir (Runge-Kutta
+    iterators); a real pump envelope, imaginary time without a reservoir or the split-step iterator keep it real (the staged path that serves it is
     chosen at phx_initialize, on the device)."""
     pump = ["--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss"]
     cases = [(pump + ["time", "iexp", 0, 1, 1], 1), (["--imagTime", 1.0] + pump, 1), (pump, 0),
-             (pump + ["time", "cos", 0, 1, 1], 0), (["--imagTime", 1.0], 0)]
+             (pump + ["time", "cos", 0, 1, 1], 0), (["--imagTime", 1.0], 0),
+             (["-ssfm", "--imagTime", 1.0] + pump, 0)]  # the split-step step takes no -i dt: n stays real
     for args, want in cases:
         i = make(["--N", 32, 32] + args).info
         assert i["complex_reservoir"] == want, args
@@ -103,7 +104,6 @@ def test_complex_reservoir_configurations():
     # a complex reservoir on the split-step iterator (the RK iterators take it on the staged path)
     (["-ssfm", "--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss", "time", "iexp", 0, 1, 1], -3),
     (["--iterator", "NoSuchKey"], -3),    # unknown key (solver_factory.hpp:59)
-    (["-ssfm", "--imagTime", 1.0, "--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss"], -3),
     (["--N", "abc", 32], -1),
     (["--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "notatype"], -1),
 ])

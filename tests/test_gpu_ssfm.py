@@ -74,6 +74,50 @@ def test_ssfm_matches_oracle(Nc, Nr, tetm, noise, prec):
     assert max(errs.values()) <= TOL[prec]
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("tetm", [False, True])
+def test_ssfm_imaginary_time_with_reservoir_matches_oracle(prec, tetm):
+    """--imagTime with the split-step iterator: the step takes no -i dt, then psi and the (real)
+    reservoir are rescaled to the amplitude after every step (solver_normalize_imag_time.cu:13-45)."""
+    from phoenix_amd._lib import Solver
+    Nc, Nr, L, dt, steps, amp = 64, 48, 31.5, 0.02, 25, 2.0
+    Ly = L * (Nr - 1) / (Nc - 1)
+    say(f"[ssfm imaginary time] tetm={tetm} prec={prec}")
+    dx = L / (Nc - 1)
+    X, Y = np.meshgrid(-L / 2 + dx * np.arange(Nc), -Ly / 2 + dx * np.arange(Nr))
+    P = 1.5 * PTH * np.exp(-0.5 * (X ** 2 + Y ** 2) / 6.0 ** 2)
+    rng = np.random.default_rng(5)
+    init = {"psi_plus": 0.5 * (rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc))),
+            "n_plus": rng.random((Nr, Nc)), "pump_plus": P}
+    if tetm:
+        init.update(psi_minus=0.5 * (rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc))),
+                    n_minus=rng.random((Nr, Nc)), pump_minus=0.6 * P)
+    args = ["--N", Nc, Nr, "--L", L, Ly, "--boundary", "periodic", "periodic", "--outEvery", 1e9, "--fftEvery", 1e9,
+            "--pump", 1.0, "add", 10, 10, 0, 0, "both", 1, "none", "gauss+noDivide", "--iterator", "SSFM",
+            "--imagTime", amp, "--tstep", dt, "--tmax", (steps - 0.5) * dt] + (["-tetm"] if tetm else [])
+    s = Solver([str(a) for a in args], prec)
+    assert s.info["complex_reservoir"] == 0
+    for k, v in init.items():
+        s.set(k, v)
+    done = s.step(steps + 5)
+    out = {k: s.get(k).astype(np.complex128 if k.startswith("psi") else np.float64) for k in init
+           if k.startswith(("psi", "n_"))}
+    s.close()
+    assert done == steps
+    p = O.params(Nc, Nr, L, Ly, tetm=tetm, pump=True)
+    p.imag_time_amplitude = amp
+    kw = dict(pump=P)
+    if tetm:
+        kw.update(psi_minus=init["psi_minus"], n_minus=init["n_minus"], pump_minus=init["pump_minus"])
+    o = O.ssfm_run(p, init["psi_plus"], init["n_plus"], dt=dt, steps=steps, **kw)
+    ref = dict(zip(["psi_plus", "n_plus", "psi_minus", "n_minus"], o))
+    errs = {k: rel_l2(out[k], ref[k]) for k in out}
+    say("  " + " ".join(f"{k} {e:.2e}" for k, e in errs.items()))
+    assert max(errs.values()) <= TOL[prec]
+    for k in out:
+        assert abs(np.sum(np.abs(out[k]) ** 2) * dx * dx - amp) < (1e-9 if prec == "fp64" else 1e-3) * amp
+
+
 def _env(Nc, Nr, L, amp, w, x, y, charge=0.0):
     dx = L / (Nc - 1)
     Ly = L * (Nr - 1) / (Nc - 1)

@@ -47,6 +47,33 @@ def test_uniform_field_closed_form():
     assert np.abs(n - y).max() < 1e-13 * abs(y)
 
 
+def test_uniform_field_imaginary_time_is_the_real_map_then_normalisation():
+    """--imagTime with the split-step iterator: the step takes no -i dt (split_step_fourier_method.cu:25-71),
+    then psi and the (real) reservoir are each rescaled to sum |f|^2 dV = amp (solver_normalize_imag_time.cu)."""
+    N, L, dt, steps, amp = 16, 7.5, 0.03, 6, 2.5
+    p = O.params(N, N, L, L, pump=True, pump_amp=1.3)
+    p.imag_time_amplitude = amp
+    psi0, n0, P = 0.4 - 0.2j, 0.8, 3.0
+    out, n = O.ssfm_run(p, np.full((N, N), psi0), np.full((N, N), n0), pump=np.full((N, N), P), dt=dt, steps=steps)
+    assert n.dtype == np.float64
+
+    def run(normalise_n):
+        x, y = psi0, n0
+        for _ in range(steps):
+            rho = abs(x) ** 2
+            A = p.g_c * rho + p.g_r * y
+            x_new = x * np.exp(dt * (p.R * y - p.gamma_c) / 2) * np.exp(-1j * dt * A / p.h_bar_s)
+            y = y + dt * (-(p.gamma_r + p.R * rho) * y + P * 1.3)
+            x = x_new * np.sqrt(amp / (N * N * abs(x_new) ** 2 * p.dV))
+            if normalise_n:
+                y = y * np.sqrt(amp / (N * N * y ** 2 * p.dV))
+        return x, y
+    x, y = run(True)
+    assert np.abs(out - x).max() < 1e-13 * abs(x)
+    assert np.abs(n - y).max() < 1e-13 * abs(y)
+    assert abs(run(False)[1] - y) > 1e-3 * abs(y)  # the reservoir's normalisation matters
+
+
 def test_linear_problem_is_strang_splitting():
     Nc, Nr, L, dt, steps = 32, 16, 15.5, 0.04, 5
     Ly = L * (Nr - 1) / (Nc - 1)
