@@ -169,6 +169,8 @@ typedef struct {
     int32_t precision;                /* significant digits, -1: stream default */
     double L_x, L_y;
     char filetype[8];                 /* --outputFileType: "txt", "npy", "ash" or "mat" */
+    char name[128];                   /* --name: every output file is <path><name>_<file> (empty: <path><file>) */
+    int32_t verbose_io;               /* -verboseIO: report each written matrix */
 } phx_output_config;
 int phx_get_output_config(phx_solver* s, phx_output_config* oc);
 

@@ -552,10 +552,13 @@ std::string parse_config(SystemConfig& c, const std::vector<std::string>& argv_i
     if ((index = A.find({"me", "m_e", "electron_mass"}, 0, "--")) != -1) c.m_e = A.num("me", ++index);
     if ((index = A.find({"hbarscaled", "hbar_scaled", "hbars"}, 0, "--")) != -1) c.h_bar_s = A.num("hbars", ++index);
     if ((index = A.find({"meff", "m_eff"}, 0, "--")) != -1) c.m_eff = A.num("m_eff", ++index);
-    if ((index = A.find("--path")) != -1) {
+    /* output directory, file-name prefix, per-file log (filehandler.cpp:32-74) */
+    if ((index = A.find({"path", "outputPath"}, 0, "--")) != -1) {
         c.path = A.str(++index);
         if (!c.path.empty() && c.path.back() != '/') c.path += "/";
     }
+    if ((index = A.find("--name")) != -1) c.output_name = A.str(++index);
+    if (A.find("-verboseIO") != -1) c.verbose_io = true;
 #define PHX_X(name, def) \
     if ((index = A.find("--" #name)) != -1) c.name = A.num(#name, ++index);
     PHX_CUSTOM_PARAMETERS(PHX_X)

@@ -94,6 +94,8 @@ struct SystemConfig {
     double history_start = 0.0;
     int fileout_precision = -1;
     std::string output_filetype = "txt"; /* --outputFileType txt|npy|ash|mat (filehandler.cpp:49-69) */
+    std::string output_name;             /* --name: prefix "<name>_" of every output file (filehandler.cpp:40-41,461-463) */
+    bool verbose_io = false;             /* -verboseIO: one line per written matrix (filehandler.cpp:72-74) */
     /* custom parameters (custom_rhs.h) */
 #define PHX_X(name, def) double name = def;
     PHX_CUSTOM_PARAMETERS(PHX_X)

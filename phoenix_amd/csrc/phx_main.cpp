@@ -17,6 +17,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <initializer_list>
 #include <map>
 #include <sstream>
@@ -35,6 +36,7 @@ namespace {
 struct Out {
     phx_output_config oc;
     std::string path;  /* --path, ending in '/' */
+    std::string prefix; /* "<--name>_" or empty: every file is path + prefix + name (filehandler.cpp:461-463) */
     std::string ftype; /* txt, npy, ash, mat */
     bool fp32 = false; /* binary files hold the build's precision */
     phx_info in;
@@ -121,11 +123,16 @@ void write_binary(FILE* f, const std::string& ftype, const std::vector<real>& re
     std::fwrite(il.data(), sizeof(real), il.size(), f);
 }
 
+/* -verboseIO: one line per written matrix (filehandler.cpp:309-312) */
+void log_write(const Out& o, const std::string& file, size_t elements) {
+    if (o.oc.verbose_io) std::printf("Output %zu elements to '%s'.\n", elements, file.c_str());
+}
+
 /* outputMatrixToFile (filehandler.cpp:233-398): the region [x0, x1) x [y0, y1), every inc-th cell;
    txt: header, then the real parts and (complex matrices) the imaginary parts (filehandler.cpp:282-307) */
 void write_matrix(const Out& o, const std::string& name, const std::vector<double>& z, int x0, int x1, int y0, int y1,
                   int inc, const std::string& head, bool cplx = true) {
-    const std::string file = o.path + name + "." + o.ftype;
+    const std::string file = o.path + o.prefix + name + "." + o.ftype;
     FILE* f = std::fopen(file.c_str(), o.ftype == "txt" ? "w" : "wb");
     if (!f) {
         std::fprintf(stderr, "Unable to write '%s'\n", file.c_str());
@@ -146,6 +153,7 @@ void write_matrix(const Out& o, const std::string& name, const std::vector<doubl
         else
             write_binary<double>(f, o.ftype, re, im, cplx, rows, cols);
         std::fclose(f);
+        log_write(o, file, (size_t)rows * cols);
         return;
     }
     std::fputs(head.c_str(), f);
@@ -157,6 +165,7 @@ void write_matrix(const Out& o, const std::string& name, const std::vector<doubl
             std::fputs(line.c_str(), f);
         }
     std::fclose(f);
+    log_write(o, file, (size_t)((y1 - y0 + inc - 1) / inc) * ((x1 - x0 + inc - 1) / inc));
 }
 
 void write_full(const Out& o, const std::string& name, const std::vector<double>& z, bool cplx, double t, double t0 = 0,
@@ -307,7 +316,7 @@ void cache_matrices(phx_solver* s, Out& o) {
         return;
     }
     o.history_counter = 1;
-    mkdir((o.path + "timeoutput").c_str(), 0755);
+    mkdir((o.path + o.prefix + "timeoutput").c_str(), 0755);
     char sfx[64];
     std::snprintf(sfx, sizeof sfx, "_%f", o.in.t); /* std::to_string of the time */
     output_matrices<real>(s, o, o.oc.hx0, o.oc.hx1, o.oc.hy0, o.oc.hy1, o.oc.hinc, sfx, "timeoutput/");
@@ -316,7 +325,7 @@ void cache_matrices(phx_solver* s, Out& o) {
 /* Solver::cacheToFiles (solver_cache.cu:42-55): scalar.txt, one row per output interval */
 void cache_to_files(const Out& o) {
     if (!o.want({"all", "max", "scalar"})) return;
-    FILE* f = std::fopen((o.path + "scalar.txt").c_str(), "w");
+    FILE* f = std::fopen((o.path + o.prefix + "scalar.txt").c_str(), "w");
     if (!f) return;
     std::fprintf(f, "index ");
     for (const auto& kv : o.scalars) std::fprintf(f, "%s ", kv.first.c_str());
@@ -356,6 +365,7 @@ int run(phx_solver* s) {
     o.path = o.oc.path;
     if (o.path.empty()) o.path = "data/";
     if (o.path.back() != '/') o.path += "/";
+    o.prefix = o.oc.name[0] ? std::string(o.oc.name) + "_" : std::string();
     mkdir(o.path.c_str(), 0755);
     output_initial<real>(s, o);
 
@@ -388,9 +398,84 @@ int run(phx_solver* s) {
     return 0;
 }
 
+/* --help / -h, --helpVerbose / -hv, with --markdown / -mk as a table (system.cpp:162-170: print, exit 0) */
+struct FlagHelp {
+    const char* flags;
+    const char* args;
+    const char* what;
+    bool verbose_only;
+};
+const FlagHelp kFlagHelp[] = {
+    {"--N, --gridsize", "N_c N_r", "grid points (even)", false},
+    {"--L, --gridlength, --xmax", "L_x L_y", "domain lengths in um; dx = L / (N - 1)", false},
+    {"--boundary", "bx by", "periodic or zero, per axis", false},
+    {"--tmax, --tend", "t", "end time in ps", false},
+    {"--tstep, --dt", "dt", "time step in ps (default: 0.5 dx dy / (m_e / e 1e8 / m_eff))", false},
+    {"--t0", "t", "start time", true},
+    {"--iterator", "key", "RK4 (default), DP45, Heun2, BoSha, CashKarp, Fehlberg2/5, NSRK78, DP85, ..., SSFM", false},
+    {"-ssfm", "", "the split-step Fourier iterator", true},
+    {"-adaptive, --adaptiveTimestep", "", "adaptive step size for iterators that have one", false},
+    {"--tol", "tolerance", "adaptive error tolerance", false},
+    {"--rkvdt", "dt_min dt_max", "adaptive step bounds", false},
+    {"--imagTime", "amplitude", "imaginary-time propagation, fields normalised to sum |f|^2 dV = amplitude", false},
+    {"-tetm", "", "two-component (TE-TM) spinor model", false},
+    {"-noReservoir", "", "no reservoir even with a pump", true},
+    {"--pump, --potential, --pulse", "envelope", "sources (several allowed; see the envelope syntax)", false},
+    {"--fftMask", "envelope", "k-space filter mask, applied every --fftEvery", false},
+    {"--initialState, --initState, --iS", "envelope", "initial wavefunction", false},
+    {"--initialReservoir, --initReservoir, --iR", "envelope", "initial reservoir", true},
+    {"--fftEvery", "dt", "interval of the k-space filter (and of the spectra output)", false},
+    {"--dw", "amplitude", "stochastic noise amplitude", false},
+    {"--initRandom", "amplitude seed", "random initial wavefunction; seed of the noise", false},
+    {"--gammaC, --gammaR, --gc, --gr, --R, --gpm, --deltaLT", "value", "model parameters", false},
+    {"--meff, --hbarscaled, --hbar, --e, --me", "value", "constants", true},
+    {"--path, --outputPath", "dir", "output directory (default data/)", false},
+    {"--name", "prefix", "prefix of every output file: <path><prefix>_<file>", false},
+    {"--output", "keys", "comma-separated output keys (default mat,scalar)", false},
+    {"--outEvery", "dt", "output interval", false},
+    {"--outputFileType", "type", "txt (default), npy, ash or mat", false},
+    {"--fileoutPrecision", "digits", "significant digits of text output", true},
+    {"--historyMatrix", "x0 x1 y0 y1 inc", "region of the matrix snapshots under timeoutput/", true},
+    {"--historyTime", "start every", "when snapshots are taken", true},
+    {"-verboseIO", "", "report each written matrix", true},
+    {"--config", "file", "read further flags from a file", false},
+    {"--subgrids, --sg, --threads, --blocksize", "...", "accepted for compatibility; one grid on one GPU here", true},
+    {"--help, -h / --helpVerbose, -hv", "", "this list (verbose: every flag and the envelope syntax); --markdown, -mk: as a table",
+     false},
+};
+
+bool has_flag(int argc, char** argv, std::initializer_list<const char*> names) {
+    for (int i = 1; i < argc; i++)
+        for (const char* n : names)
+            if (std::strcmp(argv[i], n) == 0) return true;
+    return false;
+}
+
+void print_help(bool verbose, bool markdown) {
+    std::printf(markdown ? "# PHOENIX (MI355X) command line\n\n| Flag | Arguments | Meaning |\n|---|---|---|\n"
+                         : "PHOENIX (MI355X) command line:\n\n");
+    for (const FlagHelp& f : kFlagHelp) {
+        if (f.verbose_only && !verbose) continue;
+        if (markdown)
+            std::printf("| `%s` | %s | %s |\n", f.flags, f.args, f.what);
+        else
+            std::printf("  %-44s %-18s %s\n", f.flags, f.args, f.what);
+    }
+    if (verbose)
+        std::printf("\nEnvelope syntax: amplitude behaviour width_x width_y pos_x pos_y polarization exponent charge type\n"
+                    "[time kind t0 sigma freq], or with named fields after amplitude and behaviour: width w_x w_y, pos x y,\n"
+                    "pol p, exponent e, charge m, type t, momenta k_x k_y, time kind t0 sigma freq; \"load file\" reads a\n"
+                    "matrix.  behaviour: add, multiply, replace, adaptive, complex; type: gauss, ring, noDivide, local,\n"
+                    "outerExponent, joined with '+'; time kinds: constant, cos, gauss, iexp, load.\n");
+}
+
 } // namespace
 
 int main(int argc, char** argv) {
+    if (has_flag(argc, argv, {"--help", "-h", "--helpVerbose", "-hv"})) {
+        print_help(has_flag(argc, argv, {"--helpVerbose", "-hv"}), has_flag(argc, argv, {"--markdown", "-mk"}));
+        return 0;
+    }
     phx_solver* s = nullptr;
     const int prec = PHX_MAIN_PRECISION;
     if (phx_create(&s, prec, argc, (const char* const*)argv) != PHX_OK) {

@@ -1410,6 +1410,8 @@ int phx_get_output_config(phx_solver* s, phx_output_config* oc) {
     oc->L_x = c.L_x;
     oc->L_y = c.L_y;
     std::snprintf(oc->filetype, sizeof(oc->filetype), "%s", c.output_filetype.c_str());
+    std::snprintf(oc->name, sizeof(oc->name), "%s", c.output_name.c_str());
+    oc->verbose_io = c.verbose_io;
     return PHX_OK;
 }
 

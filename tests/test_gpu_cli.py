@@ -168,6 +168,27 @@ def test_executable_writes_the_reference_outputs(tmp_path):
     assert h.shape == (8, 8) and np.abs(h - psi[0:16:2, 0:16:2]).max() < 1e-12 * np.abs(psi).max()
 
 
+def test_executable_name_prefix_and_verbose_io(tmp_path):
+    """--name X: every output file is <path>X_<file>, the history directory X_timeoutput/ included
+    (filehandler.cpp:40-41,461-463); -verboseIO reports each written matrix (filehandler.cpp:309-312)."""
+    exe = os.path.join(BIN, "PHOENIX_gpu_fp64")
+    out = tmp_path / "out"
+    args = ["--N", 32, 32, "--L", 15.5, 15.5, "--tstep", 0.04, "--tmax", 0.4, "--outEvery", 0.2, "--fftEvery", 1e9,
+            "--pump", 5.0, "add", 8, 8, 0, 0, "both", 1, "none", "gauss+noDivide", "--historyMatrix", 0, 8, 0, 8, 1,
+            "--outputPath", str(out), "--name", "run7", "-verboseIO"]
+    say("[cli --name / -verboseIO] PHOENIX_gpu_fp64 32x32")
+    r = subprocess.run([exe] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    names = sorted(os.listdir(out))
+    say("  files: " + " ".join(names))
+    for f in ("run7_initial_wavefunction_plus.txt", "run7_wavefunction_plus.txt", "run7_reservoir_plus.txt",
+              "run7_scalar.txt", "run7_timeoutput"):
+        assert f in names, f
+    assert not any(n.startswith(("wavefunction", "scalar")) for n in names)
+    assert f"Output 1024 elements to '{out}/run7_wavefunction_plus.txt'." in r.stdout
+    assert os.listdir(out / "run7_timeoutput")
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_executable_binary_output_types(prec, tmp_path):
     """--outputFileType npy / ash / mat (filehandler.cpp:49-69, simple_binary.hpp): the same matrices
