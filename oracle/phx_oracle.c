@@ -1418,9 +1418,10 @@ static int ssfm_linear(const orc_params* p, double* f, double dt) {
 int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, double* n_plus, double* n_minus,
                  const double* pump_plus, const double* pump_minus, const double* pot_plus, const double* pot_minus,
                  double dt, int steps, int64_t step0) {
-    if (p->complex_reservoir) return -4; /* the split-step iterator keeps a real reservoir here (see below) */
     const size_t N = (size_t)p->N_c * p->N_r;
     const int nc = p->tetm ? 2 : 1;
+    /* complex reservoir: n held as {re, im} (a complex temporal pump amplitude enters it whole) */
+    const int cres = p->complex_reservoir && p->use_reservoir, nel = cres ? 2 : 1;
     double* psi[2] = {psi_plus, psi_minus};
     double* n[2] = {n_plus, n_minus};
     const double* pump[2] = {pump_plus, pump_minus};
@@ -1428,7 +1429,7 @@ int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, doubl
     const int stoch = p->use_stochastic && p->stochastic_amplitude > 0.0;
     double* nn[2] = {0, 0};
     double* xi = calloc(2 * N, sizeof(double));
-    for (int k = 0; k < nc; k++) nn[k] = calloc(N, sizeof(double));
+    for (int k = 0; k < nc; k++) nn[k] = calloc(nel * N, sizeof(double));
     for (int st = 0; st < steps; st++) {
         /* per-step noise: amplitude * sqrt(dt) * N2(seed, step, cell) (solver.cu:36-45) */
         if (stoch)
@@ -1442,10 +1443,11 @@ int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, doubl
             if (ssfm_linear(p, psi[k], dt)) return -1;
         /* gp_scalar_nonlinear (kernel_gp_compute.cuh:264-382) */
         for (size_t i = 0; i < N; i++) {
-            double nrm[2] = {0, 0}, nv[2] = {0, 0};
+            double nrm[2] = {0, 0}, nv[2] = {0, 0}, nvi[2] = {0, 0};
             for (int k = 0; k < nc; k++) {
                 nrm[k] = psi[k][2 * i] * psi[k][2 * i] + psi[k][2 * i + 1] * psi[k][2 * i + 1];
-                nv[k] = p->use_reservoir ? n[k][i] : 0.0;
+                nv[k] = p->use_reservoir ? n[k][nel * i] : 0.0;
+                nvi[k] = cres ? n[k][2 * i + 1] : 0.0;
             }
             for (int k = 0; k < nc; k++) {
                 /* result = (g_c |psi|^2, -hbar/2 gamma_c) + V a + g_r n + i hbar/2 R n (+ g_pm |psi_o|^2) (- g_c/dV) */
@@ -1464,6 +1466,11 @@ int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, doubl
                 if (p->tetm) A += p->g_pm * nrm[1 - k];
                 if (stoch) A -= p->g_c / p->dV;
                 B += p->h_bar_s * 0.5 * p->R * nv[k];
+                if (cres) {
+                    /* Im n in g_r n + i hbar/2 R n: A -= hbar/2 R Im n, B += g_r Im n */
+                    A -= p->h_bar_s * 0.5 * p->R * nvi[k];
+                    B += p->g_r * nvi[k];
+                }
                 /* psi * exp((B - i A) dt / hbar) */
                 const double er = B * p->one_over_h_bar_s * dt, ei = -A * p->one_over_h_bar_s * dt;
                 const double m = exp(er), fr = m * cos(ei), fi = m * sin(ei);
@@ -1471,15 +1478,23 @@ int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, doubl
                 psi[k][2 * i] = pr * fr - pi * fi;
                 psi[k][2 * i + 1] = pr * fi + pi * fr;
                 if (p->use_reservoir) {
-                    double res = -p->gamma_r * nv[k];
+                    double res = -p->gamma_r * nv[k], resi = -p->gamma_r * nvi[k];
                     res -= p->R * nrm[k] * nv[k];
+                    resi -= p->R * nrm[k] * nvi[k];
                     if (p->src_groups[0] > 0) {
-                        for (int g = 0; g < p->src_groups[0]; g++) res += p->src_field[0][g][k][i] * p->src_amp[0][g][0];
+                        for (int g = 0; g < p->src_groups[0]; g++) {
+                            res += p->src_field[0][g][k][i] * p->src_amp[0][g][0];
+                            resi += p->src_field[0][g][k][i] * p->src_amp[0][g][1];
+                        }
                     } else if (p->use_pump) {
                         res += pump[k][i] * p->pump_amp;
                     }
-                    if (stoch) res += p->R * nv[k] / p->dV;
-                    nn[k][i] = nv[k] + res * dt;
+                    if (stoch) {
+                        res += p->R * nv[k] / p->dV;
+                        resi += p->R * nvi[k] / p->dV;
+                    }
+                    nn[k][nel * i] = nv[k] + res * dt;
+                    if (cres) nn[k][2 * i + 1] = nvi[k] + resi * dt;
                 }
             }
         }
@@ -1498,7 +1513,14 @@ int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, doubl
                     rr += p->tetm ? zi : zr;
                     ri += p->tetm ? -zr : zi;
                 }
-                if (stoch) {
+                if (stoch && cres) {
+                    /* complex reservoir: principal square root, complex product with xi */
+                    const double* nsrc = p->tetm ? nn[k] : n[0];
+                    const double complex z = ((p->R * nsrc[2 * i] + p->gamma_c) + I * (p->R * nsrc[2 * i + 1])) / (4.0 * p->dV);
+                    const double complex sq = csqrt(z);
+                    rr += xi[2 * i] * creal(sq) - xi[2 * i + 1] * cimag(sq);
+                    ri += xi[2 * i] * cimag(sq) + xi[2 * i + 1] * creal(sq);
+                } else if (stoch) {
                     /* scalar: the reservoir of the step start; TE-TM: the new one of the component */
                     const double nr = p->use_reservoir ? (p->tetm ? nn[k][i] : n[0][i]) : 0.0;
                     const double sg = sqrt((p->R * nr + p->gamma_c) / (4.0 * p->dV));
@@ -1508,7 +1530,8 @@ int orc_ssfm_run(const orc_params* p, double* psi_plus, double* psi_minus, doubl
                 psi[k][2 * i] += rr;
                 psi[k][2 * i + 1] += ri;
             }
-            if (p->use_reservoir && !p->tetm) n[0][i] = nn[0][i];
+            if (p->use_reservoir && !p->tetm)
+                for (int e = 0; e < nel; e++) n[0][nel * i + e] = nn[0][nel * i + e];
         }
         /* normalizeImaginaryTimePropagation after every iterate (solver.cu:55-56): the split-step step
            itself takes no -i dt (split_step_fourier_method.cu:25-71), so psi and the real reservoir are

@@ -649,10 +649,6 @@ std::string finalize_config(SystemConfig& c) {
     if (c.use_pumps && c.use_reservoir)
         for (int g = 0; g < c.pump.groupSize(); g++)
             if (c.pump.groups[g].kind & TMP_IEXP) c.complex_reservoir = true;
-    /* the split-step iterator here keeps a real reservoir */
-    if (c.complex_reservoir && c.iterator == "SSFM")
-        return "the split-step iterator with a complex temporal pump envelope and a reservoir (a complex reservoir) is "
-               "not provided by this library";
     return "";
 }
 

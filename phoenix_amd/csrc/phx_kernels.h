@@ -307,11 +307,12 @@ struct SsfmArgs {
     const real* pump[2][kStagedMaxGroups];
     const real* pot[2][kStagedMaxGroups];
     const typename Vec2<real>::type* pulse[2][kStagedMaxGroups];
-    real pamp[kStagedMaxGroups];
+    real pamp[kStagedMaxGroups], pamp_im[kStagedMaxGroups]; /* pump amplitudes; Im only with cres */
     typename Vec2<real>::type vamp[kStagedMaxGroups], lamp[kStagedMaxGroups];
     const typename Vec2<real>::type* noise; /* this step's noise field (stochastic only) */
     size_t cells;
     int tetm, stochastic;
+    int cres; /* complex reservoir: n_cur / n_nxt hold {re, im} cells */
     real dt;
     PhxPhys<real> ph;
 };

@@ -40,6 +40,26 @@ __host__ __device__ __forceinline__ float phx_log(float x) { return logf(x); }
 __host__ __device__ __forceinline__ double phx_log(double x) { return log(x); }
 __host__ __device__ __forceinline__ float phx_sqrt(float x) { return sqrtf(x); }
 __host__ __device__ __forceinline__ double phx_sqrt(double x) { return sqrt(x); }
+
+/* principal square root of x + i y: u = sqrt((|z| + |x|) / 2), then (u, y / 2u) for x >= 0 and
+   (|y| / 2u, u sign y) for x < 0 (no cancellation in either branch) */
+template <typename real>
+__device__ __forceinline__ void csqrt_principal(real x, real y, real& zr, real& zi) {
+    const real r = hypot(x, y);
+    if (r == real(0)) {
+        zr = real(0);
+        zi = real(0);
+        return;
+    }
+    const real u = phx_sqrt((r + fabs(x)) * real(0.5));
+    if (x >= real(0)) {
+        zr = u;
+        zi = y / (real(2) * u);
+    } else {
+        zr = fabs(y) / (real(2) * u);
+        zi = copysign(u, y);
+    }
+}
 __host__ __device__ __forceinline__ void phx_sincos(float x, float& s, float& c) {
     s = sinf(x);
     c = cosf(x);

@@ -643,6 +643,10 @@ int ssfm_step(phx_solver* s, double t_pre) {
             a.pamp[g] = (real)c.pump.temporal(g, t_pre).real();
         else
             stage_amps<real>(c.pump, t_pre, s->dt, w0, 1, &a.pamp[g]);
+        /* a complex reservoir takes Im a_g of the group (of the single group when the fields are summed) */
+        a.pamp_im[g] = 0;
+        if (c.complex_reservoir && (s->st_npump || (c.pump.groupSize() == 1 && !c.pump.allConstant())))
+            a.pamp_im[g] = (real)c.pump.temporal(s->st_npump ? g : 0, t_pre).imag();
     }
     for (int g = 0; g < a.npot; g++) {
         if (s->st_npot) {
@@ -671,6 +675,7 @@ int ssfm_step(phx_solver* s, double t_pre) {
     a.cells = N;
     a.tetm = c.tetm;
     a.stochastic = c.use_stochastic;
+    a.cres = c.complex_reservoir;
     a.dt = (real)s->dt;
     a.ph = phx::make_phys<real>(c);
     for (int k = 0; k < ncomp; k++)

@@ -18,6 +18,10 @@
  *                       the TE-TM model takes the new reservoir of each component and keeps the old
  *                       one (the reference never copies it back, so its TE-TM reservoirs do not
  *                       evolve under SSFM).
+ * A complex reservoir (n as {re, im} cells, under a complex temporal pump amplitude) carries Im n
+ * through the same formulas: the exponent gains dt g_r Im n / hbar in its real part and
+ * dt R Im n / 2 in its phase, the reservoir step takes sum_g P_g a_g whole, and the noise
+ * amplitude is the principal complex square root of (R n + gamma_c) / (4 dV).
  * One thread per cell, coalesced; the fields are the solver's full-grid arrays.
  */
 #include <hip/hip_runtime.h>
@@ -36,19 +40,27 @@ __device__ __forceinline__ float phx_exp_dev(float x) { return expf(x); }
 __device__ __forceinline__ double phx_exp_dev(double x) { return exp(x); }
 } // namespace
 
-template <typename real, bool TETM>
+template <typename real, bool TETM, bool CRES>
 __global__ __launch_bounds__(256) void ssfm_nonlinear(SsfmArgs<real> a) {
+    using v2 = typename Vec2<real>::type;
     const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
     if (i >= a.cells) return;
     constexpr int NC = TETM ? 2 : 1;
     const PhxPhys<real>& ph = a.ph;
-    typename Vec2<real>::type psi[NC];
-    real nrm[NC], n[NC];
+    v2 psi[NC];
+    real nrm[NC], n[NC], ni[NC];
 #pragma unroll
     for (int k = 0; k < NC; k++) {
         psi[k] = a.psi[k][i];
         nrm[k] = phx_fma(psi[k].x, psi[k].x, psi[k].y * psi[k].y);
-        n[k] = a.n_cur[k] ? a.n_cur[k][i] : real(0);
+        if constexpr (CRES) {
+            const v2 z = reinterpret_cast<const v2*>(a.n_cur[k])[i];
+            n[k] = z.x;
+            ni[k] = z.y;
+        } else {
+            n[k] = a.n_cur[k] ? a.n_cur[k][i] : real(0);
+            ni[k] = real(0);
+        }
     }
 #pragma unroll
     for (int k = 0; k < NC; k++) {
@@ -61,13 +73,16 @@ __global__ __launch_bounds__(256) void ssfm_nonlinear(SsfmArgs<real> a) {
         A += ph.g_r * n[k];
         if (TETM) A += ph.g_pm * nrm[1 - k];
         if (a.stochastic) A -= ph.g_c / ph.dV;
+        /* Im n in g_r n + i hbar/2 R n moves g_r Im n into the gain and hbar/2 R Im n out of the phase */
+        if constexpr (CRES) B += ph.g_r * ni[k];
         const real er = phx_fma(a.dt * ph.one_over_h_bar_s, B, a.dt * (real(0.5) * (ph.R * n[k] - ph.gamma_c)));
-        const real ei = -(A * ph.one_over_h_bar_s * a.dt);
+        real ei = -(A * ph.one_over_h_bar_s * a.dt);
+        if constexpr (CRES) ei = phx_fma(a.dt * real(0.5) * ph.R, ni[k], ei);
         real sn, cs;
         phx_sincos_dev(ei, sn, cs);
         const real m = phx_exp_dev(er);
         const real fr = m * cs, fi = m * sn;
-        typename Vec2<real>::type o;
+        v2 o;
         o.x = phx_fma(psi[k].x, fr, -(psi[k].y * fi));
         o.y = phx_fma(psi[k].x, fi, psi[k].y * fr);
         a.psi[k][i] = o;
@@ -76,18 +91,29 @@ __global__ __launch_bounds__(256) void ssfm_nonlinear(SsfmArgs<real> a) {
             rn -= ph.R * nrm[k] * n[k];
             for (int g = 0; g < a.npump; g++) rn += a.pump[k][g][i] * a.pamp[g];
             if (a.stochastic) rn += ph.R * n[k] / ph.dV;
-            a.n_nxt[k][i] = phx_fma(rn, a.dt, n[k]);
+            if constexpr (CRES) {
+                real rni = -(ph.gamma_r * ni[k]);
+                rni -= ph.R * nrm[k] * ni[k];
+                for (int g = 0; g < a.npump; g++) rni += a.pump[k][g][i] * a.pamp_im[g];
+                if (a.stochastic) rni += ph.R * ni[k] / ph.dV;
+                v2 z;
+                z.x = phx_fma(rn, a.dt, n[k]);
+                z.y = phx_fma(rni, a.dt, ni[k]);
+                reinterpret_cast<v2*>(a.n_nxt[k])[i] = z;
+            } else {
+                a.n_nxt[k][i] = phx_fma(rn, a.dt, n[k]);
+            }
         }
     }
 }
 
-template <typename real, bool TETM>
+template <typename real, bool TETM, bool CRES>
 __global__ __launch_bounds__(256) void ssfm_independent(SsfmArgs<real> a) {
+    using v2 = typename Vec2<real>::type;
     const size_t i = (size_t)blockIdx.x * 256u + threadIdx.x;
     if (i >= a.cells) return;
     constexpr int NC = TETM ? 2 : 1;
     const PhxPhys<real>& ph = a.ph;
-    const real n_old = a.n_cur[0] ? a.n_cur[0][i] : real(0);
 #pragma unroll
     for (int k = 0; k < NC; k++) {
         if (a.npulse == 0 && !a.stochastic) continue;
@@ -107,11 +133,23 @@ __global__ __launch_bounds__(256) void ssfm_independent(SsfmArgs<real> a) {
             }
         }
         if (a.stochastic) {
-            const real nr = TETM ? (a.n_nxt[k] ? a.n_nxt[k][i] : real(0)) : n_old;
-            const real sg = phx_sqrt(phx_fma(ph.R, nr, ph.gamma_c) / (real(4) * ph.dV));
+            /* scalar: the reservoir of the step start; TE-TM: the new one of the component */
+            const real* nsrc = TETM ? a.n_nxt[k] : a.n_cur[0];
             const auto xi = a.noise[i];
-            dr = phx_fma(xi.x, sg, dr);
-            di = phx_fma(xi.y, sg, di);
+            if constexpr (CRES) {
+                /* principal complex square root of (R n + gamma_c) / (4 dV), complex product with xi */
+                const v2 z = reinterpret_cast<const v2*>(nsrc)[i];
+                const real d = real(4) * ph.dV;
+                real sr, si;
+                csqrt_principal<real>(phx_fma(ph.R, z.x, ph.gamma_c) / d, (ph.R * z.y) / d, sr, si);
+                dr += xi.x * sr - xi.y * si;
+                di += xi.x * si + xi.y * sr;
+            } else {
+                const real nr = nsrc ? nsrc[i] : real(0);
+                const real sg = phx_sqrt(phx_fma(ph.R, nr, ph.gamma_c) / (real(4) * ph.dV));
+                dr = phx_fma(xi.x, sg, dr);
+                di = phx_fma(xi.y, sg, di);
+            }
         }
         auto p = a.psi[k][i];
         p.x += dr;
@@ -119,26 +157,41 @@ __global__ __launch_bounds__(256) void ssfm_independent(SsfmArgs<real> a) {
         a.psi[k][i] = p;
     }
     /* scalar model: the new reservoir becomes current (TE-TM keeps the old one) */
-    if (!TETM && a.n_nxt[0]) a.n_cur[0][i] = a.n_nxt[0][i];
+    if (!TETM && a.n_nxt[0]) {
+        if constexpr (CRES)
+            reinterpret_cast<v2*>(a.n_cur[0])[i] = reinterpret_cast<const v2*>(a.n_nxt[0])[i];
+        else
+            a.n_cur[0][i] = a.n_nxt[0][i];
+    }
 }
 
 template <typename real>
 hipError_t launch_ssfm_nonlinear(const SsfmArgs<real>& a, hipStream_t s) {
     const dim3 grid((unsigned)((a.cells + 255) / 256));
-    if (a.tetm)
-        hipLaunchKernelGGL((ssfm_nonlinear<real, true>), grid, dim3(256), 0, s, a);
+    if (a.cres && !a.n_cur[0]) return hipErrorInvalidValue;
+    if (a.tetm && a.cres)
+        hipLaunchKernelGGL((ssfm_nonlinear<real, true, true>), grid, dim3(256), 0, s, a);
+    else if (a.tetm)
+        hipLaunchKernelGGL((ssfm_nonlinear<real, true, false>), grid, dim3(256), 0, s, a);
+    else if (a.cres)
+        hipLaunchKernelGGL((ssfm_nonlinear<real, false, true>), grid, dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((ssfm_nonlinear<real, false>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((ssfm_nonlinear<real, false, false>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename real>
 hipError_t launch_ssfm_independent(const SsfmArgs<real>& a, hipStream_t s) {
     const dim3 grid((unsigned)((a.cells + 255) / 256));
-    if (a.tetm)
-        hipLaunchKernelGGL((ssfm_independent<real, true>), grid, dim3(256), 0, s, a);
+    if (a.cres && (!a.n_cur[0] || !a.n_nxt[0] || (a.tetm && !a.n_nxt[1]))) return hipErrorInvalidValue;
+    if (a.tetm && a.cres)
+        hipLaunchKernelGGL((ssfm_independent<real, true, true>), grid, dim3(256), 0, s, a);
+    else if (a.tetm)
+        hipLaunchKernelGGL((ssfm_independent<real, true, false>), grid, dim3(256), 0, s, a);
+    else if (a.cres)
+        hipLaunchKernelGGL((ssfm_independent<real, false, true>), grid, dim3(256), 0, s, a);
     else
-        hipLaunchKernelGGL((ssfm_independent<real, false>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((ssfm_independent<real, false, false>), grid, dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

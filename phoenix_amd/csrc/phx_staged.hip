@@ -77,28 +77,6 @@ __device__ __forceinline__ real noise_sigma(const StagedArgs<real>& a, long gi) 
     return phx_sqrt((RES ? phx_fma(a.ph.R, n, a.ph.gamma_c) : a.ph.gamma_c) / (real(4) * a.ph.dV));
 }
 
-/* principal square root of x + i y: u = sqrt((|z| + |x|) / 2), then (u, y / 2u) for x >= 0 and
-   (|y| / 2u, u sign y) for x < 0 (no cancellation in either branch) */
-template <typename real>
-__device__ __forceinline__ typename Vec2<real>::type csqrt_principal(real x, real y) {
-    typename Vec2<real>::type z;
-    const real r = hypot(x, y);
-    if (r == real(0)) {
-        z.x = real(0);
-        z.y = real(0);
-        return z;
-    }
-    const real u = phx_sqrt((r + fabs(x)) * real(0.5));
-    if (x >= real(0)) {
-        z.x = u;
-        z.y = y / (real(2) * u);
-    } else {
-        z.x = fabs(y) / (real(2) * u);
-        z.y = copysign(u, y);
-    }
-    return z;
-}
-
 /* the noise increment w_sum xi s of a stage sum, s = sqrt((R n+ + gamma_c) / (4 dV)); with a complex
    reservoir s is the principal complex square root and the product complex */
 template <typename real, bool RES, bool CRES>
@@ -108,7 +86,8 @@ __device__ __forceinline__ void noise_inc(const StagedArgs<real>& a, long gi, re
     if constexpr (CRES) {
         const v2 n = reinterpret_cast<const v2*>(a.x_n[0])[gi];
         const real d = real(4) * a.ph.dV;
-        const v2 sq = csqrt_principal<real>(phx_fma(a.ph.R, n.x, a.ph.gamma_c) / d, (a.ph.R * n.y) / d);
+        v2 sq;
+        csqrt_principal<real>(phx_fma(a.ph.R, n.x, a.ph.gamma_c) / d, (a.ph.R * n.y) / d, sq.x, sq.y);
         nzr = xr * sq.x - xi * sq.y;
         nzi = xr * sq.y + xi * sq.x;
     } else {

@@ -83,13 +83,14 @@ def test_envelope_flags_enable_reservoir_and_pump():
 
 
 def test_complex_reservoir_configurations():
-    """n is complex under a complex temporal pump amplitude or --imagTime with a reservoir (Runge-Kutta
-    iterators); a real pump envelope, imaginary time without a reservoir or the split-step iterator keep it real (the staged path that serves it is
-    chosen at phx_initialize, on the device)."""
+    """n is complex under a complex temporal pump amplitude (every iterator) or --imagTime with a reservoir
+    (Runge-Kutta iterators); a real pump envelope, imaginary time without a reservoir, or imaginary time on the
+    split-step iterator keep it real (the path that serves it is chosen at phx_initialize, on the device)."""
     pump = ["--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss"]
     cases = [(pump + ["time", "iexp", 0, 1, 1], 1), (["--imagTime", 1.0] + pump, 1), (pump, 0),
              (pump + ["time", "cos", 0, 1, 1], 0), (["--imagTime", 1.0], 0),
-             (["-ssfm", "--imagTime", 1.0] + pump, 0)]  # the split-step step takes no -i dt: n stays real
+             (["-ssfm", "--imagTime", 1.0] + pump, 0),  # the split-step step takes no -i dt: n stays real
+             (["-ssfm"] + pump + ["time", "iexp", 0, 1, 1], 1)]
     for args, want in cases:
         i = make(["--N", 32, 32] + args).info
         assert i["complex_reservoir"] == want, args
@@ -101,8 +102,6 @@ def test_complex_reservoir_configurations():
     (["--N", 31, 32], -1),               # odd dimension
     (["--N", 32, 32, "--tstep", 0], -1),  # dt <= 0
     (["--N", 32, 32, "--tstep", 2, "--tmax", 1], -1),  # dt > t_max
-    # a complex reservoir on the split-step iterator (the RK iterators take it on the staged path)
-    (["-ssfm", "--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "gauss", "time", "iexp", 0, 1, 1], -3),
     (["--iterator", "NoSuchKey"], -3),    # unknown key (solver_factory.hpp:59)
     (["--N", "abc", 32], -1),
     (["--pump", 1, "add", 10, 10, 0, 0, "both", 1, "none", "notatype"], -1),

@@ -198,3 +198,68 @@ def test_ssfm_source_chains_and_pulse_match_oracle(prec, tetm):
     r0 = O.ssfm_run(p, init["psi_plus"], init["n_plus"], pump=P0, pot=V0, dt=dt, steps=steps,
                     psi_minus=init.get("psi_minus"), n_minus=init.get("n_minus"), pump_minus=pump_m, pot_minus=pot_m)
     assert rel_l2(r0[0], psi) > 1e-4
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("tetm,noise,imag", [(False, False, 0.0), (False, True, 0.0), (True, True, 0.0),
+                                             (False, False, 2.0)])
+def test_ssfm_complex_pump_envelope_matches_oracle(prec, tetm, noise, imag):
+    """SSFM with a pump of complex (iexp) temporal envelope: the reservoir is complex ({re, im} cells),
+    its Euler step takes P a whole, Im n enters the nonlinear exponent, the noise amplitude is the
+    principal complex square root; with --imagTime psi and the complex reservoir are normalised after
+    every step."""
+    from phoenix_amd._lib import Solver
+    Nc, Nr, L, dt, steps, seed = 64, 48, 31.5, 0.02, 25, 77
+    tp = (0.2, 0.4, 6.0)
+    Ly = L * (Nr - 1) / (Nc - 1)
+    say(f"[ssfm complex reservoir] tetm={tetm} noise={noise} imag={imag} prec={prec}")
+    P = 1.5 * PTH * np.real(_env(Nc, Nr, L, 1.0, 6.0, 0.0, 0.0))
+    rng = np.random.default_rng(91 + 2 * tetm + noise)
+    init = {"psi_plus": 0.3 * (rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc))),
+            "n_plus": rng.random((Nr, Nc)) + 0.2j * rng.standard_normal((Nr, Nc)), "pump_plus": P}
+    if tetm:
+        init.update(psi_minus=0.3 * (rng.standard_normal((Nr, Nc)) + 1j * rng.standard_normal((Nr, Nc))),
+                    n_minus=rng.random((Nr, Nc)) - 0.2j * rng.standard_normal((Nr, Nc)), pump_minus=0.5 * P)
+    args = ["--N", Nc, Nr, "--L", L, Ly, "--boundary", "periodic", "periodic", "--outEvery", 1e9, "--fftEvery", 1e9,
+            "--iterator", "SSFM", "--tstep", dt, "--tmax", (steps - 0.5) * dt,
+            "--pump", 1.0, "add", 6.0, 6.0, 0, 0, "both", 1, "none", "gauss+noDivide", "time", "iexp", *tp]
+    if tetm:
+        args += ["-tetm"]
+    if noise:
+        args += ["--dw", 0.05, "--initRandom", 0.0, seed]
+    if imag:
+        args += ["--imagTime", imag]
+    s = Solver([str(a) for a in args], prec)
+    info = s.info
+    assert info["complex_reservoir"] == 1 and info["iterator"] == "SSFM"
+    for k, v in init.items():
+        s.set(k, v)
+    done = s.step(steps + 5)
+    out = {k: s.get(k).astype(np.complex128) for k in init if k.startswith(("psi", "n_"))}
+    assert s.info["staged"] == 0
+    s.close()
+    assert done == steps
+    psi, n = init["psi_plus"], init["n_plus"]
+    pm, nm = init.get("psi_minus"), init.get("n_minus")
+    t = 0.0
+    for st in range(steps):
+        p = O.params(Nc, Nr, L, Ly, tetm=tetm, pump=True, stochastic=0.05 if noise else 0.0, noise_seed=seed)
+        p.complex_reservoir = 1
+        p.imag_time_amplitude = imag
+        O.set_chain(p, "pump", [(P, 0.5 * P if tetm else None, _t_iexp(t, *tp), 0.0)])
+        r = O.ssfm_run(p, psi, n, dt=dt, steps=1, psi_minus=pm, n_minus=nm, step0=st)
+        if tetm:
+            psi, n, pm, nm = r
+        else:
+            psi, n = r
+        t = t + dt
+    ref = {"psi_plus": psi, "n_plus": n, "psi_minus": pm, "n_minus": nm}
+    errs = {k: rel_l2(out[k], ref[k]) for k in out}
+    say("  " + " ".join(f"{k} {e:.2e}" for k, e in errs.items()))
+    assert max(errs.values()) <= TOL[prec]
+    if not tetm:  # the pump's phase reached the reservoir (TE-TM reservoirs do not evolve under SSFM)
+        assert np.abs(n.imag - init["n_plus"].imag).max() > 1e-3
+    if imag:
+        dV = (L / (Nc - 1)) ** 2
+        for k in out:
+            assert abs(np.sum(np.abs(out[k]) ** 2) * dV - imag) < (1e-9 if prec == "fp64" else 1e-3) * imag

@@ -13,6 +13,8 @@ Checks:
   - imaginary time: the same map with dt -> -i dt for psi and n, then both normalised;
   - noise: per-cell map with additive stage noise xi * sqrt_c((R n0 + gamma_c)/(4 dV));
   - TE-TM: a uniform spinor with g_pm and two complex reservoirs;
+  - split-step iterator: a uniform field follows the closed-form nonlinear map and complex Euler reservoir
+    step (gp_scalar_nonlinear with complex n and pump amplitude);
   - real inputs: the complex-reservoir path equals the real-reservoir path (already pinned);
   - planted slips (+i dt on n, the real part of n under the square root, a real pump amplitude) miss.
 """
@@ -199,3 +201,33 @@ def test_real_inputs_give_the_real_reservoir_result():
     assert np.abs(out[1][1].imag).max() == 0.0
     assert np.abs(out[1][0] - out[0][0]).max() <= 1e-14 * np.abs(out[0][0]).max()
     assert np.abs(out[1][1].real - out[0][1]).max() <= 1e-14 * np.abs(out[0][1]).max()
+
+
+def test_split_step_uniform_field_with_complex_pump_amplitude():
+    """Split-step iterator (gp_scalar_nonlinear, kernel_gp_compute.cuh:264-305) with a complex reservoir: a
+    uniform field is untouched by the linear half steps, so one step is
+        psi <- psi exp(dt (-(i / hbar) (g_c |psi|^2 + g_r n) + (R n - gamma_c) / 2)),
+        n <- n + dt (-(gamma_r + R |psi|^2) n + P a)
+    in complex arithmetic, a the pump amplitude of the step start."""
+    N, L, dt, steps = 16, 7.5, 0.03, 8
+    p = O.params(N, N, L, L, pump=True, g_c=6e-5, g_r=1e-4)
+    p.complex_reservoir = 1
+    P, a = 3.0, 0.6 + 0.8j
+    O.set_chain(p, "pump", [(np.full((N, N), P), None, a, a)])
+    psi0, n0 = 0.4 - 0.2j, 0.8 + 0.3j
+    out, n = O.ssfm_run(p, np.full((N, N), psi0), np.full((N, N), n0), dt=dt, steps=steps)
+    assert n.dtype == np.complex128
+
+    def run(amp):
+        x, y = psi0, n0
+        for _ in range(steps):
+            rho = abs(x) ** 2
+            x_new = x * np.exp(dt * (-1j / p.h_bar_s * (p.g_c * rho + p.g_r * y) + (p.R * y - p.gamma_c) / 2))
+            y = y + dt * (-(p.gamma_r + p.R * rho) * y + P * amp)
+            x = x_new
+        return x, y
+    x, y = run(a)
+    assert np.abs(out - x).max() < 1e-13 * abs(x)
+    assert np.abs(n - y).max() < 1e-13 * abs(y)
+    slip = run(a.real)  # the pump's real part only (a real reservoir's rule)
+    assert abs(slip[1] - y) > 1e-3 * abs(y) and abs(slip[0] - x) > 1e-6 * abs(x)
