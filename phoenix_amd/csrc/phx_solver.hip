@@ -62,6 +62,9 @@ struct phx_solver {
     size_t rsize = 8; /* sizeof(real) */
     hipStream_t own_stream = nullptr, stream = nullptr;
     bool initialized = false;
+    /* the kernels' physics constants, derived once per solver (they do not change after phx_create) */
+    PhxPhys<float> ph_f{};
+    PhxPhys<double> ph_d{};
     /* device fields: [component][buffer] */
     void* psi[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     void* n[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
@@ -170,8 +173,18 @@ int env_field(phx_solver* s, const phx::Envelope& e, uint32_t pol, std::vector<s
 }
 
 template <typename real>
+const PhxPhys<real>& phys_of(const phx_solver* s) {
+    if constexpr (sizeof(real) == 4)
+        return s->ph_f;
+    else
+        return s->ph_d;
+}
+
+template <typename real>
 int initialize_t(phx_solver* s) {
     auto& c = s->cfg;
+    s->ph_f = phx::make_phys<float>(c);
+    s->ph_d = phx::make_phys<double>(c);
     const size_t N = s->cells();
     const int ncomp = c.tetm ? 2 : 1;
     int n_it = 0;
@@ -396,7 +409,7 @@ int rk4_launch(phx_solver* s, double t_pre, double dt) {
     stage_amps<real>(c.pump, t_pre, dt, w, 4, a.pamp);
     stage_amps<real>(c.potential, t_pre, dt, w, 4, a.vamp_h);
     for (int i = 0; i < 4; i++) a.vamp_h[i] = (real)((double)a.vamp_h[i] * c.one_over_h_bar_s);
-    a.ph = phx::make_phys<real>(c);
+    a.ph = phys_of<real>(s);
     PHX_HIP(phx::launch_rk4_scalar<real>(a, s->tune, s->stream));
     s->cur = nxt;
     return PHX_OK;
@@ -455,7 +468,7 @@ int generic_launch(phx_solver* s, phx::RkTableau tab, double t_k, double dt) {
     a.noise = (const v2*)s->noise;
     a.partials = s->partials;
     a.sums = s->sums;
-    a.ph = phx::make_phys<real>(c);
+    a.ph = phys_of<real>(s);
     PHX_HIP(phx::launch_rk_generic<real>(a, tab, s->stream));
     return PHX_OK;
 }
@@ -508,7 +521,7 @@ int staged_attempt(phx_solver* s, phx::RkTableau tab_id, double t_k, double dt) 
     a.dtr = imag ? real(0) : (real)dt;
     a.dti = imag ? -(real)dt : real(0);
     a.partials = s->st_partials;
-    a.ph = phx::make_phys<real>(c);
+    a.ph = phys_of<real>(s);
     /* temporal amplitudes of every group at t and t + dt, in the build's precision */
     std::complex<real> amp0[3][phx::kStagedMaxGroups], amp1[3][phx::kStagedMaxGroups];
     const phx::Envelope* env[3] = {&c.pump, &c.potential, &c.pulse};
@@ -677,7 +690,7 @@ int ssfm_step(phx_solver* s, double t_pre) {
     a.stochastic = c.use_stochastic;
     a.cres = c.complex_reservoir;
     a.dt = (real)s->dt;
-    a.ph = phx::make_phys<real>(c);
+    a.ph = phys_of<real>(s);
     for (int k = 0; k < ncomp; k++)
         PHX_HIP(phx::launch_fft_propagate<real>(a.psi[k], s->tw_c, s->tw_r, s->ssfm_rowf, s->ssfm_colf, c.N_c, c.N_r, s->stream));
     PHX_HIP(phx::launch_ssfm_nonlinear<real>(a, s->stream));
@@ -876,7 +889,7 @@ phx::Rk4Args<real> rk4_args(phx_solver* s, void* psi_in, void* n_in, void* psi_o
     stage_amps<real>(c.pump, s->t, s->dt, w, 4, a.pamp);
     stage_amps<real>(c.potential, s->t, s->dt, w, 4, a.vamp_h);
     for (int i = 0; i < 4; i++) a.vamp_h[i] = (real)((double)a.vamp_h[i] * c.one_over_h_bar_s);
-    a.ph = phx::make_phys<real>(c);
+    a.ph = phys_of<real>(s);
     return a;
 }
 
