@@ -48,7 +48,8 @@ hipError_t launch_reduce_partials(const double* partials, int nwaves, double* su
    branches: halo lanes, lanes and rows outside a zero-BC domain load through out-of-range
    offsets (0) and store through them (dropped); rows past the last grid row skip their store
    with a wave-uniform branch. */
-template <typename real, typename TAB, int TY, bool TETM, bool RES, bool STOCH>
+/* POT = false: no potential (the potential's loads and descriptors compiled out; a.pot must be null) */
+template <typename real, typename TAB, int TY, bool TETM, bool RES, bool STOCH, bool POT = true>
 __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
     using v2 = typename Vec2<real>::type;
     constexpr int S = TAB::S;
@@ -124,7 +125,7 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
             p0i[k][r] = v.y;
             n0[k][r] = RES ? load_real_s<real>(d_n[k], vr, 0u) : real(0);
             pp[k][r] = load_real_s<real>(d_p[k], vr, 0u);
-            vv[k][r] = load_real_s<real>(d_v[k], vr, 0u);
+            vv[k][r] = POT ? load_real_s<real>(d_v[k], vr, 0u) : real(0);
         }
         if (STOCH) {
             /* this step's noise field (launch_noise_field); 0 outside a zero-BC domain */
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
         for (int k = 0; k < NC; k++) {
             const real nv = RES ? in_n(k, s, e) : real(0);
             const real pmp = pp[k][e] * pamp;
-            const real poth = vv[k][e] * vamp_h;
+            const real poth = POT ? vv[k][e] * vamp_h : real(0);
             real kr, ki, kn;
             if constexpr (TETM) {
                 /* cross term of the other component o (kernel_gp_compute.cuh:123-130):
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(256) void rk_generic_fused(RkGenArgs<real> a) {
 }
 
 /* ---------------------------------------------------------------- launch */
-template <typename real, typename TAB, int TY, bool TETM, bool RES, bool STOCH>
+template <typename real, typename TAB, int TY, bool TETM, bool RES, bool STOCH, bool POT = true>
 static hipError_t launch_gen(const RkGenArgs<real>& a0, hipStream_t s) {
     RkGenArgs<real> a = a0;
     /* whole-array descriptors: every field must stay below the 32-bit offset range */
@@ -330,7 +331,7 @@ static hipError_t launch_gen(const RkGenArgs<real>& a0, hipStream_t s) {
     a.nstrips = (a.Nc + outw - 1) / outw;
     a.nbands = (a.Nr + TY - 1) / TY;
     const int bandgroups = (a.nbands + kWavesPerBlock - 1) / kWavesPerBlock;
-    hipLaunchKernelGGL((rk_generic_fused<real, TAB, TY, TETM, RES, STOCH>), dim3(a.nstrips * bandgroups),
+    hipLaunchKernelGGL((rk_generic_fused<real, TAB, TY, TETM, RES, STOCH, POT>), dim3(a.nstrips * bandgroups),
                        dim3(kWave * kWavesPerBlock), 0, s, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || !TAB::ERR) return e;

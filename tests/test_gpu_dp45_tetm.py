@@ -161,22 +161,30 @@ def test_spinor_adaptive_stochastic(prec):
     assert e <= (1e-11 if prec == "fp64" else 1e-4)
 
 
-def test_spinor_tile_heights_agree():
+@pytest.mark.parametrize("pot", [False, True])
+def test_spinor_tile_heights_agree(pot):
     """The FP32 TE-TM adaptive kernel runs 16- or 32-row tiles (PHX_GEN_TY, read at every launch):
     the same accepted and rejected counts, results equal to rounding (rel L2 <= 1e-6; the FP64 error sum
-    is reduced over a different grouping of per-wave partials, so not bitwise)."""
+    is reduced over a different grouping of per-wave partials, so not bitwise).  The 32-row tiles come
+    with and without the potential compiled in; both forms are checked against the 16-row kernel."""
     import os
     N, L, dt0, tmax, seed = 150, 74.5, 0.04, 0.6, 9
     psi, n, Pp, psim, nm, Pm = inputs(N, L, 41, tetm=True)
     fields = {"psi_plus": psi, "n_plus": n, "pump_plus": Pp, "psi_minus": psim, "n_minus": nm, "pump_minus": Pm}
     args = ["-tetm", "--iterator", "DP45", "-adaptive", "--tol", 1e-6, "--rkvdt", 1e-4, 0.1, "--tstep", dt0,
             "--tmax", tmax, "--dw", 0.05, "--initRandom", 0.0, seed]
+    if pot:
+        dx = L / (N - 1)
+        X, Y = np.meshgrid(-L / 2 + dx * np.arange(N), -L / 2 + dx * np.arange(N))
+        V = 2e-3 * np.exp(-0.5 * ((X - 3) ** 2 + (Y + 2) ** 2) / 6.0 ** 2)
+        fields.update(potential_plus=V, potential_minus=V[::-1].copy())
+        args += ["--potential", 1.0, "add", 10, 10, 0, 0, "both", 1, "none", "gauss+noDivide"]
     res = {}
     old = os.environ.get("PHX_GEN_TY")
     try:
         for ty in (16, 32):
             os.environ["PHX_GEN_TY"] = str(ty)
-            say(f"[spinor tile height] TY={ty}")
+            say(f"[spinor tile height] TY={ty} potential={pot}")
             out, done, info = gpu("fp32", N, L, args, fields, 100000)
             res[ty] = (out, done, info["rejected"])
     finally:
