@@ -46,7 +46,10 @@ constexpr int kOutW = kWave - 2 * kHalo; /* 56 */
    both east/west moves folded into DPP adds.  ISO (dx == dy, r == 1 exactly) drops
    the multiply by r; the two forms give identical bits for dx == dy, so a grid gets
    the same result from either kernel (an ensemble mixing both uses the general one). */
-template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool PERY, int CP = 0, int CPS = CP>
+/* CAMP: the pump's temporal amplitude is 1 at every stage (a constant envelope), so P enters as loaded and
+   the multiply by the amplitude drops out (x * 1 is exact: the same bits as the general form). */
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool PERY, int CP = 0, int CPS = CP,
+          bool CAMP = false>
 __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int band) {
     using v2 = typename Vec2<real>::type;
     const int lane = threadIdx.x & (kWave - 1);
@@ -136,7 +139,7 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
             const int e = it - 1;
             if (e >= 1 && e <= E - 2) {
                 real kr, ki, kn;
-                rhs(p0r[e], p0i[e], p0r[e - 1] + p0r[e + 1], p0i[e - 1] + p0i[e + 1], n0[e], pp[e] * a.pamp[0],
+                rhs(p0r[e], p0i[e], p0r[e - 1] + p0r[e + 1], p0i[e - 1] + p0i[e + 1], n0[e], (CAMP ? pp[e] : pp[e] * a.pamp[0]),
                     vv[e] * a.vamp_h[0], kr, ki, kn);
                 y1r[e] = phx_fma(dth, kr, p0r[e]);
                 y1i[e] = phx_fma(dth, ki, p0i[e]);
@@ -154,7 +157,7 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
             if (e >= 2 && e <= E - 3) {
                 real kr, ki, kn;
                 rhs(y1r[e], y1i[e], y1r[e - 1] + y1r[e + 1], y1i[e - 1] + y1i[e + 1], RES ? yn1[e] : real(0),
-                    pp[e] * a.pamp[1], vv[e] * a.vamp_h[1], kr, ki, kn);
+                    (CAMP ? pp[e] : pp[e] * a.pamp[1]), vv[e] * a.vamp_h[1], kr, ki, kn);
                 y2r[e] = phx_fma(dth, kr, p0r[e]);
                 y2i[e] = phx_fma(dth, ki, p0i[e]);
                 acr[e] = phx_fma(dt3, kr, acr[e]);
@@ -171,7 +174,7 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
             if (e >= 3 && e <= E - 4) {
                 real kr, ki, kn;
                 rhs(y2r[e], y2i[e], y2r[e - 1] + y2r[e + 1], y2i[e - 1] + y2i[e + 1], RES ? yn2[e] : real(0),
-                    pp[e] * a.pamp[2], vv[e] * a.vamp_h[2], kr, ki, kn);
+                    (CAMP ? pp[e] : pp[e] * a.pamp[2]), vv[e] * a.vamp_h[2], kr, ki, kn);
                 y3r[e] = phx_fma(dt1, kr, p0r[e]);
                 y3i[e] = phx_fma(dt1, ki, p0i[e]);
                 acr[e] = phx_fma(dt3, kr, acr[e]);
@@ -188,7 +191,7 @@ __device__ __forceinline__ void rk4_tile(const Rk4Args<real>& a, int strip, int 
             if (e >= 4 && e <= E - 5) {
                 real kr, ki, kn;
                 rhs(y3r[e], y3i[e], y3r[e - 1] + y3r[e + 1], y3i[e - 1] + y3i[e + 1], RES ? yn3[e] : real(0),
-                    pp[e] * a.pamp[3], vv[e] * a.vamp_h[3], kr, ki, kn);
+                    (CAMP ? pp[e] : pp[e] * a.pamp[3]), vv[e] * a.vamp_h[3], kr, ki, kn);
                 /* rows past the last grid row store through an empty descriptor: no branch */
                 const int grow = row0 + rdir * e;
                 const bool row_ok = grow < Nr;
@@ -225,7 +228,7 @@ __global__ __launch_bounds__(256) void rk4_scalar_fused(Rk4Args<real> a) {
    time out too, so every wave exits); the state is then undefined.  Tiles of the last
    band and strip must be at least kHalo cells deep/wide, so that a halo never reaches
    past the neighbouring tile (rk4_flow_shape_ok). */
-template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool PERY>
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool PERY, bool CAMP = false>
 __global__ __launch_bounds__(256) void rk4_scalar_flow(Rk4Args<real> a, Rk4Flow f) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -278,7 +281,7 @@ __global__ __launch_bounds__(256) void rk4_scalar_flow(Rk4Args<real> a, Rk4Flow 
         s.psi_out = even ? (typename Vec2<real>::type*)a.psi_out : (typename Vec2<real>::type*)a.psi_in;
         s.n_in = even ? a.n_in : a.n_out;
         s.n_out = even ? a.n_out : (real*)a.n_in;
-        rk4_tile<real, TY, ISO, PF, RES, PUMP, POT, PERY, kDeviceScope>(s, strip, band);
+        rk4_tile<real, TY, ISO, PF, RES, PUMP, POT, PERY, kDeviceScope, kDeviceScope, CAMP>(s, strip, band);
         /* all of this step's stores have completed before the counter moves */
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         if (lane == 0) __hip_atomic_store(mine, f.base + k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -392,14 +395,14 @@ inline bool rk4_flow_shape_ok(int Nc, int Nr, int ty) {
     return Nr >= ty && Nc >= kOutW && last_rows >= kHalo && last_cols >= kHalo;
 }
 
-template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT>
+template <typename real, int TY, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool CAMP = false>
 inline hipError_t launch_flow_ty(const Rk4Args<real>& a0, const Rk4Flow& f, hipStream_t s) {
     Rk4Args<real> a = a0;
     a.nstrips = (a.Nc + kOutW - 1) / kOutW;
     a.nbands = (a.Nr + TY - 1) / TY;
     const int nblocks = a.nstrips * ((a.nbands + kWavesPerBlock - 1) / kWavesPerBlock);
-    const void* fn = a.periodic_y ? (const void*)&rk4_scalar_flow<real, TY, ISO, PF, RES, PUMP, POT, true>
-                                  : (const void*)&rk4_scalar_flow<real, TY, ISO, PF, RES, PUMP, POT, false>;
+    const void* fn = a.periodic_y ? (const void*)&rk4_scalar_flow<real, TY, ISO, PF, RES, PUMP, POT, true, CAMP>
+                                  : (const void*)&rk4_scalar_flow<real, TY, ISO, PF, RES, PUMP, POT, false, CAMP>;
     /* every wave must be resident at once: the cooperative launch refuses a grid that is not */
     int per_cu = 0, dev = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kWave * kWavesPerBlock, 0) != hipSuccess ||
@@ -412,14 +415,14 @@ inline hipError_t launch_flow_ty(const Rk4Args<real>& a0, const Rk4Flow& f, hipS
     return hipLaunchCooperativeKernel(fn, dim3(nblocks), dim3(kWave * kWavesPerBlock), args, 0, s);
 }
 
-template <typename real, bool ISO, int PF, bool RES, bool PUMP, bool POT>
+template <typename real, bool ISO, int PF, bool RES, bool PUMP, bool POT, bool CAMP = false>
 inline hipError_t launch_flow_tys(const Rk4Args<real>& a, const Rk4Tune& t, const Rk4Flow& f, hipStream_t s) {
     switch (t.ty) {
-        case 8: return launch_flow_ty<real, 8, ISO, PF, RES, PUMP, POT>(a, f, s);
-        case 10: return launch_flow_ty<real, 10, ISO, PF, RES, PUMP, POT>(a, f, s);
-        case 12: return launch_flow_ty<real, 12, ISO, PF, RES, PUMP, POT>(a, f, s);
-        case 20: return launch_flow_ty<real, 20, ISO, PF, RES, PUMP, POT>(a, f, s);
-        case 16: return launch_flow_ty<real, 16, ISO, PF, RES, PUMP, POT>(a, f, s);
+        case 8: return launch_flow_ty<real, 8, ISO, PF, RES, PUMP, POT, CAMP>(a, f, s);
+        case 10: return launch_flow_ty<real, 10, ISO, PF, RES, PUMP, POT, CAMP>(a, f, s);
+        case 12: return launch_flow_ty<real, 12, ISO, PF, RES, PUMP, POT, CAMP>(a, f, s);
+        case 20: return launch_flow_ty<real, 20, ISO, PF, RES, PUMP, POT, CAMP>(a, f, s);
+        case 16: return launch_flow_ty<real, 16, ISO, PF, RES, PUMP, POT, CAMP>(a, f, s);
         default: return hipErrorNotSupported;
     }
 }
@@ -428,6 +431,11 @@ template <typename real, bool RES, bool PUMP, bool POT>
 inline hipError_t launch_flow_flags(const Rk4Args<real>& a, const Rk4Tune& t, const Rk4Flow& f, hipStream_t s) {
     const bool iso = a.ph.kin_yx == real(1);
     if constexpr (RES && PUMP && !POT) {
+        /* a pump of constant envelope (amplitude 1 at every stage) at the prefetch depths the
+           multi-step autotuner picks (4 rows FP64, 6 rows FP32) */
+        const bool camp = a.pamp[0] == real(1) && a.pamp[1] == real(1) && a.pamp[2] == real(1) && a.pamp[3] == real(1);
+        if (iso && camp && (t.pf == 4 || t.pf == 5)) return launch_flow_tys<real, true, 4, RES, PUMP, POT, true>(a, t, f, s);
+        if (iso && camp && (t.pf == 6 || t.pf == 7)) return launch_flow_tys<real, true, 6, RES, PUMP, POT, true>(a, t, f, s);
         if (iso && t.pf == 3) return launch_flow_tys<real, true, 3, RES, PUMP, POT>(a, t, f, s);
         if (iso && (t.pf == 4 || t.pf == 5)) return launch_flow_tys<real, true, 4, RES, PUMP, POT>(a, t, f, s);
         if (iso && (t.pf == 6 || t.pf == 7)) return launch_flow_tys<real, true, 6, RES, PUMP, POT>(a, t, f, s);
